@@ -1,0 +1,214 @@
+"""Headline benchmark: 3D M'4 remesh cycle (P2M + M2P) on one MI355X.
+
+Workload (the metric's configuration): periodic unit cube, 512^3 cells,
+134,217,728 particles (one per cell, jittered +-0.3h, advected at CFL 0.5 by
+an 8-mode divergence-free field), 3 vorticity components.  One step = one
+remesh cycle: P2M of the advected particles (binning included) into the
+512^3 node field, then M2P onto a fresh jittered lattice.  Inputs are
+generated on the device and resident in HBM before the timed region.
+
+`value` is the plain FP64 form (the reference's double specifics); the
+FP32-compensated form is timed in the same run and reported beside it with
+its max / rms error against the FP64 GPU result.  The CPU baseline is the
+oracle (this repo's C restatement of the reference, OpenMP slab
+decomposition over the host cores) on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512, help="cells per axis")
+    ap.add_argument("--lda", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-f32c", action="store_true", help="skip the FP32-compensated leg")
+    ap.add_argument("--cpu-n", type=int, default=256, help="cells per axis of the CPU sample")
+    return ap.parse_args()
+
+
+def make_inputs(core, wl, N, lda, seed, torch):
+    """Advected jittered lattice + vorticity, and a fresh lattice for M2P (device)."""
+    mesh = core.Mesh(3, N + 1)
+    x = core.gen_jittered_lattice(N, seed)
+    modes = wl.fourier_modes(seed)
+    v = core.gen_divfree_velocity(x, modes)
+    umax = torch.sqrt((v * v).sum(1)).max().item()
+    dt = 0.5 * (1.0 / N) / umax
+    x.add_(v, alpha=dt)
+    del v
+    core.impose_part_bc(mesh, x)
+    wmodes = wl.fourier_modes(seed + 1000)
+    w = core.gen_divfree_velocity(x, wmodes)                    # vorticity from another 8-mode field
+    w /= w.abs().max()
+    xfresh = core.gen_jittered_lattice(N, seed + 1)
+    return mesh, x, w.contiguous(), xfresh, dt
+
+
+def run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan):
+    core.p2m(mesh, x, up, field, [0, 0, 0], 2, prec, p_plan)
+    core.m2p(mesh, xfresh, field, [0, 0, 0], out, 2, prec, m_plan)
+
+
+def time_form(core, torch, mesh, x, up, xfresh, prec, steps, warmup):
+    N = mesh.cells[0]
+    lda = up.shape[1]
+    vdt = core.VAL_DTYPE[prec]
+    field = torch.empty(mesh.compact_shape(lda), dtype=vdt, device="cuda")
+    out = torch.empty((xfresh.shape[0], lda), dtype=vdt, device="cuda")
+    p_plan, m_plan = core.Plan(), core.Plan()
+    p_plan.set_timing(True)
+    m_plan.set_timing(True)
+    for _ in range(warmup):
+        run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan)
+    torch.cuda.synchronize()
+    phases = {"bin": [], "weights": [], "deposit": [], "m2p": []}
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan)
+        tp = p_plan.timings()
+        tm = m_plan.timings()
+        phases["bin"].append(tp[0])
+        phases["weights"].append(tp[1])
+        phases["deposit"].append(tp[2])
+        phases["m2p"].append(tm[0])
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / steps
+    avg = {k: sum(v) / len(v) for k, v in phases.items()}
+    return ms, wall, avg, field, out
+
+
+def cpu_baseline(oracle_mod, core, wl, torch, n, lda, seed):
+    """Oracle P2M + M2P (OpenMP, all host cores of this process) on an n^3 sample."""
+    ncores = len(os.sched_getaffinity(0))
+    nthreads = min(ncores, 16)
+    mesh, x, w, xfresh, _ = make_inputs(core, wl, n, lda, seed, torch)
+    xh = x.cpu().numpy()
+    uh = w.cpu().numpy()
+    xf = xfresh.cpu().numpy()
+    del x, w, xfresh
+    t = oracle_mod.topo(3, [n + 1] * 3)
+    g = [1, 1, 1]
+    reps, times = 0, []
+    t_start = time.perf_counter()
+    while reps < 2 or (reps < 5 and time.perf_counter() - t_start < 20.0):
+        t0 = time.perf_counter()
+        info, f = oracle_mod.p2m_par(t, 2, xh, uh, g, nthreads)
+        info2, back = oracle_mod.m2p_par(t, 2, xf, f, g, nthreads)
+        times.append(time.perf_counter() - t0)
+        assert info == 0 and info2 == 0
+        reps += 1
+    best = sorted(times)[len(times) // 2]
+    npart = n ** 3
+    return {"value": npart / best, "unit": "particles/s", "cores": nthreads, "kind": "port",
+            "sample": f"{n}^3 cells, {npart} particles, lda={lda}, one P2M+M2P remesh cycle (FP64), "
+                      f"median of {reps} cycles, {best:.3f} s/cycle"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+        sys.exit(2)
+    import torch
+    from ppm_legacy_amd import core, workloads as wl
+
+    torch.cuda.set_device(0)
+    N, lda = args.n, args.lda
+    np_ = N ** 3
+    mesh, x, w64, xfresh, dt = make_inputs(core, wl, N, lda, args.seed, torch)
+    torch.cuda.synchronize()
+    print(f"# inputs ready: {N}^3 cells, {np_} particles, lda={lda}, dt={dt:.3e}", file=sys.stderr, flush=True)
+
+    ms64, wall64, ph64, f64, o64 = time_form(core, torch, mesh, x, w64, xfresh, core.PREC_F64, args.steps,
+                                             args.warmup)
+    print(f"# fp64 cycle {ms64:.3f} ms  phases {ph64}", file=sys.stderr, flush=True)
+    # algorithmic bytes (positions FP64 in both forms): per pass Np*(8d + s*lda) + Nn*s*lda
+    nn = N ** 3
+    b64_pass = np_ * (24 + 8 * lda) + nn * 8 * lda
+    res = {}
+    if not args.no_f32c:
+        w32 = w64.float()
+        ms32, wall32, ph32, f32, o32 = time_form(core, torch, mesh, x, w32, xfresh, core.PREC_F32C, args.steps,
+                                                 args.warmup)
+        fmax = f64.abs().max().item()
+        omax = o64.abs().max().item()
+        df = (f32.double() - f64)
+        do = (o32.double() - o64)
+        b32_pass = np_ * (24 + 4 * lda) + nn * 4 * lda
+        res = {
+            "particles_per_s": np_ / (ms32 * 1e-3),
+            "ms_per_step": ms32,
+            "speedup_vs_fp64": ms64 / ms32,
+            "cycle_hbm_frac": 2 * b32_pass / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "field_err_max_rel": df.abs().max().item() / fmax,
+            "field_err_rms_rel": df.pow(2).mean().sqrt().item() / fmax,
+            "m2p_err_max_rel": do.abs().max().item() / omax,
+            "m2p_err_rms_rel": do.pow(2).mean().sqrt().item() / omax,
+            "phases_ms": ph32,
+        }
+        del f32, o32, w32, df, do
+        print(f"# fp32c cycle {ms32:.3f} ms  phases {ph32}", file=sys.stderr, flush=True)
+
+    # dominant kernel of the FP64 cycle, priced with the algorithmic bytes of its pass
+    dom = max(("deposit", "m2p"), key=lambda k: ph64[k])
+    achieved = b64_pass / (ph64[dom] * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        orc.lib()
+        orc._setup()
+        del f64, o64
+        torch.cuda.empty_cache()
+        cpu = cpu_baseline(orc, core, wl, torch, args.cpu_n, lda, args.seed)
+    out = {
+        "metric": "particles/s 3D M'4 remesh (P2M+M2P) @512^3; % HBM peak; FP32c vs FP64 GPU",
+        "value": np_ / (ms64 * 1e-3),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox4x32-10 jittered lattice advected by an 8-mode divergence-free field)",
+        "config": {"workload": f"3D M'4 remesh cycle (P2M+M2P), periodic {N}^3 cells, {np_} particles, "
+                               f"lda={lda}, CFL 0.5 advection, M2P onto a fresh jittered lattice",
+                   "cells": N, "particles": np_, "lda": lda, "kernel": "M'4", "seed": args.seed},
+        "cycle_hbm_frac": 2 * b64_pass / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "phases_ms": ph64,
+        "roofline": {"bound": "hbm", "kernel": "k_p2m_gather" if dom == "deposit" else "k_m2p_gather",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "fp32c": res or None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,230 @@
+/*
+ * ppmg.h -- C ABI of the MI355X particle<->mesh interpolation library
+ * (libppmg.so, HIP for gfx950).  Drop-in replacement for PPM's
+ * ppm_interp_p2m / ppm_interp_m2p / ppm_rmsh_* / ppm_util_rank3d path.
+ *
+ * Two layers:
+ *  1. Core (re-entrant, device pointers, explicit stream and plan):
+ *     ppmg_p2m, ppmg_m2p, ppmg_p2m_adjoint, ppmg_util_rank, ...
+ *  2. PPM-compatible registry (topo_id / mesh_id global state, host or
+ *     device pointers, the reference's argument lists): ppmg_init,
+ *     ppmg_mktopo_field, ppmg_interp_p2m, ppmg_interp_m2p, ppmg_rmsh_*.
+ *     The Fortran modules in ppm_legacy_amd/fortran bind these through
+ *     ISO_C_BINDING.
+ *
+ * Array layouts are the reference's Fortran (column-major) layouts:
+ *   xp(dim, Np)        coordinate fastest
+ *   up(lda, Np)        component fastest
+ *   field_up(lda, i, j[, k])   component fastest, then x, y, z.
+ * No C++ exception crosses this ABI and the library never aborts: every
+ * call returns an info value (0 = success, negative = PPM severity,
+ * ppm_param.h:372-375) and records the ppm_err_* code and a message,
+ * readable with ppmg_last_error().
+ *
+ * Reference paths below are relative to the reference tree's ppm/src/.
+ */
+#ifndef PPMG_H
+#define PPMG_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* info severities (ppm_param.h:372-375) */
+#define PPMG_OK              0
+#define PPMG_ERROR_FATAL    -1
+#define PPMG_ERROR_ERROR    -2
+#define PPMG_ERROR_WARNING  -3
+#define PPMG_ERROR_NOTICE   -4
+
+/* error codes (ppm_error.h:161-299), plus device-side codes of this library */
+#define PPMG_ERR_ALLOC        101
+#define PPMG_ERR_PART_RANGE   402
+#define PPMG_ERR_PART_UNASS   405
+#define PPMG_ERR_INDEX_CORR   410
+#define PPMG_ERR_ARGUMENT     503
+#define PPMG_ERR_NOINIT       903
+#define PPMG_ERR_DEVICE       9100   /* a HIP runtime call failed */
+#define PPMG_ERR_FIELD_BOUNDS 9101   /* a stencil node lies outside the stored field */
+
+/* interpolation kernels (ppm_param.h:405-406) */
+#define PPMG_KERNEL_BSP2 1
+#define PPMG_KERNEL_MP4  2
+
+/* precision forms */
+#define PPMG_PREC_F64  0   /* double xp, double values: the reference's *_d* specifics */
+#define PPMG_PREC_F32C 1   /* double xp, cell index and offset in FP64, float weights and
+                              values, Kahan-compensated per-node / per-particle sums */
+#define PPMG_PREC_F32  2   /* float xp and values: the reference's *_s* specifics
+                              (sums compensated as in F32C) */
+
+/* memory space of caller arrays for the registry layer */
+#define PPMG_MEM_DEVICE 0
+#define PPMG_MEM_HOST   1
+
+/* Mesh of one sub covering the whole domain (what ppm_topo_mkfield builds on
+ * a single process): nm = PPM's Nm (cells + 1, both boundary nodes),
+ * node i (0-based) at min_phys + i*h, h = (max_phys-min_phys)/(nm-1).
+ * bcdef per face x-,x+,y-,y+,z-,z+ with PPM's values (1 = periodic,
+ * ppm_param.h:333-338). */
+typedef struct ppmg_mesh {
+    int    dim;
+    int    nm[3];
+    double min_phys[3];
+    double max_phys[3];
+    int    bcdef[6];
+} ppmg_mesh;
+
+/* A stored box of nodes: node (i,j,k) (0-based, may be negative or >= nm-1
+ * for ghost / periodic-image nodes) sits at
+ *   data[c + lda*((i-lo[0]) + ext[0]*((j-lo[1]) + ext[1]*(k-lo[2])))].
+ * PPM's field_up(lda, 1-g:Nm+g, ...) is lo = -g, ext = nm + 2g.
+ * The compact periodic layout (unique nodes only) is lo = 0, ext = nm - 1.
+ * Element type is double for PPMG_PREC_F64, float otherwise. */
+typedef struct ppmg_field {
+    void* data;
+    int   lda;
+    int   lo[3];
+    int   ext[3];
+} ppmg_field;
+
+typedef struct ppmg_plan ppmg_plan;   /* device scratch + the forward sort (opaque) */
+
+/* ------------------------------------------------------------ plans -- */
+int ppmg_plan_create(ppmg_plan** plan);
+int ppmg_plan_destroy(ppmg_plan* plan);
+
+/* ------------------------------------------------------------- core -- */
+/* P2M: replaces ppm_interp_p2m_{s,d}{s,v}_{2d,3d} (ppm_interp_p2m.f:127-162,
+ * generic ppm_module_interp_p2m.f:57-70) including its ghost fold
+ * (ppm_map_field_ghost init/ghost_put/push/send/pop, ppm_interp_p2m.f:3670-3716).
+ * Stored nodes real in every axis receive the folded sum; nodes outside the
+ * real range in some axis receive the raw (unfolded) scatter, as the
+ * reference leaves them.  Particles are binned by a stable counting sort whose
+ * permutation and cell keys stay in the plan for ppmg_p2m_adjoint. */
+int ppmg_p2m(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void* xp, int lda,
+             const void* up, const ppmg_field* field, ppmg_plan* plan, void* stream);
+
+/* M2P: replaces ppm_interp_m2p_{s,d}{s,v}_{2d,3d} (ppm_interp_m2p.f:126-161,
+ * generic ppm_module_interp_m2p.f:46-59) preceded by the caller's ghost_get
+ * (ppm_map_field_ghost.f:627-669): periodic images are read in-kernel with
+ * ghost_get's node mapping; non-periodic ghost nodes are read as stored.
+ * up is overwritten. */
+int ppmg_m2p(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void* xp, int lda,
+             const ppmg_field* field, void* up, ppmg_plan* plan, void* stream);
+
+/* Adjoint of ppmg_p2m (no reference counterpart).  g = dL/dm on the stored
+ * nodes of a layout that stores every unique node once (compact periodic, or
+ * any box on non-periodic axes).  grad_up = dL/du (np, lda), grad_xp = dL/dx
+ * (np, dim); either may be NULL.  Uses the permutation and cell keys that the
+ * last ppmg_p2m on this plan left (same mesh, kernel, np and xp required). */
+int ppmg_p2m_adjoint(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void* xp,
+                     int lda, const void* up, const ppmg_field* g, void* grad_up, void* grad_xp,
+                     ppmg_plan* plan, void* stream);
+
+/* Stable cell ranking: replaces ppm_util_rank3d_{s,d} / ppm_util_rank2d_{s,d}
+ * (ppm_util_rank3d.f:147-599, ppm_util_rank2d.f:143-560).  ncells = Nm of the
+ * reference (cells per axis), ngl = Ngl(1:2*dim).  lpdx (capacity np) gets the
+ * 1-based particle indices in cell order, lhbx (prod(ncells+ngl_lo+ngl_hi)+1)
+ * the 1-based cell heads; *icount = particles inside the mesh.  prec selects
+ * float (PPMG_PREC_F32) or double positions and bounds. */
+int ppmg_util_rank(int dim, int prec, int64_t np, const void* xp, const void* xmin,
+                   const void* xmax, const int* ncells, const int* ngl, int* lpdx, int* lhbx,
+                   int64_t* icount, ppmg_plan* plan, void* stream);
+
+/* ppm_impose_part_bc (ppm_impose_part_bc.f:67-214): periodic wrap in place. */
+int ppmg_impose_part_bc(const ppmg_mesh* mesh, int prec, int64_t np, void* xp, void* stream);
+
+/* ppm_rmsh_comp_weights (ppm_rmsh_comp_weights.f:96-104): wx{1,2,3} are
+ * (2*ks, np), ks = 2 (M'4) or 1 (BSP2); wx3 ignored in 2D. */
+int ppmg_rmsh_comp_weights(const ppmg_mesh* mesh, int kernel, int prec, int64_t np,
+                           const void* xp, void* wx1, void* wx2, void* wx3, void* stream);
+
+/* ppm_rmsh_remesh (ppm_rmsh_remesh.f:88-124): scatter with precomputed
+ * weights plus the ghost fold, into field (zeroed first). */
+int ppmg_rmsh_remesh(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void* xp,
+                     int lda, const void* up, const void* wx1, const void* wx2, const void* wx3,
+                     const ppmg_field* field, ppmg_plan* plan, void* stream);
+
+/* ppm_rmsh_create_part (ppm_rmsh_create_part.f:90-171): nodes 1..nnx,... whose
+ * strength |u| (lda 1) or ||u*w||_2 lies in (cutoff[0], cutoff[1]) become
+ * particles in node order; resetpos puts them on the nodes.  cutoff_weights
+ * may be NULL.  At most maxnp are written; *np_out is the full count. */
+int ppmg_rmsh_create_part(const ppmg_mesh* mesh, int prec, const ppmg_field* field,
+                          const double* cutoff, const double* cutoff_weights, int resetpos,
+                          int64_t maxnp, int64_t* np_out, void* xp, void* up, ppmg_plan* plan,
+                          void* stream);
+
+/* Phase timing with HIP events on the call's stream (off by default; the
+ * PPMG_TIMING environment variable switches it on for new plans).
+ * ppmg_plan_timings waits for the last call on the plan and returns up to n
+ * phase durations in ms: P2M / remesh = {binning, weights, deposit},
+ * M2P = {gather}, adjoint = {gather}; it returns the number of phases. */
+int ppmg_plan_set_timing(ppmg_plan* plan, int on);
+int ppmg_plan_timings(ppmg_plan* plan, double* ms, int n);
+
+/* Device pointers to the binning the last ppmg_p2m / ppmg_rmsh_remesh left in
+ * the plan: key[np] (cell of each particle, x fastest: i + nbin_x*(j + nbin_y*k),
+ * nbin = Nm-1 on periodic axes, Nm otherwise), perm[np] (particle indices in
+ * stable cell order, 0-based) and start[ncells+1] (cell heads, 0-based).
+ * The ppm_util_rank3d analogue of these is ppmg_util_rank (1-based). */
+int ppmg_plan_sort_view(const ppmg_plan* plan, const int** key, const int** perm, const int** start,
+                        int64_t* np, int64_t* ncells);
+
+/* ------------------------------------------------------ diagnostics -- */
+int ppmg_last_error(int* code, char* msg, int msglen);
+int ppmg_set_debug(int debug);              /* ppm_debug semantics (ppm_init.f:161) */
+int ppmg_device_sync(void* stream);
+
+/* --------------------------------------------- PPM-compatible registry -- */
+/* ppm_init(dim,prec,tolexp,comm,debug,info) (ppm_init.f:161): prec is
+ * PPM's ppm_kind_double/single, here PPMG_PREC_F64 / PPMG_PREC_F32. */
+int ppmg_init(int dim, int prec, int tolexp, int debug);
+int ppmg_finalize(void);
+/* Selects the arithmetic form used by the double-precision registry entry
+ * points (PPMG_PREC_F64 or PPMG_PREC_F32C); the reference signatures stay
+ * unchanged. */
+int ppmg_set_precision_mode(int prec);
+/* Single-sub field topology and mesh: what ppm_topo_mkfield_d
+ * (ppm_topo_mkfield.f:316-319) builds on one process. */
+int ppmg_mktopo_field(int dim, const double* min_phys, const double* max_phys, const int* bcdef,
+                      const int* nm, const int* ghostsize, int* topo_id, int* mesh_id);
+int ppmg_topo_destroy(int topo_id);
+
+/* ppm_interp_p2m(xp,Np,up,[lda,]topo_id,mesh_id,kernel,ghostsize,field_up,info)
+ * (ppm_interp_p2m.f:130-160).  prec: PPMG_PREC_F64 (d specifics, honours the
+ * precision mode) or PPMG_PREC_F32 (s specifics).  field_up is the full
+ * padded array with lower bound 1-ghostsize. */
+int ppmg_interp_p2m(const void* xp, int np, const void* up, int lda, int topo_id, int mesh_id,
+                    int kernel, const int* ghostsize, void* field_up, int prec, int mem);
+/* ppm_interp_m2p(xp,Np,up,[lda,]topo_id,mesh_id,kernel,ghostsize,field_up,info)
+ * (ppm_interp_m2p.f:129-159). */
+int ppmg_interp_m2p(const void* xp, int np, void* up, int lda, int topo_id, int mesh_id,
+                    int kernel, const int* ghostsize, const void* field_up, int prec, int mem);
+/* ppm_rmsh_comp_weights(xp,Np,topo_id,mesh_id,kernel,info,wx1,wx2,wx3)
+ * (ppm_rmsh_comp_weights.f:97-103). */
+int ppmg_rmsh_comp_weights_reg(const void* xp, int np, int topo_id, int mesh_id, int kernel,
+                               void* wx1, void* wx2, void* wx3, int prec, int mem);
+/* ppm_rmsh_remesh(xp,Np,up,[lda,]topo_id,mesh_id,kernel,ghostsize,info,field_up,wx1,wx2,wx3)
+ * (ppm_rmsh_remesh.f:91-121); field_up (lower bound 1-ghostsize) is written. */
+int ppmg_rmsh_remesh_reg(const void* xp, int np, const void* up, int lda, int topo_id,
+                         int mesh_id, int kernel, const int* ghostsize, void* field_up,
+                         const void* wx1, const void* wx2, const void* wx3, int prec, int mem);
+/* ppm_util_rank3d / rank2d with host or device arrays. */
+int ppmg_util_rank_reg(int dim, const void* xp, int np, const void* xmin, const void* xmax,
+                       const int* ncells, const int* ngl, int* lpdx, int* lhbx, int* icount,
+                       int prec, int mem);
+
+/* ---------------------------------------------- synthetic workloads -- */
+/* Device generators matching ppm_legacy_amd/workloads.py (Philox4x32-10). */
+int ppmg_gen_jittered_lattice(int dim, int n, double jitter, uint64_t seed, double* xp,
+                              void* stream);
+int ppmg_gen_normals(int64_t count, int lda, uint64_t seed, int prec, void* out, void* stream);
+int ppmg_gen_divfree_velocity(int64_t np, const double* xp, int nmodes, const double* k,
+                              const double* A, const double* phi, double* u, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,173 @@
+// ppmg_internal.h -- shared device/host definitions of libppmg (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "ppmg.h"
+
+namespace ppmg {
+
+// Mesh as the kernels see it.  One sub covering the domain; node i (0-based)
+// at minv + i*h; N = cells per axis = nm - 1; nbin = bin cells per axis: N on
+// periodic axes (INT(x0) == N is the periodic image of cell 0), N + 1 on
+// non-periodic ones (x == max_phys is inside the closed upper face).
+struct DevMesh {
+    int dim;
+    int N[3];
+    int nbin[3];
+    int periodic[3];     // both faces periodic (bcdef 1)
+    int upper_open[3];   // reference's sub test: x < max required on this face
+    double minv[3];
+    double maxv[3];
+    double dxi[3];       // inverse spacing of the operation (see spacing_*)
+};
+
+struct FieldGeom {
+    int lda;
+    int lo[3];
+    int ext[3];
+    int swap[3];         // periodic axis storing both node 0 and node N: ghost_get node map
+};
+
+// Kahan-compensated float accumulator; value() = s - c.
+struct KahanF {
+    float s, c;
+    __device__ __forceinline__ void init() { s = 0.f; c = 0.f; }
+    __device__ __forceinline__ void add(float v) {
+        float y = v - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ float value() const { return s - c; }
+};
+
+struct PlainD {
+    double s;
+    __device__ __forceinline__ void init() { s = 0.0; }
+    __device__ __forceinline__ void add(double v) { s += v; }
+    __device__ __forceinline__ double value() const { return s; }
+};
+
+template <typename ValT> struct AccOf;
+template <> struct AccOf<double> { using type = PlainD; };
+template <> struct AccOf<float> { using type = KahanF; };
+
+// ---------------------------------------------------------------- weights --
+// M'4 on the nodes INT(x0)-1 .. INT(x0)+2 (0-based) in the 3D unrolled form
+// of ppm_interp_p2m.f:824-853 (same polynomials as ppm_interp_m2p.f).
+template <typename T>
+__device__ __forceinline__ void mp4_unrolled(T frac, T a[4]) {
+    T x10 = frac + T(1);
+    T x11 = x10 - T(1);
+    T x12 = x10 - T(2);
+    T x13 = x10 - T(3);
+    a[0] = T(2) + (T(-4) + (T(2.5) - T(0.5) * x10) * x10) * x10;
+    a[1] = T(1) + (T(-2.5) + T(1.5) * x11) * (x11 * x11);
+    a[2] = T(1) + (T(-2.5) - T(1.5) * x12) * (x12 * x12);
+    a[3] = T(2) + (T(4) + (T(2.5) + T(0.5) * x13) * x13) * x13;
+}
+
+// M'4 by distance branch, the 2D form of ppm_interp_p2m.f:3620-3651.
+template <typename T>
+__device__ __forceinline__ T mp4_branch(T r) {
+    if (r < T(1)) return T(1) - (r * r) * (T(2.5) - T(1.5) * r);
+    return T(2) + (T(-4) + (T(2.5) - T(0.5) * r) * r) * r;
+}
+
+// d/ds of the four M'4 weights (s = x0 - node) in the same unrolled variables.
+template <typename T>
+__device__ __forceinline__ void mp4_unrolled_deriv(T frac, T d[4]) {
+    T x10 = frac + T(1);
+    T x11 = x10 - T(1);
+    T x12 = x10 - T(2);
+    T x13 = x10 - T(3);
+    d[0] = T(-4) + (T(5) - T(1.5) * x10) * x10;
+    d[1] = (T(-5) + T(4.5) * x11) * x11;
+    d[2] = (T(-5) - T(4.5) * x12) * x12;
+    d[3] = T(4) + (T(5) + T(1.5) * x13) * x13;
+}
+
+// 1D weights of one axis.  KS = 4 (M'4) or 2 (BSP2); DIM selects the
+// reference's formula family (3D unrolled vs 2D branch form for M'4).
+template <int KS, int DIM, typename T>
+__device__ __forceinline__ void weights_1d(T frac, T a[KS]) {
+    if constexpr (KS == 4) {
+        if constexpr (DIM == 3) {
+            mp4_unrolled<T>(frac, a);
+        } else {
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                T r = frac - T(o - 1);
+                a[o] = mp4_branch<T>(r < T(0) ? -r : r);
+            }
+        }
+    } else {
+        // ppm_interp_p2m.f:3077-3104: a10 = 1 - x10, a11 = 1 + (x10 - 1)
+        a[0] = T(1) - frac;
+        a[1] = T(1) + (frac - T(1));
+    }
+}
+
+// -------------------------------------------------------------- host side --
+struct Status {
+    int info = 0;
+    int code = 0;
+    std::string msg;
+};
+
+void set_error(int severity, int code, const std::string& msg);
+void clear_error();
+int debug_level();
+
+#define PPMG_HIP(call)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ::ppmg::set_error(PPMG_ERROR_FATAL, PPMG_ERR_DEVICE,                         \
+                              std::string(#call) + ": " + hipGetErrorString(e_));        \
+            return PPMG_ERROR_FATAL;                                                     \
+        }                                                                                \
+    } while (0)
+
+// Growable device buffer owned by a plan.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);
+    void release();
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace ppmg
+
+// Opaque plan: scratch of the binning and the forward record for the adjoint.
+struct ppmg_plan {
+    ppmg::DevBuf key, slot, count, start, perm, wsorted, usorted, scan_tmp, err, misc, misc2;
+    int64_t np = 0;
+    int64_t ncells = 0;
+    int dim = 0, kernel = 0, prec = 0;
+    ppmg::DevMesh mesh{};
+    bool have_forward = false;
+    hipEvent_t ev[8] = {};
+    bool ev_init = false;
+    bool timing = false;
+    int nev = 0;                 // events recorded by the last call
+    hipStream_t ev_stream = nullptr;
+    double last_ms[8] = {};
+};
+
+namespace ppmg {
+// Phase boundary marker of a timed plan (no-op unless timing is on).
+inline void mark(ppmg_plan* plan, hipStream_t s, bool first = false) {
+    if (!plan || !plan->timing) return;
+    if (!plan->ev_init) {
+        for (auto& e : plan->ev) (void)hipEventCreate(&e);
+        plan->ev_init = true;
+    }
+    if (first) plan->nev = 0;
+    if (plan->nev < 8) (void)hipEventRecord(plan->ev[plan->nev++], s);
+    plan->ev_stream = s;
+}
+}  // namespace ppmg

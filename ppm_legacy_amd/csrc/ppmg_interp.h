@@ -1,0 +1,21 @@
+// ppmg_interp.h -- launchers of the general interpolation path (ppmg_interp.hip)
+// and of the 3D M'4 fast path (ppmg_p2m3d.hip).
+#pragma once
+#include "ppmg_internal.h"
+
+namespace ppmg {
+
+int p2m_general(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp, int lda,
+                const void* up, const FieldGeom& fg, void* field, const void* const* wtab, hipStream_t s);
+int m2p_general(const DevMesh& m, int kernel, int prec, int64_t np, const void* xp, int lda, const FieldGeom& fg,
+                const void* field, void* up, int* err, hipStream_t s);
+int adjoint_general(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp, int lda,
+                    const void* up, const FieldGeom& fg, const void* g, void* gu, void* gx, hipStream_t s);
+int comp_weights_general(const DevMesh& m, int kernel, int prec, int64_t np, const void* xp, void* w1, void* w2,
+                         void* w3, hipStream_t s);
+int impose_bc_general(const DevMesh& m, int prec, int64_t np, void* xp, hipStream_t s);
+int create_part_general(ppmg_plan* plan, const DevMesh& m, int prec, const FieldGeom& fg, const void* field,
+                        const double* cutoff, const double* cw, int resetpos, int64_t maxnp, int64_t* np_out,
+                        void* xp, void* up, hipStream_t s);
+
+}  // namespace ppmg

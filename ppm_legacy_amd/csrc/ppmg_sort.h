@@ -1,0 +1,25 @@
+// ppmg_sort.h -- counting sort / ranking entry points shared inside libppmg.
+#pragma once
+#include "ppmg_internal.h"
+
+namespace ppmg {
+
+// Geometry of ppm_util_rank3d / rank2d (ppm_util_rank3d.f:309-355).
+struct RankGeom {
+    int dim;
+    int nm[3];        // cells per axis (the reference's Nm)
+    int ngl_lo[3];
+    int ntot[3];      // nm + ngl_lo + ngl_hi (1 on unused axes)
+    double xmin[3], xmax[3];
+    double rdx[3];    // Nm / (xmax - xmin) in the position precision
+};
+
+int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, int64_t total, hipStream_t s);
+int fill_zero(void* p, size_t bytes, hipStream_t s);
+// Bins np particles into the plan (key, perm, start); *nbad = particles outside the sub.
+int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp,
+                  hipStream_t s, int64_t* nbad);
+int rank_particles(ppmg_plan* plan, const RankGeom& g, int prec, int64_t np, const void* xp,
+                   int* lpdx, int* lhbx, int64_t* icount, hipStream_t s);
+
+}  // namespace ppmg

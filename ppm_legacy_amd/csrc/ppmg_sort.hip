@@ -1,0 +1,336 @@
+// ppmg_sort.hip -- stable counting sort of particles into cells (gfx950).
+//
+// Replaces the cell ranking of ppm_util_rank3d / rank2d
+// (ppm_util_rank3d.f:309-532): count particles per cell, exclusive prefix
+// over the cells, scatter in particle order.  On the GPU the count and the
+// scatter use integer atomics, which place the particles of one cell in an
+// arbitrary order; a per-cell pass then restores increasing particle index,
+// so the permutation equals the reference's lpdx exactly.
+#include "ppmg_internal.h"
+#include "ppmg_sort.h"
+
+namespace ppmg {
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* sh, int64_t& excl) {
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wid = t >> 6;
+    // wavefront inclusive scan (64 lanes)
+    int64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wid] = x;
+    __syncthreads();
+    if (t == 0) {
+        int64_t run = 0;
+        for (int w = 0; w < SCAN_BLOCK / 64; ++w) {
+            int64_t s = sh[w];
+            sh[w] = run;
+            run += s;
+        }
+        sh[SCAN_BLOCK / 64] = run;
+    }
+    __syncthreads();
+    excl = x - v + sh[wid];
+    int64_t total = sh[SCAN_BLOCK / 64];
+    __syncthreads();
+    return total;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const int* __restrict__ in, int64_t n,
+                                                            int64_t* __restrict__ block_sums) {
+    __shared__ int64_t sh[SCAN_BLOCK / 64 + 1];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        int64_t idx = base + (int64_t)i * SCAN_BLOCK + threadIdx.x;
+        if (idx < n) s += in[idx];
+    }
+    int64_t excl;
+    int64_t tot = block_exclusive_scan(s, sh, excl);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the block sums by one workgroup, chunk by chunk.
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_sums(int64_t* __restrict__ sums, int64_t nb) {
+    __shared__ int64_t sh[SCAN_BLOCK / 64 + 1];
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nb; base += SCAN_BLOCK) {
+        int64_t idx = base + threadIdx.x;
+        int64_t v = idx < nb ? sums[idx] : 0;
+        int64_t excl;
+        int64_t tot = block_exclusive_scan(v, sh, excl);
+        if (idx < nb) sums[idx] = carry + excl;
+        carry += tot;
+    }
+}
+
+// Each thread owns SCAN_ITEMS consecutive elements of its tile.
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_down(const int* __restrict__ in, int64_t n,
+                                                          const int64_t* __restrict__ offs,
+                                                          int* __restrict__ out, int64_t total_out) {
+    __shared__ int64_t sh[SCAN_BLOCK / 64 + 1];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int vals[SCAN_ITEMS];
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        int64_t idx = base + i;
+        vals[i] = idx < n ? in[idx] : 0;
+        s += vals[i];
+    }
+    int64_t excl;
+    block_exclusive_scan(s, sh, excl);
+    int64_t run = offs[blockIdx.x] + excl;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        int64_t idx = base + i;
+        if (idx < n) out[idx] = (int)run;
+        run += vals[i];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_BLOCK - 1) out[n] = (int)total_out;
+}
+
+int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, int64_t total, hipStream_t s) {
+    int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb < 1) nb = 1;
+    if (tmp.ensure(sizeof(int64_t) * (size_t)nb)) return PPMG_ERROR_FATAL;
+    int64_t* sums = tmp.as<int64_t>();
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, sums, nb);
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums, out, total);
+    PPMG_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------ binning ---
+// Cell key per particle with the index arithmetic of ppm_interp_p2m.f:800-822
+// (x0 = (x - min)*dxi, INT truncation) and the sub test of :535-561.
+template <int DIM, typename PosT>
+__global__ void __launch_bounds__(256) k_bin(const PosT* __restrict__ xp, int64_t np, DevMesh m,
+                                             int* __restrict__ key, int* __restrict__ slot,
+                                             int* __restrict__ count, int* __restrict__ err) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const PosT* x = xp + p * DIM;
+    bool ok = true;
+    int cell[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        PosT v = x[d];
+        PosT mn = (PosT)m.minv[d];
+        PosT mx = (PosT)m.maxv[d];
+        if (!(v >= mn && v <= mx)) ok = false;
+        if (!(v < mx || !m.upper_open[d])) ok = false;
+        PosT x0 = (v - mn) * (PosT)m.dxi[d];
+        int i = ok ? (int)x0 : 0;
+        if (m.periodic[d] && i >= m.N[d]) i -= m.N[d];
+        if (i < 0 || i >= m.nbin[d]) ok = false;
+        cell[d] = i;
+    }
+    if (!ok) {
+        atomicAdd(err, 1);
+        key[p] = -1;
+        return;
+    }
+    int k = cell[0] + m.nbin[0] * (cell[1] + m.nbin[1] * cell[2]);
+    key[p] = k;
+    slot[p] = atomicAdd(&count[k], 1);
+}
+
+// Cell key with the arithmetic of ppm_util_rank3d.f:414-475 (FLOOR, ghost
+// cells, upper-boundary corrections, -1 for particles in no box).
+template <int DIM, typename PosT>
+__global__ void __launch_bounds__(256) k_rank_keys(const PosT* __restrict__ xp, int64_t np,
+                                                   RankGeom g, int* __restrict__ key,
+                                                   int* __restrict__ slot, int* __restrict__ count,
+                                                   int* __restrict__ stats) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const PosT* x = xp + p * DIM;
+    int idx[3] = {0, 0, 0};
+    int corr = 0;
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        PosT mn = (PosT)g.xmin[d], mx = (PosT)g.xmax[d];
+        PosT rd = (PosT)g.rdx[d];
+        int i = (int)floor((x[d] - mn) * rd) + g.ngl_lo[d];
+        if (x[d] >= mx && i < g.nm[d] + g.ngl_lo[d]) { i = g.nm[d] + g.ngl_lo[d]; ++corr; }
+        idx[d] = i;
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        PosT mx = (PosT)g.xmax[d];
+        if (x[d] < mx && idx[d] >= g.nm[d] + g.ngl_lo[d]) { idx[d] = g.nm[d] + g.ngl_lo[d] - 1; ++corr; }
+        if (idx[d] < 0 || idx[d] >= g.ntot[d]) in = false;
+    }
+    if (corr) atomicAdd(&stats[1], corr);
+    if (!in) {
+        key[p] = -1;
+        atomicAdd(&stats[0], 1);
+        return;
+    }
+    int k = idx[0] + g.ntot[0] * (idx[1] + g.ntot[1] * idx[2]);
+    key[p] = k;
+    slot[p] = atomicAdd(&count[k], 1);
+}
+
+__global__ void __launch_bounds__(256) k_scatter(const int* __restrict__ key, const int* __restrict__ slot,
+                                                 const int* __restrict__ start, int64_t np,
+                                                 int* __restrict__ perm) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int k = key[p];
+    if (k < 0) return;
+    perm[start[k] + slot[p]] = (int)p;
+}
+
+// Restore increasing particle index inside every cell (stability).
+__global__ void __launch_bounds__(256) k_stabilize(const int* __restrict__ start, int64_t ncells,
+                                                   int* __restrict__ perm) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int b = start[c], e = start[c + 1];
+    if (e - b < 2) return;
+    for (int i = b + 1; i < e; ++i) {
+        int v = perm[i];
+        int j = i - 1;
+        while (j >= b && perm[j] > v) {
+            perm[j + 1] = perm[j];
+            --j;
+        }
+        perm[j + 1] = v;
+    }
+}
+
+__global__ void k_to_one_based(int* __restrict__ a, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] += 1;
+}
+
+__global__ void k_fill_zero(uint32_t* __restrict__ p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0u;
+}
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b > 0 ? (n + b - 1) / b : 1); }
+
+// Zero fill with a kernel (graph-capture safe, unlike hipMemsetAsync).
+int fill_zero(void* p, size_t bytes, hipStream_t s) {
+    int64_t n = (int64_t)(bytes / 4);
+    if (n <= 0) return 0;
+    int64_t g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)g), dim3(256), 0, s, (uint32_t*)p, n);
+    PPMG_HIP(hipGetLastError());
+    return 0;
+}
+
+int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp,
+                  hipStream_t s, int64_t* nbad) {
+    int64_t ncells = (int64_t)m.nbin[0] * m.nbin[1] * m.nbin[2];
+    if (plan->key.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
+        plan->slot.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
+        plan->perm.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
+        plan->count.ensure(sizeof(int) * (size_t)ncells) ||
+        plan->start.ensure(sizeof(int) * (size_t)(ncells + 1)) ||
+        plan->err.ensure(sizeof(int) * 4))
+        return PPMG_ERROR_FATAL;
+    if (fill_zero(plan->count.p, sizeof(int) * (size_t)ncells, s) || fill_zero(plan->err.p, sizeof(int) * 4, s))
+        return PPMG_ERROR_FATAL;
+    if (np > 0) {
+        unsigned g = nblk(np, 256);
+        if (m.dim == 3) {
+            if (prec == PPMG_PREC_F32)
+                hipLaunchKernelGGL((k_bin<3, float>), dim3(g), dim3(256), 0, s, (const float*)xp, np, m,
+                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), plan->err.as<int>());
+            else
+                hipLaunchKernelGGL((k_bin<3, double>), dim3(g), dim3(256), 0, s, (const double*)xp, np, m,
+                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), plan->err.as<int>());
+        } else {
+            if (prec == PPMG_PREC_F32)
+                hipLaunchKernelGGL((k_bin<2, float>), dim3(g), dim3(256), 0, s, (const float*)xp, np, m,
+                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), plan->err.as<int>());
+            else
+                hipLaunchKernelGGL((k_bin<2, double>), dim3(g), dim3(256), 0, s, (const double*)xp, np, m,
+                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), plan->err.as<int>());
+        }
+        PPMG_HIP(hipGetLastError());
+    }
+    int hbad = 0;
+    PPMG_HIP(hipMemcpyAsync(&hbad, plan->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    PPMG_HIP(hipStreamSynchronize(s));
+    *nbad = hbad;
+    if (hbad) return 0;
+    if (exclusive_scan(plan->count.as<int>(), ncells, plan->start.as<int>(), plan->scan_tmp, np, s))
+        return PPMG_ERROR_FATAL;
+    if (np > 0) {
+        hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 256)), dim3(256), 0, s, plan->key.as<int>(),
+                           plan->slot.as<int>(), plan->start.as<int>(), np, plan->perm.as<int>());
+        hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, plan->start.as<int>(),
+                           ncells, plan->perm.as<int>());
+        PPMG_HIP(hipGetLastError());
+    }
+    plan->np = np;
+    plan->ncells = ncells;
+    return 0;
+}
+
+int rank_particles(ppmg_plan* plan, const RankGeom& g, int prec, int64_t np, const void* xp,
+                   int* lpdx, int* lhbx, int64_t* icount, hipStream_t s) {
+    int64_t ncells = (int64_t)g.ntot[0] * g.ntot[1] * g.ntot[2];
+    if (plan->key.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
+        plan->slot.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
+        plan->count.ensure(sizeof(int) * (size_t)ncells) || plan->err.ensure(sizeof(int) * 4))
+        return PPMG_ERROR_FATAL;
+    if (fill_zero(plan->count.p, sizeof(int) * (size_t)ncells, s) || fill_zero(plan->err.p, sizeof(int) * 4, s))
+        return PPMG_ERROR_FATAL;
+    if (np > 0) {
+        unsigned gr = nblk(np, 256);
+        int* st = plan->err.as<int>();
+        if (g.dim == 3) {
+            if (prec == PPMG_PREC_F32)
+                hipLaunchKernelGGL((k_rank_keys<3, float>), dim3(gr), dim3(256), 0, s, (const float*)xp, np, g,
+                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), st);
+            else
+                hipLaunchKernelGGL((k_rank_keys<3, double>), dim3(gr), dim3(256), 0, s, (const double*)xp, np, g,
+                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), st);
+        } else {
+            if (prec == PPMG_PREC_F32)
+                hipLaunchKernelGGL((k_rank_keys<2, float>), dim3(gr), dim3(256), 0, s, (const float*)xp, np, g,
+                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), st);
+            else
+                hipLaunchKernelGGL((k_rank_keys<2, double>), dim3(gr), dim3(256), 0, s, (const double*)xp, np, g,
+                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), st);
+        }
+        PPMG_HIP(hipGetLastError());
+    }
+    int stats[2] = {0, 0};
+    PPMG_HIP(hipMemcpyAsync(stats, plan->err.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    PPMG_HIP(hipStreamSynchronize(s));
+    int64_t inside = np - stats[0];
+    if (exclusive_scan(plan->count.as<int>(), ncells, lhbx, plan->scan_tmp, inside, s)) return PPMG_ERROR_FATAL;
+    if (np > 0) {
+        hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 256)), dim3(256), 0, s, plan->key.as<int>(),
+                           plan->slot.as<int>(), lhbx, np, lpdx);
+        hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, lhbx, ncells, lpdx);
+        hipLaunchKernelGGL(k_to_one_based, dim3(nblk(inside, 256)), dim3(256), 0, s, lpdx, inside);
+    }
+    hipLaunchKernelGGL(k_to_one_based, dim3(nblk(ncells + 1, 256)), dim3(256), 0, s, lhbx, ncells + 1);
+    PPMG_HIP(hipGetLastError());
+    *icount = inside;
+    return 0;
+}
+
+}  // namespace ppmg

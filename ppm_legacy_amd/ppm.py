@@ -1,0 +1,187 @@
+"""PPM-compatible interface (the reference's names, argument meaning and info
+convention) over libppmg's topo_id / mesh_id registry.
+
+    info = ppm_init(dim, prec, tolexp, comm, debug)
+    info, topo_id, mesh_id = ppm_mktopo(min_phys, max_phys, bcdef, Nm, ghostsize)
+    info = ppm_interp_p2m(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, field_up, lda=None)
+    info = ppm_interp_m2p(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, field_up, lda=None)
+    info = ppm_rmsh_comp_weights(xp, Np, topo_id, mesh_id, kernel, wx1, wx2, wx3)
+    info = ppm_rmsh_remesh(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, field_up, wx1, wx2, wx3, lda=None)
+    info, xp, up = ppm_rmsh_create_part(field_up, topo_id, mesh_id, cutoff, ghostsize, resetpos, cutoff_weights)
+    info = ppm_util_rank3d(xp, Np, xmin, xmax, Nm, Ngl, lpdx, lhbx)
+    info = ppm_impose_part_bc(xp, Npart, topo_id)
+
+Arrays are numpy arrays (host: staged to the GPU by the library) or torch
+tensors (CUDA tensors stay on the device), with the Fortran layouts in C
+order: xp (Np, dim), up (Np, lda) or (Np,), field_up (.., Nm+2g, Nm+2g, lda)
+or without the lda axis for the scalar specifics.  Outputs are written in
+place; info is 0 or the reference's negative severity, and
+ppm_legacy_amd._lib.last_error() gives the ppm_err_* code and message.
+The double specifics run in the form chosen by ppm_set_precision_mode
+(PREC_F64 by default, or PREC_F32C); float32 arrays select the single
+specifics.  Reference: ppm_interp_p2m.f:130-160, ppm_interp_m2p.f:129-159,
+ppm_rmsh_remesh.f:91-121, ppm_rmsh_comp_weights.f:97-103,
+ppm_rmsh_create_part.f:90-171, ppm_util_rank3d.f:147-151,
+ppm_impose_part_bc.f:67-214, ppm_init.f:161, ppm_topo_mkfield.f:316-319.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import core
+from ._lib import KERNEL_BSP2, KERNEL_MP4, PREC_F32, PREC_F32C, PREC_F64, lib
+
+ppm_param_rmsh_kernel_bsp2 = KERNEL_BSP2
+ppm_param_rmsh_kernel_mp4 = KERNEL_MP4
+ppm_param_bcdef_periodic = 1
+ppm_param_bcdef_freespace = 2
+ppm_kind_double = PREC_F64
+ppm_kind_single = PREC_F32
+
+_topos = {}
+
+
+def _mem_ptr(a):
+    """(memory space, pointer, element dtype) of a numpy array or torch tensor."""
+    if isinstance(a, torch.Tensor):
+        if not a.is_contiguous():
+            raise ValueError("arrays must be contiguous")
+        return (L.MEM_DEVICE if a.is_cuda else L.MEM_HOST), ctypes.c_void_p(a.data_ptr()), a.dtype
+    if isinstance(a, np.ndarray):
+        if not a.flags.c_contiguous:
+            raise ValueError("arrays must be C contiguous")
+        return L.MEM_HOST, a.ctypes.data_as(ctypes.c_void_p), a.dtype
+    raise TypeError("expected a numpy array or a torch tensor")
+
+
+def _prec_of(dtype):
+    return PREC_F32 if dtype in (np.float32, np.dtype(np.float32), torch.float32) else PREC_F64
+
+
+def _gs(ghostsize, dim):
+    g = [int(v) for v in ghostsize] + [0] * (3 - len(ghostsize))
+    return (ctypes.c_int * 3)(*g[:3])
+
+
+def ppm_init(dim, prec=ppm_kind_double, tolexp=-15, comm=None, debug=0):
+    return lib().ppmg_init(int(dim), int(prec), int(tolexp), int(debug))
+
+
+def ppm_finalize():
+    _topos.clear()
+    return lib().ppmg_finalize()
+
+
+def ppm_set_precision_mode(prec):
+    return lib().ppmg_set_precision_mode(int(prec))
+
+
+def ppm_mktopo(min_phys, max_phys, bcdef, Nm, ghostsize):
+    """Field topology of one sub covering the domain: (info, topo_id, mesh_id)."""
+    dim = len(Nm)
+    mn = (ctypes.c_double * 3)(*list(min_phys))
+    mx = (ctypes.c_double * 3)(*list(max_phys))
+    bc = (ctypes.c_int * 6)(*list(bcdef))
+    nm = (ctypes.c_int * 3)(*list(Nm))
+    tid, mid = ctypes.c_int(), ctypes.c_int()
+    info = lib().ppmg_mktopo_field(dim, mn, mx, bc, nm, _gs(ghostsize, dim), ctypes.byref(tid), ctypes.byref(mid))
+    if info == 0:
+        _topos[tid.value] = core.Mesh(dim, list(Nm), list(min_phys), list(max_phys), list(bcdef))
+    return info, tid.value, mid.value
+
+
+def _lda_of(up, lda):
+    if lda is not None:
+        return int(lda)
+    return 1 if up.ndim == 1 else int(up.shape[1])
+
+
+def ppm_interp_p2m(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, field_up, lda=None):
+    mem, pxp, dt = _mem_ptr(xp)
+    _, pup, _ = _mem_ptr(up)
+    fmem, pf, _ = _mem_ptr(field_up)
+    if fmem != mem:
+        raise ValueError("xp, up and field_up must live in the same memory space")
+    return lib().ppmg_interp_p2m(pxp, int(Np), pup, _lda_of(up, lda), int(topo_id), int(mesh_id), int(kernel),
+                                 _gs(ghostsize, xp.shape[1]), pf, _prec_of(dt), mem)
+
+
+def ppm_interp_m2p(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, field_up, lda=None):
+    mem, pxp, dt = _mem_ptr(xp)
+    _, pup, _ = _mem_ptr(up)
+    _, pf, _ = _mem_ptr(field_up)
+    return lib().ppmg_interp_m2p(pxp, int(Np), pup, _lda_of(up, lda), int(topo_id), int(mesh_id), int(kernel),
+                                 _gs(ghostsize, xp.shape[1]), pf, _prec_of(dt), mem)
+
+
+def ppm_rmsh_comp_weights(xp, Np, topo_id, mesh_id, kernel, wx1, wx2, wx3=None):
+    mem, pxp, dt = _mem_ptr(xp)
+    w = [_mem_ptr(a)[1] if a is not None else ctypes.c_void_p(0) for a in (wx1, wx2, wx3)]
+    return lib().ppmg_rmsh_comp_weights_reg(pxp, int(Np), int(topo_id), int(mesh_id), int(kernel), w[0], w[1], w[2],
+                                            _prec_of(dt), mem)
+
+
+def ppm_rmsh_remesh(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, field_up, wx1, wx2, wx3=None, lda=None):
+    mem, pxp, dt = _mem_ptr(xp)
+    _, pup, _ = _mem_ptr(up)
+    _, pf, _ = _mem_ptr(field_up)
+    w = [_mem_ptr(a)[1] if a is not None else ctypes.c_void_p(0) for a in (wx1, wx2, wx3)]
+    return lib().ppmg_rmsh_remesh_reg(pxp, int(Np), pup, _lda_of(up, lda), int(topo_id), int(mesh_id), int(kernel),
+                                      _gs(ghostsize, xp.shape[1]), pf, w[0], w[1], w[2], _prec_of(dt), mem)
+
+
+def ppm_util_rank3d(xp, Np, xmin, xmax, Nm, Ngl, lpdx, lhbx):
+    """lpdx (capacity Np) and lhbx (nbox+1) receive the 1-based ranking; returns (info, icount)."""
+    mem, pxp, dt = _mem_ptr(xp)
+    _, plp, _ = _mem_ptr(lpdx)
+    _, plh, _ = _mem_ptr(lhbx)
+    dim = xp.shape[1]
+    ctype = ctypes.c_float if _prec_of(dt) == PREC_F32 else ctypes.c_double
+    mn = (ctype * 3)(*list(xmin))
+    mx = (ctype * 3)(*list(xmax))
+    nc = (ctypes.c_int * 3)(*list(Nm))
+    ng = (ctypes.c_int * 6)(*list(Ngl))
+    ic = ctypes.c_int()
+    info = lib().ppmg_util_rank_reg(dim, pxp, int(Np), mn, mx, nc, ng, plp, plh, ctypes.byref(ic), _prec_of(dt), mem)
+    return info, ic.value
+
+
+ppm_util_rank2d = ppm_util_rank3d
+
+
+def _device_tensor(a):
+    return a if isinstance(a, torch.Tensor) and a.is_cuda else torch.as_tensor(np.asarray(a)).cuda()
+
+
+def ppm_impose_part_bc(xp, Npart, topo_id):
+    mesh = _topos.get(int(topo_id))
+    if mesh is None:
+        return -2
+    t = _device_tensor(xp)
+    try:
+        core.impose_part_bc(mesh, t[: int(Npart)])
+    except L.PpmgError as e:
+        return e.info
+    if t is not xp:
+        src = t.cpu().numpy() if isinstance(xp, np.ndarray) else t.cpu()
+        xp[...] = src
+    return 0
+
+
+def ppm_rmsh_create_part(field_up, topo_id, mesh_id, cutoff, ghostsize, resetpos=False, cutoff_weights=None):
+    """Returns (info, xp, up) of the new particles (device tensors)."""
+    mesh = _topos.get(int(topo_id))
+    if mesh is None:
+        return -2, None, None
+    f = _device_tensor(field_up)
+    if f.dim() == mesh.dim:
+        f = f.unsqueeze(-1)
+    prec = PREC_F32 if f.dtype == torch.float32 else PREC_F64
+    lo = [-int(g) for g in ghostsize]
+    try:
+        xp, up = core.create_part(mesh, f.contiguous(), lo, cutoff, cutoff_weights, resetpos, prec)
+    except L.PpmgError as e:
+        return e.info, None, None
+    return 0, xp, up

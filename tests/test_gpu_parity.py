@@ -1,0 +1,381 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Bars: bit-exact cell keys and sort permutation; FP64
+fields within 1e-12 of the field maximum; FP32-compensated and FP32 within
+1e-6 of the field maximum (tolerances written per test)."""
+import numpy as np
+import pytest
+import torch
+
+import bruteforce as bf
+from ppm_legacy_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+MP4, BSP2 = 2, 1
+F64, F32C, F32 = 0, 1, 2
+# F32 (the reference's single specifics) is checked against the float oracle,
+# which sums in plain float; 3e-6 of the field maximum covers that oracle's
+# own rounding (the GPU form is compensated).  F32C is checked against the
+# FP64 oracle at the 1e-6 bar.
+TOL = {F64: 1e-12, F32C: 1e-6, F32: 3e-6}
+
+
+@pytest.fixture(scope="module")
+def core():
+    from ppm_legacy_amd import core as c
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return c
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _dev(a, dtype):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).cuda()
+
+
+def _pos_dtype(prec):
+    return torch.float32 if prec == F32 else torch.float64
+
+
+def _val_dtype(prec):
+    return torch.float64 if prec == F64 else torch.float32
+
+
+def _np_val(prec):
+    return np.float64 if prec == F64 else np.float32
+
+
+def _case(dim, N, n, seed, lda, lo=0.0, L=1.0):
+    rng = np.random.default_rng(seed)
+    xp = lo + L * rng.uniform(0, 1, size=(n, dim))
+    up = rng.normal(size=(n, lda))
+    return xp, up
+
+
+# ------------------------------------------------------------ config 1 --
+def test_p2m_config1_64cubed_fp64(core, oracle):
+    """Plain 3D M'4 P2M, FP64, 64^3 periodic, 262,144 jittered particles (seed 1)."""
+    N, g = 64, [2, 2, 2]
+    xp = wl.jittered_lattice(N, seed=1)
+    up = wl.strengths(1, len(xp), 1)
+    t = oracle.topo(3, [N + 1] * 3)
+    info, ref = oracle.p2m(t, MP4, xp, up, g)
+    assert info == 0
+    mesh = core.Mesh(3, N + 1)
+    field = torch.zeros(mesh.padded_shape(1, g), dtype=torch.float64, device="cuda")
+    plan = core.p2m(mesh, _dev(xp, torch.float64), _dev(up, torch.float64), field, [-2, -2, -2], MP4, F64)
+    got = field.cpu().numpy()
+    # every stored node: real nodes folded, ghosts raw, as the reference leaves them
+    assert _rel(got, ref) < 1e-12
+    assert _rel(oracle.real_nodes(got, g, 3), oracle.real_nodes(ref, g, 3)) < 1e-12
+    uniq = oracle.unique_nodes(got, g, 3)
+    assert abs(uniq.sum() - up.sum()) < 1e-9 * np.abs(up).sum()
+    # cell keys and permutation: bit-exact against the reference's index arithmetic
+    key, perm, start = plan.sort_view()
+    x0 = (xp - 0.0) * 64.0
+    cells = x0.astype(np.int64)                 # INT() truncation, ppm_interp_p2m.f:804-822
+    ref_key = cells[:, 0] + N * (cells[:, 1] + N * cells[:, 2])
+    assert np.array_equal(key.cpu().numpy(), ref_key)
+    assert np.array_equal(perm.cpu().numpy(), np.argsort(ref_key, kind="stable"))
+    assert np.array_equal(start.cpu().numpy(), np.concatenate([[0], np.cumsum(np.bincount(ref_key, minlength=N ** 3))]))
+
+
+# ----------------------------------------------------------- variants --
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+@pytest.mark.parametrize("prec", [F64, F32C, F32])
+@pytest.mark.parametrize("lda", [1, 3, 5])
+def test_p2m_variants(core, oracle, dim, kernel, prec, lda):
+    N = 12 if dim == 3 else 20
+    xp, up = _case(dim, N, 1500, 100 + dim * 10 + kernel + lda, lda, lo=-0.5, L=2.0)
+    g = [2] * dim
+    t = oracle.topo(dim, [N + 1] * dim, [-0.5] * dim, [1.5] * dim)
+    odt = np.float32 if prec == F32 else np.float64
+    info, ref = oracle.p2m(t, kernel, xp.astype(odt), up.astype(odt), g, dtype=odt)
+    assert info == 0
+    mesh = core.Mesh(dim, N + 1, [-0.5] * dim, [1.5] * dim)
+    field = torch.zeros(mesh.padded_shape(lda, g), dtype=_val_dtype(prec), device="cuda")
+    core.p2m(mesh, _dev(xp, _pos_dtype(prec)), _dev(up, _val_dtype(prec)), field, [-2] * dim, kernel, prec)
+    assert _rel(field.cpu().numpy(), ref) < TOL[prec]
+
+
+def test_p2m_compact_equals_padded_and_is_deterministic(core):
+    N = 16
+    xp, up = _case(3, N, 4000, 7, 3)
+    mesh = core.Mesh(3, N + 1)
+    padded = torch.zeros(mesh.padded_shape(3, [1, 1, 1]), dtype=torch.float64, device="cuda")
+    compact = torch.zeros(mesh.compact_shape(3), dtype=torch.float64, device="cuda")
+    x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
+    core.p2m(mesh, x, u, padded, [-1] * 3)
+    core.p2m(mesh, x, u, compact, [0] * 3)
+    again = torch.zeros_like(compact)
+    core.p2m(mesh, x, u, again, [0] * 3)
+    assert torch.equal(compact, padded[1:1 + N, 1:1 + N, 1:1 + N])
+    assert torch.equal(compact, again)                      # bitwise reproducible
+    ref = bf.p2m_periodic(xp, up, N)
+    assert _rel(compact.cpu().numpy(), ref) < 1e-12
+
+
+def test_p2m_nonperiodic_axes(core, oracle):
+    N = 10
+    xp, up = _case(3, N, 800, 11, 2)
+    bc = [2, 2, 1, 1, 2, 2]
+    g = [2, 1, 2]
+    t = oracle.topo(3, [N + 1] * 3, bcdef=bc)
+    info, ref = oracle.p2m(t, MP4, xp, up, g)
+    assert info == 0
+    mesh = core.Mesh(3, N + 1, bcdef=bc)
+    field = torch.zeros(mesh.padded_shape(2, g), dtype=torch.float64, device="cuda")
+    core.p2m(mesh, _dev(xp, torch.float64), _dev(up, torch.float64), field, [-2, -1, -2])
+    assert _rel(field.cpu().numpy(), ref) < 1e-12
+
+
+def test_p2m_clustered_occupancy(core, oracle):
+    """Gaussian blobs: cells with tens of particles; permutation stays stable."""
+    N = 16
+    xp = wl.gaussian_blobs(20000, seed=4)
+    up = wl.strengths(4, len(xp), 3)
+    t = oracle.topo(3, [N + 1] * 3)
+    info, ref = oracle.p2m(t, MP4, xp, up, [1, 1, 1])
+    mesh = core.Mesh(3, N + 1)
+    field = torch.zeros(mesh.padded_shape(3, [1] * 3), dtype=torch.float64, device="cuda")
+    plan = core.p2m(mesh, _dev(xp, torch.float64), _dev(up, torch.float64), field, [-1] * 3)
+    assert _rel(field.cpu().numpy(), ref) < 1e-12
+    key, perm, _ = plan.sort_view()
+    k = key.cpu().numpy()
+    assert np.bincount(k).max() > 50
+    assert np.array_equal(perm.cpu().numpy(), np.argsort(k, kind="stable"))
+
+
+def test_p2m_unassigned_particle_is_fatal_405(core):
+    from ppm_legacy_amd import _lib
+    mesh = core.Mesh(3, 9)
+    xp = torch.tensor([[0.5, 0.5, 0.5], [0.5, 1.0, 0.5]], dtype=torch.float64, device="cuda")
+    up = torch.ones((2, 1), dtype=torch.float64, device="cuda")
+    field = torch.full(mesh.compact_shape(1), 7.0, dtype=torch.float64, device="cuda")
+    with pytest.raises(_lib.PpmgError) as e:
+        core.p2m(mesh, xp, up, field, [0, 0, 0])
+    assert e.value.info == -1 and e.value.code == _lib.ERR_PART_UNASS
+    assert torch.all(field == 7.0)          # field untouched, like the reference's early return
+
+
+def test_p2m_empty(core):
+    mesh = core.Mesh(3, 9)
+    xp = torch.zeros((0, 3), dtype=torch.float64, device="cuda")
+    up = torch.zeros((0, 2), dtype=torch.float64, device="cuda")
+    field = torch.full(mesh.compact_shape(2), 3.0, dtype=torch.float64, device="cuda")
+    core.p2m(mesh, xp, up, field, [0, 0, 0])
+    assert torch.all(field == 0)
+
+
+# ----------------------------------------------------------------- M2P --
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+@pytest.mark.parametrize("prec", [F64, F32C, F32])
+@pytest.mark.parametrize("lda", [1, 3, 6])
+def test_m2p_variants(core, oracle, dim, kernel, prec, lda):
+    N = 12 if dim == 3 else 24
+    g = [2] * dim
+    rng = np.random.default_rng(200 + dim + kernel + lda)
+    t = oracle.topo(dim, [N + 1] * dim)
+    f = rng.normal(size=oracle.field_shape(t, lda, g))        # ghosts and node N differ from node 0
+    xp = rng.uniform(0, 1, size=(2000, dim))
+    odt = np.float32 if prec == F32 else np.float64
+    info, ref = oracle.m2p(t, kernel, xp.astype(odt), f.astype(odt), g, dtype=odt)
+    assert info == 0
+    mesh = core.Mesh(dim, N + 1)
+    up = torch.empty((2000, lda), dtype=_val_dtype(prec), device="cuda")
+    core.m2p(mesh, _dev(xp, _pos_dtype(prec)), _dev(f, _val_dtype(prec)), [-2] * dim, up, kernel, prec)
+    assert _rel(up.cpu().numpy(), ref) < TOL[prec]
+
+
+def test_m2p_compact_layout(core):
+    N = 16
+    rng = np.random.default_rng(9)
+    uniq = rng.normal(size=(N, N, N, 3))
+    xp = rng.uniform(0, 1, size=(3000, 3))
+    mesh = core.Mesh(3, N + 1)
+    up = torch.empty((3000, 3), dtype=torch.float64, device="cuda")
+    core.m2p(mesh, _dev(xp, torch.float64), _dev(uniq, torch.float64), [0, 0, 0], up)
+    assert _rel(up.cpu().numpy(), bf.m2p_periodic(xp, uniq, N)) < 1e-12
+
+
+# ------------------------------------------------------------- adjoint --
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+@pytest.mark.parametrize("prec", [F64, F32C])
+def test_adjoint_vs_oracle(core, oracle, kernel, prec):
+    N = 16
+    xp, up = _case(3, N, 3000, 300 + kernel, 2)
+    gcot = np.random.default_rng(5).normal(size=(N, N, N, 2))
+    t = oracle.topo(3, [N + 1] * 3)
+    info, gu_ref, gx_ref = oracle.p2m_adjoint(t, kernel, xp, up, gcot)
+    assert info == 0
+    mesh = core.Mesh(3, N + 1)
+    x, u = _dev(xp, torch.float64), _dev(up, _val_dtype(prec))
+    field = torch.empty(mesh.compact_shape(2), dtype=_val_dtype(prec), device="cuda")
+    plan = core.p2m(mesh, x, u, field, [0] * 3, kernel, prec)
+    gu, gx = core.p2m_adjoint(mesh, x, u, _dev(gcot, _val_dtype(prec)), [0] * 3, plan, kernel, prec)
+    tol = 1e-12 if prec == F64 else 2e-6
+    assert _rel(gu.cpu().numpy(), gu_ref) < tol
+    assert _rel(gx.cpu().numpy(), gx_ref) < tol
+
+
+def test_autograd_matches_fd_and_oracle(core, oracle):
+    """Config 3 in miniature: L = 0.5 sum w m^2, gradients by autograd vs central FD (FP64)."""
+    from ppm_legacy_amd import autograd
+    N = 16
+    xp = wl.jittered_lattice(N, seed=3)
+    up = wl.strengths(3, len(xp), 1)
+    w = np.random.default_rng(3).normal(size=(N, N, N, 1))
+    mesh = core.Mesh(3, N + 1)
+    x = _dev(xp, torch.float64).requires_grad_(True)
+    u = _dev(up, torch.float64).requires_grad_(True)
+    wt = _dev(w, torch.float64)
+    m = autograd.p2m(x, u, mesh)
+    loss = 0.5 * (wt * m * m).sum()
+    loss.backward()
+    t = oracle.topo(3, [N + 1] * 3)
+    _, f = oracle.p2m(t, MP4, xp, up, [2, 2, 2])
+    mref = oracle.unique_nodes(f, [2, 2, 2], 3)
+    _, gu_ref, gx_ref = oracle.p2m_adjoint(t, MP4, xp, up, w * mref)
+    assert _rel(u.grad.cpu().numpy(), gu_ref) < 1e-12
+    assert _rel(x.grad.cpu().numpy(), gx_ref) < 1e-12
+    h = 1.0 / N
+    step = 1e-6 * h
+    rng = np.random.default_rng(0)
+    for p in rng.choice(len(xp), size=16, replace=False):
+        for d in range(3):
+            vals = []
+            for sgn in (1, -1):
+                xs = xp.copy()
+                xs[p, d] += sgn * step
+                _, fs = oracle.p2m(t, MP4, xs, up, [2, 2, 2])
+                ms = oracle.unique_nodes(fs, [2, 2, 2], 3)
+                vals.append(0.5 * (w * ms * ms).sum())
+            fd = (vals[0] - vals[1]) / (2 * step)
+            assert abs(fd - x.grad[p, d].item()) <= 1e-6 * max(1.0, abs(fd))
+
+
+# --------------------------------------------------------------- rank3d --
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_util_rank_bit_exact(core, oracle, dim, prec):
+    rng = np.random.default_rng(50 + dim)
+    dt = np.float32 if prec == F32 else np.float64
+    xp = rng.uniform(-0.1, 1.1, size=(20000, dim)).astype(dt)
+    xp[::5] = xp[1::5][: len(xp[::5])]
+    xp[3] = 1.0                                          # exactly on xmax
+    Nc = [13, 7, 9][:dim]
+    Ngl = [1, 0, 2, 1, 2, 0][: 2 * dim] if dim == 3 else [1, 0, 1, 2]
+    lp_ref, lh_ref, ic_ref, nout, _ = oracle.rank(xp, [0.0] * dim, [1.0] * dim, Nc, Ngl, dtype=dt)
+    lp, lh = core.util_rank(_dev(xp, torch.float32 if prec == F32 else torch.float64), [0.0] * dim, [1.0] * dim,
+                            Nc, Ngl)
+    assert np.array_equal(lp.cpu().numpy(), lp_ref)
+    assert np.array_equal(lh.cpu().numpy(), lh_ref)
+    assert nout > 0
+
+
+# --------------------------------------------------------------- remesh --
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+def test_comp_weights_and_remesh(core, oracle, dim, kernel):
+    N = 12
+    xp, up = _case(dim, N, 1000, 400 + dim + kernel, 3)
+    g = [2] * dim
+    t = oracle.topo(dim, [N + 1] * dim)
+    info, wref = oracle.comp_weights(t, kernel, xp)
+    assert info == 0
+    mesh = core.Mesh(dim, N + 1)
+    x = _dev(xp, torch.float64)
+    w = core.comp_weights(mesh, x, kernel)
+    for d in range(dim):
+        assert np.max(np.abs(w[d].cpu().numpy() - wref[d])) < 1e-15
+    info, fref = oracle.remesh(t, kernel, xp, up, g, wref)
+    assert info == 0
+    field = torch.zeros(mesh.padded_shape(3, g), dtype=torch.float64, device="cuda")
+    core.remesh(mesh, x, _dev(up, torch.float64), w, field, [-2] * dim, kernel)
+    assert _rel(field.cpu().numpy(), fref) < 1e-12
+
+
+def test_create_part(core, oracle):
+    N = 10
+    g = [1, 1, 1]
+    t = oracle.topo(3, [N + 1] * 3)
+    f = np.random.default_rng(6).normal(size=oracle.field_shape(t, 3, g))
+    cut = [0.7, 1.9]
+    info, xref, uref = oracle.create_part(t, f, g, cut)
+    assert info == 0
+    mesh = core.Mesh(3, N + 1)
+    xp, up = core.create_part(mesh, _dev(f, torch.float64), [-1] * 3, cut)
+    assert np.array_equal(up.cpu().numpy(), uref)
+    assert np.array_equal(xp.cpu().numpy(), xref)
+    cw = [0.5, 2.0, 1.0]
+    info, xref, uref = oracle.create_part(t, f, g, cut, cutoff_weights=np.array(cw))
+    xp, up = core.create_part(mesh, _dev(f, torch.float64), [-1] * 3, cut, cutoff_weights=cw)
+    assert np.array_equal(up.cpu().numpy(), uref)
+
+
+def test_impose_part_bc(core, oracle):
+    t = oracle.topo(3, [9, 9, 9], [-1.0, 0.0, 0.0], [1.0, 1.0, 2.0])
+    rng = np.random.default_rng(12)
+    xp = rng.uniform(-1.5, 2.5, size=(5000, 3))
+    ref = oracle.impose_part_bc(t, xp)
+    mesh = core.Mesh(3, 9, [-1.0, 0.0, 0.0], [1.0, 1.0, 2.0])
+    x = _dev(xp, torch.float64)
+    core.impose_part_bc(mesh, x)
+    assert np.array_equal(x.cpu().numpy(), ref)
+
+
+# ------------------------------------------------------------ generators --
+def test_device_generators_match_host(core):
+    N = 24
+    xd = core.gen_jittered_lattice(N, seed=77).cpu().numpy()
+    assert np.array_equal(xd, wl.jittered_lattice(N, seed=77))
+    ud = core.gen_normals(1000, 3, 77).cpu().numpy()
+    assert np.allclose(ud, wl.strengths(77, 1000, 3), rtol=1e-13, atol=1e-13)
+    modes = wl.fourier_modes(9)
+    vd = core.gen_divfree_velocity(core.gen_jittered_lattice(8, 1), modes).cpu().numpy()
+    assert np.allclose(vd, wl.divfree_velocity(wl.jittered_lattice(8, 1), modes), rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------------- registry --
+def test_registry_host_arrays_like_the_reference(oracle):
+    """ppm_init / ppm_mktopo / ppm_interp_p2m / m2p with host (numpy) arrays."""
+    from ppm_legacy_amd import ppm
+    assert ppm.ppm_init(3, ppm.ppm_kind_double, -15, None, 0) == 0
+    N = 16
+    g = [2, 2, 2]
+    info, topo, meshid = ppm.ppm_mktopo([0.0] * 3, [1.0] * 3, [1] * 6, [N + 1] * 3, g)
+    assert info == 0
+    xp = wl.jittered_lattice(N, seed=21)
+    up = wl.strengths(21, len(xp), 3)
+    field = np.zeros((N + 1 + 4,) * 3 + (3,))
+    assert ppm.ppm_interp_p2m(xp, len(xp), up, topo, meshid, ppm.ppm_param_rmsh_kernel_mp4, g, field) == 0
+    t = oracle.topo(3, [N + 1] * 3)
+    _, ref = oracle.p2m(t, MP4, xp, up, g)
+    assert _rel(field, ref) < 1e-12
+    back = np.zeros_like(up)
+    assert ppm.ppm_interp_m2p(xp, len(xp), back, topo, meshid, ppm.ppm_param_rmsh_kernel_mp4, g, field) == 0
+    _, bref = oracle.m2p(t, MP4, xp, ref, g)
+    assert _rel(back, bref) < 1e-12
+    # single specifics
+    xs, us = xp.astype(np.float32), up.astype(np.float32)
+    fs = np.zeros(field.shape, np.float32)
+    assert ppm.ppm_interp_p2m(xs, len(xs), us, topo, meshid, ppm.ppm_param_rmsh_kernel_mp4, g, fs) == 0
+    _, sref = oracle.p2m(t, MP4, xs, us, g, dtype=np.float32)
+    assert _rel(fs, sref) < 1e-6
+    # FP32-compensated form behind the unchanged double signature
+    assert ppm.ppm_set_precision_mode(ppm.PREC_F32C) == 0
+    fc = np.zeros_like(field)
+    assert ppm.ppm_interp_p2m(xp, len(xp), up, topo, meshid, ppm.ppm_param_rmsh_kernel_mp4, g, fc) == 0
+    assert _rel(fc, ref) < 1e-6
+    assert ppm.ppm_set_precision_mode(ppm.PREC_F64) == 0
+    # the reference's fatal error on a particle outside the sub
+    bad = xp.copy()
+    bad[5, 1] = 1.0
+    assert ppm.ppm_interp_p2m(bad, len(bad), up, topo, meshid, 2, g, field) == -1
+    assert ppm.ppm_finalize() == 0
