@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-f32c", action="store_true", help="skip the FP32-compensated leg")
     ap.add_argument("--cpu-n", type=int, default=256, help="cells per axis of the CPU sample")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -76,7 +78,7 @@ def time_form(core, torch, mesh, x, up, xfresh, prec, steps, warmup):
     for _ in range(warmup):
         run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan)
     torch.cuda.synchronize()
-    phases = {"bin": [], "weights": [], "deposit": [], "m2p": []}
+    phases = {"bin": [], "weights": [], "deposit": [], "m2p_bin": [], "m2p": []}
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -89,7 +91,9 @@ def time_form(core, torch, mesh, x, up, xfresh, prec, steps, warmup):
         phases["bin"].append(tp[0])
         phases["weights"].append(tp[1])
         phases["deposit"].append(tp[2])
-        phases["m2p"].append(tm[0])
+        # M2P phases: (binning, gather) on the tiled path, (gather) on the general one
+        phases["m2p_bin"].append(tm[0] if len(tm) > 1 else 0.0)
+        phases["m2p"].append(tm[-1])
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -135,6 +139,14 @@ def main():
     from ppm_legacy_amd import core, workloads as wl
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")     # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8, "launches": 3}))
+        return
     N, lda = args.n, args.lda
     np_ = N ** 3
     mesh, x, w64, xfresh, dt = make_inputs(core, wl, N, lda, args.seed, torch)
@@ -201,7 +213,7 @@ def main():
                    "cells": N, "particles": np_, "lda": lda, "kernel": "M'4", "seed": args.seed},
         "cycle_hbm_frac": 2 * b64_pass / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "phases_ms": ph64,
-        "roofline": {"bound": "hbm", "kernel": "k_p2m_gather" if dom == "deposit" else "k_m2p_gather",
+        "roofline": {"bound": "hbm", "kernel": "k_p3_tile" if dom == "deposit" else "k_m3_tile",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "fp32c": res or None,

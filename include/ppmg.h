@@ -191,6 +191,9 @@ int ppmg_set_precision_mode(int prec);
 int ppmg_mktopo_field(int dim, const double* min_phys, const double* max_phys, const int* bcdef,
                       const int* nm, const int* ghostsize, int* topo_id, int* mesh_id);
 int ppmg_topo_destroy(int topo_id);
+/* Descriptor of a registered topology (what the reference keeps in
+ * ppm_module_data / ppm_module_data_mesh for a topo_id, ppm_module_data.f:116-190). */
+int ppmg_topo_info(int topo_id, int* dim, int* nm, double* min_phys, double* max_phys, int* bcdef);
 
 /* ppm_interp_p2m(xp,Np,up,[lda,]topo_id,mesh_id,kernel,ghostsize,field_up,info)
  * (ppm_interp_p2m.f:130-160).  prec: PPMG_PREC_F64 (d specifics, honours the
@@ -211,6 +214,15 @@ int ppmg_rmsh_comp_weights_reg(const void* xp, int np, int topo_id, int mesh_id,
 int ppmg_rmsh_remesh_reg(const void* xp, int np, const void* up, int lda, int topo_id,
                          int mesh_id, int kernel, const int* ghostsize, void* field_up,
                          const void* wx1, const void* wx2, const void* wx3, int prec, int mem);
+/* ppm_impose_part_bc(xp,Npart,topo_id,info) (ppm_impose_part_bc.f:67-214). */
+int ppmg_impose_part_bc_reg(void* xp, int np, int topo_id, int prec, int mem);
+/* ppm_rmsh_create_part(xp,np,up,[lda,]field_up,topo_id,mesh_id,cutoff,info,resetpos,
+ * cutoff_weights) (ppm_rmsh_create_part.f:90-171) on the padded field_up;
+ * *np_out receives the number of new particles, at most maxnp are written
+ * (call once with maxnp = 0 to size the arrays, as the reference's counting pass). */
+int ppmg_rmsh_create_part_reg(const void* field_up, int lda, int topo_id, int mesh_id, const int* ghostsize,
+                              const double* cutoff, const double* cutoff_weights, int resetpos, int maxnp,
+                              int* np_out, void* xp, void* up, int prec, int mem);
 /* ppm_util_rank3d / rank2d with host or device arrays. */
 int ppmg_util_rank_reg(int dim, const void* xp, int np, const void* xmin, const void* xmax,
                        const int* ncells, const int* ngl, int* lpdx, int* lhbx, int* icount,

@@ -53,7 +53,7 @@ EXPORTS = [
     "ppmg_plan_set_timing", "ppmg_plan_timings", "ppmg_plan_sort_view", "ppmg_last_error", "ppmg_set_debug", "ppmg_device_sync", "ppmg_init",
     "ppmg_finalize", "ppmg_set_precision_mode", "ppmg_mktopo_field", "ppmg_topo_destroy",
     "ppmg_interp_p2m", "ppmg_interp_m2p", "ppmg_rmsh_comp_weights_reg", "ppmg_rmsh_remesh_reg",
-    "ppmg_util_rank_reg", "ppmg_gen_jittered_lattice", "ppmg_gen_normals", "ppmg_gen_divfree_velocity",
+    "ppmg_util_rank_reg", "ppmg_topo_info", "ppmg_impose_part_bc_reg", "ppmg_rmsh_create_part_reg", "ppmg_gen_jittered_lattice", "ppmg_gen_normals", "ppmg_gen_divfree_velocity",
 ]
 
 _lib = None

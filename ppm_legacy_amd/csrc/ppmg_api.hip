@@ -248,7 +248,14 @@ int ppmg_p2m(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
         return fail(PPMG_ERROR_FATAL, PPMG_ERR_PART_UNASS,
                     "MAJOR PROBLEM: " + std::to_string(nbad) + " particles lie in no sub");
     mark(plan, s);
-    if (p2m_general(plan, m, kernel, prec, np, xp, lda, up, fg, field->data, nullptr, s)) return g_status.info;
+    // PPMG_GENERAL=1 forces the general node-gather path (used to cross-check the fast path)
+    const char* gen = std::getenv("PPMG_GENERAL");
+    const bool force_general = gen && gen[0] && gen[0] != '0';
+    if (!force_general && p2m3d_fast_applicable(m, kernel, prec, fg)) {
+        if (p2m3d_fast(plan, m, prec, np, xp, lda, up, field->data, s)) return g_status.info;
+    } else if (p2m_general(plan, m, kernel, prec, np, xp, lda, up, fg, field->data, nullptr, s)) {
+        return g_status.info;
+    }
     plan->mesh = m;
     plan->dim = m.dim;
     plan->kernel = kernel;
@@ -276,6 +283,19 @@ int ppmg_m2p(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     hipStream_t s = as_stream(stream);
     if (plan->err.ensure(sizeof(int) * 4) || fill_zero(plan->err.p, sizeof(int) * 4, s)) return g_status.info;
     mark(plan, s, true);
+    const char* gen = std::getenv("PPMG_GENERAL");
+    const bool force_general = gen && gen[0] && gen[0] != '0';
+    if (!force_general && m2p3d_fast_applicable(m, kernel, fg)) {
+        int rc = m2p3d_fast(plan, m, prec, np, xp, lda, field->data, up, s);
+        if (rc < 0) return g_status.info;
+        if (rc == 0) {
+            mark(plan, s);
+            plan->have_forward = false;   // the binning now describes these particles
+            return 0;
+        }
+    }
+    // the binning of the tiled path counts particles outside the sub in err[0]: start from zero
+    if (fill_zero(plan->err.p, sizeof(int) * 4, s)) return g_status.info;
     if (m2p_general(m, kernel, prec, np, xp, lda, fg, field->data, up, plan->err.as<int>(), s)) return g_status.info;
     mark(plan, s);
     int bad = 0;

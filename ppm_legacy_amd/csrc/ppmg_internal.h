@@ -41,6 +41,11 @@ struct KahanF {
         s = t;
     }
     __device__ __forceinline__ float value() const { return s - c; }
+    // fold another compensated partial sum into this one
+    __device__ __forceinline__ void merge(const KahanF& o) {
+        add(o.s);
+        add(-o.c);
+    }
 };
 
 struct PlainD {
@@ -48,11 +53,31 @@ struct PlainD {
     __device__ __forceinline__ void init() { s = 0.0; }
     __device__ __forceinline__ void add(double v) { s += v; }
     __device__ __forceinline__ double value() const { return s; }
+    __device__ __forceinline__ void merge(const PlainD& o) { s += o.s; }
 };
 
 template <typename ValT> struct AccOf;
 template <> struct AccOf<double> { using type = PlainD; };
 template <> struct AccOf<float> { using type = KahanF; };
+
+// Cell index and fractional offset as the reference computes them: x0 is
+// rounded first, x0 = (x - min)*dxi, then INT truncation and frac = x0 - INT(x0)
+// (ppm_interp_p2m.f:800-822).  Contraction is off so that no FMA fuses the
+// product into the subtraction (that would skip the rounding of x0 and move
+// frac by up to one ulp of x0, visible in single precision).
+template <typename T>
+__device__ __forceinline__ T cell_offset(T x, T mn, T dxi, int& i) {
+#pragma clang fp contract(off)
+    T x0 = (x - mn) * dxi;
+    i = (int)x0;
+    return x0 - (T)i;
+}
+
+template <typename T>
+__device__ __forceinline__ T scaled_coord(T x, T mn, T dxi) {
+#pragma clang fp contract(off)
+    return (x - mn) * dxi;
+}
 
 // ---------------------------------------------------------------- weights --
 // M'4 on the nodes INT(x0)-1 .. INT(x0)+2 (0-based) in the 3D unrolled form

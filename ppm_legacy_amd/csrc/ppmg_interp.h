@@ -18,4 +18,15 @@ int create_part_general(ppmg_plan* plan, const DevMesh& m, int prec, const Field
                         const double* cutoff, const double* cw, int resetpos, int64_t maxnp, int64_t* np_out,
                         void* xp, void* up, hipStream_t s);
 
+// 3D M'4 tiled deposit (ppmg_p2m3d.hip): fully periodic mesh, compact layout.
+bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg);
+int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
+               void* field, hipStream_t s);
+
+// 3D M'4 tiled gather (ppmg_m2p3d.hip): bins the particles into the plan first;
+// returns 1 without touching up when a particle lies outside the sub.
+bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
+int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* field,
+               void* up, hipStream_t s);
+
 }  // namespace ppmg

@@ -44,9 +44,8 @@ k_sorted_weights(const int* __restrict__ perm, const PosT* __restrict__ xp, cons
         const PosT* x = xp + (int64_t)p * DIM;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            PosT x0 = (x[d] - (PosT)m.minv[d]) * (PosT)m.dxi[d];
-            int i = (int)x0;
-            PosT frac = x0 - (PosT)i;
+            int i;
+            PosT frac = cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], i);
             ValT a[KS];
             weights_1d<KS, DIM, ValT>((ValT)frac, a);
 #pragma unroll
@@ -153,16 +152,18 @@ k_m2p_gather(DevMesh m, FieldGeom f, const ValT* __restrict__ field, const PosT*
     int nb[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        PosT x0 = (x[d] - (PosT)m.minv[d]) * (PosT)m.dxi[d];
-        int i = (int)x0;
-        weights_1d<KS, DIM, ValT>((ValT)(x0 - (PosT)i), a[d]);
+        int i;
+        PosT frac = cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], i);
+        weights_1d<KS, DIM, ValT>((ValT)frac, a[d]);
         nb[d] = i - BASE;
     }
-    Acc acc[LDA];
-#pragma unroll
-    for (int c = 0; c < LDA; ++c) acc[c].init();
-    bool bad = false;
     constexpr int KZ = DIM == 3 ? KS : 1;
+    Acc acc[KZ][LDA];   // one partial sum per z offset: independent FMA chains
+#pragma unroll
+    for (int oz = 0; oz < KZ; ++oz)
+#pragma unroll
+        for (int c = 0; c < LDA; ++c) acc[oz][c].init();
+    bool bad = false;
 #pragma unroll
     for (int ox = 0; ox < KS; ++ox) {
         int ix = read_index(m, f, 0, nb[0] + ox);
@@ -180,14 +181,18 @@ k_m2p_gather(DevMesh m, FieldGeom f, const ValT* __restrict__ field, const PosT*
                 if (DIM == 3) w = w * a[2][oz];
                 const ValT* fv = field + ((int64_t)ix + (int64_t)f.ext[0] * (iy + (int64_t)f.ext[1] * iz)) * f.lda + c0;
 #pragma unroll
-                for (int c = 0; c < LDA; ++c) acc[c].add(w * fv[c]);
+                for (int c = 0; c < LDA; ++c) acc[oz][c].add(w * fv[c]);
             }
         }
     }
     if (bad) atomicAdd(err, 1);
+#pragma unroll
+    for (int oz = 1; oz < KZ; ++oz)
+#pragma unroll
+        for (int c = 0; c < LDA; ++c) acc[0][c].merge(acc[oz][c]);
     ValT* u = up + p * lda + c0;
 #pragma unroll
-    for (int c = 0; c < LDA; ++c) u[c] = (ValT)acc[c].value();
+    for (int c = 0; c < LDA; ++c) u[c] = (ValT)acc[0][c].value();
 }
 
 // Adjoint of P2M (no reference counterpart), one thread per particle in the
@@ -209,8 +214,8 @@ k_adjoint(DevMesh m, FieldGeom f, const ValT* __restrict__ g, const PosT* __rest
     ValT a[3][KS], da[3][KS];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        PosT x0 = (x[d] - (PosT)m.minv[d]) * (PosT)m.dxi[d];
-        ValT frac = (ValT)(x0 - (PosT)(int)x0);
+        int i;
+        ValT frac = (ValT)cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], i);
         weights_1d<KS, DIM, ValT>(frac, a[d]);
         if constexpr (KS == 4) {
             mp4_unrolled_deriv<ValT>(frac, da[d]);
@@ -282,7 +287,7 @@ k_comp_weights(const PosT* __restrict__ xp, int64_t np, DevMesh m, ValT* __restr
     const PosT* x = xp + p * DIM;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        PosT x0 = (x[d] - (PosT)m.minv[d]) * (PosT)m.dxi[d];
+        PosT x0 = scaled_coord<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d]);
         int ip = (int)x0 + 1;                       // INT(..) + 2 - istart, istart = 1
         int lo = KS == 4 ? ip - 1 : ip;
 #pragma unroll

@@ -182,6 +182,76 @@ int ppmg_topo_destroy(int topo_id) {
     return 0;
 }
 
+int ppmg_topo_info(int topo_id, int* dim, int* nm, double* min_phys, double* max_phys, int* bcdef) {
+    clear_error();
+    Topo t;
+    if (lookup(topo_id, 1, &t)) return current_info();
+    if (dim) *dim = t.mesh.dim;
+    for (int d = 0; d < 3; ++d) {
+        if (nm) nm[d] = t.mesh.nm[d];
+        if (min_phys) min_phys[d] = t.mesh.min_phys[d];
+        if (max_phys) max_phys[d] = t.mesh.max_phys[d];
+    }
+    if (bcdef)
+        for (int f = 0; f < 6; ++f) bcdef[f] = t.mesh.bcdef[f];
+    return 0;
+}
+
+int ppmg_impose_part_bc_reg(void* xp, int np, int topo_id, int prec, int mem) {
+    clear_error();
+    Topo t;
+    if (lookup(topo_id, 1, &t)) return current_info();
+    if (np <= 0) return 0;
+    const bool single = prec == PPMG_PREC_F32;
+    const size_t bytes = (single ? 4 : 8) * (size_t)np * t.mesh.dim;
+    const void* dxp = nullptr;
+    if (stage_in(g_stage.xp, xp, bytes, mem, &dxp)) return current_info();
+    int rc = ppmg_impose_part_bc(&t.mesh, single ? PPMG_PREC_F32 : PPMG_PREC_F64, np, const_cast<void*>(dxp), nullptr);
+    if (rc) return rc;
+    if (mem == PPMG_MEM_HOST) PPMG_HIP(hipMemcpy(xp, dxp, bytes, hipMemcpyDeviceToHost));
+    PPMG_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+int ppmg_rmsh_create_part_reg(const void* field_up, int lda, int topo_id, int mesh_id, const int* ghostsize,
+                              const double* cutoff, const double* cutoff_weights, int resetpos, int maxnp,
+                              int* np_out, void* xp, void* up, int prec, int mem) {
+    clear_error();
+    Topo t;
+    if (lookup(topo_id, mesh_id, &t)) return current_info();
+    if (!ghostsize || !np_out) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "ghostsize or np_out is NULL");
+    int dim = t.mesh.dim;
+    int g[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) g[d] = ghostsize[d];
+    const bool single = prec == PPMG_PREC_F32;
+    const size_t esz = single ? 4 : 8;
+    ppmg_field f = padded_field(t, lda, g, nullptr);
+    int64_t fcount = field_count(f);
+    ppmg_plan* plan = stage_plan();
+    if (!plan) return current_info();
+    const void* dfield = nullptr;
+    void *dxp = nullptr, *dup = nullptr;
+    if (stage_in(g_stage.field, field_up, esz * (size_t)fcount, mem, &dfield)) return current_info();
+    if (stage_out_ptr(g_stage.xp, xp, esz * (size_t)(maxnp > 0 ? maxnp : 1) * dim, mem, &dxp)) return current_info();
+    if (stage_out_ptr(g_stage.up, up, esz * (size_t)(maxnp > 0 ? maxnp : 1) * lda, mem, &dup)) return current_info();
+    f.data = const_cast<void*>(dfield);
+    int64_t n = 0;
+    int rc = ppmg_rmsh_create_part(&t.mesh, single ? PPMG_PREC_F32 : PPMG_PREC_F64, &f, cutoff, cutoff_weights,
+                                   resetpos, maxnp, &n, dxp, dup, plan, nullptr);
+    *np_out = (int)n;
+    if (rc && !(maxnp == 0 && n > 0)) return rc;
+    clear_error();
+    if (mem == PPMG_MEM_HOST && maxnp > 0) {
+        int64_t w = n < maxnp ? n : maxnp;
+        if (w > 0) {
+            PPMG_HIP(hipMemcpy(up, dup, esz * (size_t)w * lda, hipMemcpyDeviceToHost));
+            if (resetpos) PPMG_HIP(hipMemcpy(xp, dxp, esz * (size_t)w * dim, hipMemcpyDeviceToHost));
+        }
+    }
+    PPMG_HIP(hipDeviceSynchronize());
+    return n > maxnp && maxnp > 0 ? reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ALLOC, "maxnp smaller than the new particles") : 0;
+}
+
 int ppmg_interp_p2m(const void* xp, int np, const void* up, int lda, int topo_id, int mesh_id, int kernel,
                     const int* ghostsize, void* field_up, int prec, int mem) {
     clear_error();

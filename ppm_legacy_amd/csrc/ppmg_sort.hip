@@ -45,16 +45,33 @@ __device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* sh, 
     return total;
 }
 
+// Each thread owns SCAN_ITEMS consecutive elements of its tile, read (and in
+// k_scan_down written) as 128-bit vectors when the tile is complete.
+template <bool VEC>
+__device__ __forceinline__ void load_items(const int* __restrict__ in, int64_t n, int64_t base, int (&v)[SCAN_ITEMS]) {
+    if (VEC && base + SCAN_ITEMS <= n) {
+        const int4* q = reinterpret_cast<const int4*>(in + base);
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS / 4; ++k) {
+            int4 t = q[k];
+            v[4 * k] = t.x; v[4 * k + 1] = t.y; v[4 * k + 2] = t.z; v[4 * k + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i) v[i] = base + i < n ? in[base + i] : 0;
+    }
+}
+
+template <bool VEC>
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const int* __restrict__ in, int64_t n,
                                                             int64_t* __restrict__ block_sums) {
     __shared__ int64_t sh[SCAN_BLOCK / 64 + 1];
-    int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    load_items<VEC>(in, n, base, v);
     int64_t s = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) {
-        int64_t idx = base + (int64_t)i * SCAN_BLOCK + threadIdx.x;
-        if (idx < n) s += in[idx];
-    }
+    for (int i = 0; i < SCAN_ITEMS; ++i) s += v[i];
     int64_t excl;
     int64_t tot = block_exclusive_scan(s, sh, excl);
     if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
@@ -74,28 +91,34 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_sums(int64_t* __restrict__ 
     }
 }
 
-// Each thread owns SCAN_ITEMS consecutive elements of its tile.
+template <bool VEC>
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_down(const int* __restrict__ in, int64_t n,
                                                           const int64_t* __restrict__ offs,
                                                           int* __restrict__ out, int64_t total_out) {
     __shared__ int64_t sh[SCAN_BLOCK / 64 + 1];
     int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
     int vals[SCAN_ITEMS];
+    load_items<VEC>(in, n, base, vals);
     int64_t s = 0;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) {
-        int64_t idx = base + i;
-        vals[i] = idx < n ? in[idx] : 0;
-        s += vals[i];
-    }
+    for (int i = 0; i < SCAN_ITEMS; ++i) s += vals[i];
     int64_t excl;
     block_exclusive_scan(s, sh, excl);
     int64_t run = offs[blockIdx.x] + excl;
+    int o[SCAN_ITEMS];
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        int64_t idx = base + i;
-        if (idx < n) out[idx] = (int)run;
+        o[i] = (int)run;
         run += vals[i];
+    }
+    if (VEC && base + SCAN_ITEMS <= n) {
+        int4* q = reinterpret_cast<int4*>(out + base);
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS / 4; ++k) q[k] = make_int4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i)
+            if (base + i < n) out[base + i] = o[i];
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_BLOCK - 1) out[n] = (int)total_out;
 }
@@ -105,9 +128,13 @@ int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, int64_t tota
     if (nb < 1) nb = 1;
     if (tmp.ensure(sizeof(int64_t) * (size_t)nb)) return PPMG_ERROR_FATAL;
     int64_t* sums = tmp.as<int64_t>();
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums);
+    // 128-bit loads and stores need 16-byte aligned arrays (caller arrays may not be)
+    const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    if (vec) hipLaunchKernelGGL(k_scan_reduce<true>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums);
+    else hipLaunchKernelGGL(k_scan_reduce<false>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, sums, nb);
-    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums, out, total);
+    if (vec) hipLaunchKernelGGL(k_scan_down<true>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums, out, total);
+    else hipLaunchKernelGGL(k_scan_down<false>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums, out, total);
     PPMG_HIP(hipGetLastError());
     return 0;
 }
@@ -131,7 +158,7 @@ __global__ void __launch_bounds__(256) k_bin(const PosT* __restrict__ xp, int64_
         PosT mx = (PosT)m.maxv[d];
         if (!(v >= mn && v <= mx)) ok = false;
         if (!(v < mx || !m.upper_open[d])) ok = false;
-        PosT x0 = (v - mn) * (PosT)m.dxi[d];
+        PosT x0 = scaled_coord<PosT>(v, mn, (PosT)m.dxi[d]);
         int i = ok ? (int)x0 : 0;
         if (m.periodic[d] && i >= m.N[d]) i -= m.N[d];
         if (i < 0 || i >= m.nbin[d]) ok = false;
@@ -164,7 +191,7 @@ __global__ void __launch_bounds__(256) k_rank_keys(const PosT* __restrict__ xp, 
     for (int d = 0; d < DIM; ++d) {
         PosT mn = (PosT)g.xmin[d], mx = (PosT)g.xmax[d];
         PosT rd = (PosT)g.rdx[d];
-        int i = (int)floor((x[d] - mn) * rd) + g.ngl_lo[d];
+        int i = (int)floor(scaled_coord<PosT>(x[d], mn, rd)) + g.ngl_lo[d];
         if (x[d] >= mx && i < g.nm[d] + g.ngl_lo[d]) { i = g.nm[d] + g.ngl_lo[d]; ++corr; }
         idx[d] = i;
     }
