@@ -150,11 +150,13 @@ int ppmg_rmsh_remesh(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, co
 /* ppm_rmsh_create_part (ppm_rmsh_create_part.f:90-171): nodes 1..nnx,... whose
  * strength |u| (lda 1) or ||u*w||_2 lies in (cutoff[0], cutoff[1]) become
  * particles in node order; resetpos puts them on the nodes.  cutoff_weights
- * may be NULL.  At most maxnp are written; *np_out is the full count. */
+ * may be NULL.  slave (may be NULL) is the reference's field_wp: its values at
+ * the same nodes go to wp (slave->lda per particle).  At most maxnp are
+ * written; *np_out is the full count. */
 int ppmg_rmsh_create_part(const ppmg_mesh* mesh, int prec, const ppmg_field* field,
                           const double* cutoff, const double* cutoff_weights, int resetpos,
-                          int64_t maxnp, int64_t* np_out, void* xp, void* up, ppmg_plan* plan,
-                          void* stream);
+                          int64_t maxnp, int64_t* np_out, void* xp, void* up,
+                          const ppmg_field* slave, void* wp, ppmg_plan* plan, void* stream);
 
 /* Phase timing with HIP events on the call's stream (off by default; the
  * PPMG_TIMING environment variable switches it on for new plans).
@@ -217,12 +219,14 @@ int ppmg_rmsh_remesh_reg(const void* xp, int np, const void* up, int lda, int to
 /* ppm_impose_part_bc(xp,Npart,topo_id,info) (ppm_impose_part_bc.f:67-214). */
 int ppmg_impose_part_bc_reg(void* xp, int np, int topo_id, int prec, int mem);
 /* ppm_rmsh_create_part(xp,np,up,[lda,]field_up,topo_id,mesh_id,cutoff,info,resetpos,
- * cutoff_weights) (ppm_rmsh_create_part.f:90-171) on the padded field_up;
- * *np_out receives the number of new particles, at most maxnp are written
- * (call once with maxnp = 0 to size the arrays, as the reference's counting pass). */
+ * cutoff_weights,field_wp,wp[,lda2]) (ppm_rmsh_create_part.f:90-171) on the padded
+ * field_up (and field_wp, lda2 components, when not NULL); *np_out receives the
+ * number of new particles, at most maxnp are written (call once with maxnp = 0
+ * to size the arrays, as the reference's counting pass). */
 int ppmg_rmsh_create_part_reg(const void* field_up, int lda, int topo_id, int mesh_id, const int* ghostsize,
                               const double* cutoff, const double* cutoff_weights, int resetpos, int maxnp,
-                              int* np_out, void* xp, void* up, int prec, int mem);
+                              int* np_out, void* xp, void* up, const void* field_wp, int lda2, void* wp,
+                              int prec, int mem);
 /* ppm_util_rank3d / rank2d with host or device arrays. */
 int ppmg_util_rank_reg(int dim, const void* xp, int np, const void* xmin, const void* xmax,
                        const int* ncells, const int* ngl, int* lpdx, int* lhbx, int* icount,

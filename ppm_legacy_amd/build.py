@@ -76,9 +76,9 @@ def build_fortran(verbose=True):
         raise RuntimeError("flang not found")
     out = os.path.join(BUILD, "fortran")
     os.makedirs(out, exist_ok=True)
-    order = ["ppm_module_ppmg.f90", "ppm_module_interp_p2m.f90", "ppm_module_interp_m2p.f90",
-             "ppm_module_rmsh.f90", "ppm_module_util_rank.f90"]
-    srcs = [os.path.join(fdir, f) for f in order if os.path.exists(os.path.join(fdir, f))]
+    order = ["ppm_module_ppmg.f90", "ppm_module_core.f90", "ppm_module_interp_p2m.f90",
+             "ppm_module_interp_m2p.f90", "ppm_module_rmsh.f90", "ppm_module_util_rank.f90"]
+    srcs = [os.path.join(fdir, f) for f in order]
     objs = []
     for s in srcs:
         o = os.path.join(out, os.path.basename(s) + ".o")
@@ -95,7 +95,7 @@ def build_fortran(verbose=True):
     if os.path.exists(drv):
         exe = os.path.join(HERE, "ppmg_fortran_check")
         cmd = [flang, "-O2", "-I", out, drv, lib, "-L", HERE, "-lppmg",
-               f"-Wl,-rpath,{HERE}", "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
+               "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"flang link failed:\n{r.stdout}\n{r.stderr}")

@@ -244,8 +244,11 @@ def remesh(mesh, xp, up, wx, field, lo, kernel=KERNEL_MP4, prec=PREC_F64, plan=N
     return plan
 
 
-def create_part(mesh, field, lo, cutoff, cutoff_weights=None, resetpos=True, prec=PREC_F64, maxnp=None, plan=None):
-    """ppm_rmsh_create_part: returns (xp, up) of the nodes inside the strength band."""
+def create_part(mesh, field, lo, cutoff, cutoff_weights=None, resetpos=True, prec=PREC_F64, maxnp=None, plan=None,
+                slave=None, slave_lo=None):
+    """ppm_rmsh_create_part: returns (xp, up) of the nodes inside the strength band,
+    or (xp, up, wp) when a slave field (the reference's field_wp) is given; wp
+    holds the slave's values at the same nodes.  xp is None unless resetpos."""
     _check_tensor(field, VAL_DTYPE[prec], "field")
     lda = field.shape[-1]
     if maxnp is None:
@@ -259,11 +262,20 @@ def create_part(mesh, field, lo, cutoff, cutoff_weights=None, resetpos=True, pre
     cw = None if cutoff_weights is None else (ctypes.c_double * lda)(*[float(c) for c in cutoff_weights])
     npo = ctypes.c_int64()
     plan = plan or Plan()
+    sref, wp = None, None
+    if slave is not None:
+        _check_tensor(slave, VAL_DTYPE[prec], "slave")
+        sd = field_desc(mesh, slave, lo if slave_lo is None else slave_lo)
+        sref = ctypes.byref(sd)
+        wp = torch.empty((max(maxnp, 1), slave.shape[-1]), dtype=VAL_DTYPE[prec], device=field.device)
     check(lib().ppmg_rmsh_create_part(mesh.ref(), prec, ctypes.byref(f), cut, cw, int(resetpos),
-                                      ctypes.c_int64(maxnp), ctypes.byref(npo), _ptr(xp), _ptr(up), plan.handle,
-                                      _stream()))
+                                      ctypes.c_int64(maxnp), ctypes.byref(npo), _ptr(xp), _ptr(up), sref, _ptr(wp),
+                                      plan.handle, _stream()))
     n = npo.value
-    return xp[:n], up[:n]
+    xo = xp[:n] if resetpos else None
+    if slave is not None:
+        return xo, up[:n], wp[:n]
+    return xo, up[:n]
 
 
 def gen_jittered_lattice(n, seed, dim=3, jitter=0.3, device="cuda"):

@@ -423,7 +423,7 @@ int ppmg_rmsh_remesh(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, co
 
 int ppmg_rmsh_create_part(const ppmg_mesh* mesh, int prec, const ppmg_field* field, const double* cutoff,
                           const double* cutoff_weights, int resetpos, int64_t maxnp, int64_t* np_out, void* xp,
-                          void* up, ppmg_plan* plan, void* stream) {
+                          void* up, const ppmg_field* slave, void* wp, ppmg_plan* plan, void* stream) {
     clear_error();
     if (check_plan(plan)) return PPMG_ERROR_ERROR;
     if (!cutoff || !np_out) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "cutoff or np_out is NULL");
@@ -440,8 +440,18 @@ int ppmg_rmsh_create_part(const ppmg_mesh* mesh, int prec, const ppmg_field* fie
     }
     if (maxnp > 0 && (!up || (resetpos && !xp)))
         return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "output arrays are NULL");
+    FieldGeom fw;
+    if (slave) {
+        if (make_field(slave, m, &fw)) return g_status.info;
+        for (int d = 0; d < m.dim; ++d) {
+            int need_hi = m.upper_open[d] ? m.N[d] - 1 : m.N[d];
+            if (fw.lo[d] > 0 || fw.lo[d] + fw.ext[d] - 1 < need_hi)
+                return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "slave field does not store the real nodes");
+        }
+        if (maxnp > 0 && !wp) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "wp is NULL");
+    }
     if (create_part_general(plan, m, prec, fg, field->data, cutoff, cutoff_weights, resetpos, maxnp, np_out, xp, up,
-                            as_stream(stream)))
+                            slave ? &fw : nullptr, slave ? slave->data : nullptr, wp, as_stream(stream)))
         return g_status.info;
     if (*np_out > maxnp)
         return fail(PPMG_ERROR_ERROR, PPMG_ERR_ALLOC, "output capacity smaller than the number of new particles");

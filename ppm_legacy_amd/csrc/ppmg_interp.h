@@ -16,7 +16,7 @@ int comp_weights_general(const DevMesh& m, int kernel, int prec, int64_t np, con
 int impose_bc_general(const DevMesh& m, int prec, int64_t np, void* xp, hipStream_t s);
 int create_part_general(ppmg_plan* plan, const DevMesh& m, int prec, const FieldGeom& fg, const void* field,
                         const double* cutoff, const double* cw, int resetpos, int64_t maxnp, int64_t* np_out,
-                        void* xp, void* up, hipStream_t s);
+                        void* xp, void* up, const FieldGeom* fw, const void* wfield, void* wp, hipStream_t s);
 
 // 3D M'4 tiled deposit (ppmg_p2m3d.hip): fully periodic mesh, compact layout.
 bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg);

@@ -366,7 +366,7 @@ template <typename PosT, typename ValT>
 __global__ void __launch_bounds__(256)
 k_cp_write(DevMesh m, FieldGeom f, const ValT* __restrict__ field, CPArgs a, int64_t nn,
            const int* __restrict__ flag, const int* __restrict__ pos, PosT* __restrict__ xp,
-           ValT* __restrict__ up) {
+           ValT* __restrict__ up, FieldGeom fw, const ValT* __restrict__ wfield, ValT* __restrict__ wp) {
 #pragma clang fp contract(off)
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nn || !flag[t]) return;
@@ -379,6 +379,12 @@ k_cp_write(DevMesh m, FieldGeom f, const ValT* __restrict__ field, CPArgs a, int
     idx[2] = (int)(r / a.nn[1]);
     int64_t off = ((int64_t)(idx[0] - f.lo[0]) + (int64_t)f.ext[0] * ((idx[1] - f.lo[1]) + (int64_t)f.ext[1] * (idx[2] - f.lo[2]))) * f.lda;
     for (int c = 0; c < f.lda; ++c) up[q * f.lda + c] = field[off + c];
+    if (wfield) {
+        // slave field: its values at the same node (ppm_rmsh_create_part.f:626-632)
+        int64_t woff = ((int64_t)(idx[0] - fw.lo[0]) +
+                        (int64_t)fw.ext[0] * ((idx[1] - fw.lo[1]) + (int64_t)fw.ext[1] * (idx[2] - fw.lo[2]))) * fw.lda;
+        for (int c = 0; c < fw.lda; ++c) wp[q * fw.lda + c] = wfield[woff + c];
+    }
     if (a.resetpos) {
         for (int d = 0; d < m.dim; ++d) {
             PosT dx = ((PosT)m.maxv[d] - (PosT)m.minv[d]) / (PosT)m.N[d];
@@ -543,7 +549,8 @@ int impose_bc_general(const DevMesh& m, int prec, int64_t np, void* xp, hipStrea
 
 template <typename PosT, typename ValT>
 static int create_part_typed(ppmg_plan* plan, const DevMesh& m, const FieldGeom& fg, const ValT* field,
-                             const CPArgs& a, int64_t* np_out, PosT* xp, ValT* up, hipStream_t s) {
+                             const CPArgs& a, int64_t* np_out, PosT* xp, ValT* up, const FieldGeom& fw,
+                             const ValT* wfield, ValT* wp, hipStream_t s) {
     int64_t nn = (int64_t)a.nn[0] * a.nn[1] * a.nn[2];
     if (plan->misc.ensure(sizeof(int) * (size_t)(nn + 1)) || plan->misc2.ensure(sizeof(int) * (size_t)(nn + 1)))
         return PPMG_ERROR_FATAL;
@@ -559,7 +566,8 @@ static int create_part_typed(ppmg_plan* plan, const DevMesh& m, const FieldGeom&
         PPMG_HIP(hipMemcpyAsync(&last[0], pos + nn - 1, sizeof(int), hipMemcpyDeviceToHost, s));
         PPMG_HIP(hipMemcpyAsync(&last[1], flag + nn - 1, sizeof(int), hipMemcpyDeviceToHost, s));
     }
-    hipLaunchKernelGGL((k_cp_write<PosT, ValT>), dim3(g), dim3(256), 0, s, m, fg, field, a, nn, flag, pos, xp, up);
+    hipLaunchKernelGGL((k_cp_write<PosT, ValT>), dim3(g), dim3(256), 0, s, m, fg, field, a, nn, flag, pos, xp, up, fw,
+                       wfield, wp);
     PPMG_HIP(hipGetLastError());
     PPMG_HIP(hipStreamSynchronize(s));
     *np_out = (int64_t)last[0] + last[1];
@@ -568,7 +576,9 @@ static int create_part_typed(ppmg_plan* plan, const DevMesh& m, const FieldGeom&
 
 int create_part_general(ppmg_plan* plan, const DevMesh& m, int prec, const FieldGeom& fg, const void* field,
                         const double* cutoff, const double* cw, int resetpos, int64_t maxnp, int64_t* np_out,
-                        void* xp, void* up, hipStream_t s) {
+                        void* xp, void* up, const FieldGeom* fw, const void* wfield, void* wp, hipStream_t s) {
+    FieldGeom fws = fw ? *fw : fg;
+    if (!fw) wfield = nullptr;
     CPArgs a{};
     for (int d = 0; d < 3; ++d) a.nn[d] = 1;
     for (int d = 0; d < m.dim; ++d) a.nn[d] = m.upper_open[d] ? m.N[d] : m.N[d] + 1;
@@ -580,10 +590,13 @@ int create_part_general(ppmg_plan* plan, const DevMesh& m, int prec, const Field
     a.resetpos = resetpos;
     a.maxnp = maxnp;
     if (prec == PPMG_PREC_F64)
-        return create_part_typed<double, double>(plan, m, fg, (const double*)field, a, np_out, (double*)xp, (double*)up, s);
+        return create_part_typed<double, double>(plan, m, fg, (const double*)field, a, np_out, (double*)xp, (double*)up,
+                                                 fws, (const double*)wfield, (double*)wp, s);
     if (prec == PPMG_PREC_F32C)
-        return create_part_typed<double, float>(plan, m, fg, (const float*)field, a, np_out, (double*)xp, (float*)up, s);
-    return create_part_typed<float, float>(plan, m, fg, (const float*)field, a, np_out, (float*)xp, (float*)up, s);
+        return create_part_typed<double, float>(plan, m, fg, (const float*)field, a, np_out, (double*)xp, (float*)up,
+                                                fws, (const float*)wfield, (float*)wp, s);
+    return create_part_typed<float, float>(plan, m, fg, (const float*)field, a, np_out, (float*)xp, (float*)up, fws,
+                                           (const float*)wfield, (float*)wp, s);
 }
 
 }  // namespace ppmg

@@ -215,7 +215,8 @@ int ppmg_impose_part_bc_reg(void* xp, int np, int topo_id, int prec, int mem) {
 
 int ppmg_rmsh_create_part_reg(const void* field_up, int lda, int topo_id, int mesh_id, const int* ghostsize,
                               const double* cutoff, const double* cutoff_weights, int resetpos, int maxnp,
-                              int* np_out, void* xp, void* up, int prec, int mem) {
+                              int* np_out, void* xp, void* up, const void* field_wp, int lda2, void* wp,
+                              int prec, int mem) {
     clear_error();
     Topo t;
     if (lookup(topo_id, mesh_id, &t)) return current_info();
@@ -235,9 +236,20 @@ int ppmg_rmsh_create_part_reg(const void* field_up, int lda, int topo_id, int me
     if (stage_out_ptr(g_stage.xp, xp, esz * (size_t)(maxnp > 0 ? maxnp : 1) * dim, mem, &dxp)) return current_info();
     if (stage_out_ptr(g_stage.up, up, esz * (size_t)(maxnp > 0 ? maxnp : 1) * lda, mem, &dup)) return current_info();
     f.data = const_cast<void*>(dfield);
+    ppmg_field fw{};
+    void* dwp = nullptr;
+    if (field_wp) {
+        if (lda2 < 1) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "lda2 must be >= 1");
+        fw = padded_field(t, lda2, g, nullptr);
+        const void* dw = nullptr;
+        if (stage_in(g_stage.conv_field, field_wp, esz * (size_t)field_count(fw), mem, &dw)) return current_info();
+        if (stage_out_ptr(g_stage.w[0], wp, esz * (size_t)(maxnp > 0 ? maxnp : 1) * lda2, mem, &dwp))
+            return current_info();
+        fw.data = const_cast<void*>(dw);
+    }
     int64_t n = 0;
     int rc = ppmg_rmsh_create_part(&t.mesh, single ? PPMG_PREC_F32 : PPMG_PREC_F64, &f, cutoff, cutoff_weights,
-                                   resetpos, maxnp, &n, dxp, dup, plan, nullptr);
+                                   resetpos, maxnp, &n, dxp, dup, field_wp ? &fw : nullptr, dwp, plan, nullptr);
     *np_out = (int)n;
     if (rc && !(maxnp == 0 && n > 0)) return rc;
     clear_error();
@@ -245,6 +257,7 @@ int ppmg_rmsh_create_part_reg(const void* field_up, int lda, int topo_id, int me
         int64_t w = n < maxnp ? n : maxnp;
         if (w > 0) {
             PPMG_HIP(hipMemcpy(up, dup, esz * (size_t)w * lda, hipMemcpyDeviceToHost));
+            if (field_wp) PPMG_HIP(hipMemcpy(wp, dwp, esz * (size_t)w * lda2, hipMemcpyDeviceToHost));
             if (resetpos) PPMG_HIP(hipMemcpy(xp, dxp, esz * (size_t)w * dim, hipMemcpyDeviceToHost));
         }
     }

@@ -97,10 +97,11 @@ MODULE ppm_module_ppmg
       INTEGER(C_INT) :: ghostsize(*)
     END FUNCTION
     INTEGER(C_INT) FUNCTION ppmg_rmsh_create_part_reg(field_up, lda, topo_id, mesh_id, ghostsize, cutoff, &
-        cutoff_weights, resetpos, maxnp, np_out, xp, up, prec, mem) BIND(C, NAME='ppmg_rmsh_create_part_reg')
+        cutoff_weights, resetpos, maxnp, np_out, xp, up, field_wp, lda2, wp, prec, mem) &
+        BIND(C, NAME='ppmg_rmsh_create_part_reg')
       IMPORT :: C_INT, C_PTR, C_DOUBLE
-      TYPE(C_PTR), VALUE :: field_up, cutoff_weights, xp, up
-      INTEGER(C_INT), VALUE :: lda, topo_id, mesh_id, resetpos, maxnp, prec, mem
+      TYPE(C_PTR), VALUE :: field_up, cutoff_weights, xp, up, field_wp, wp
+      INTEGER(C_INT), VALUE :: lda, topo_id, mesh_id, resetpos, maxnp, lda2, prec, mem
       INTEGER(C_INT) :: ghostsize(*), np_out
       REAL(C_DOUBLE) :: cutoff(*)
     END FUNCTION

@@ -119,7 +119,8 @@ CONTAINS
     INTEGER, INTENT(IN) :: np, topo_id, mesh_id, kernel
     INTEGER, INTENT(OUT) :: info
     info = ppmg_rmsh_comp_weights_reg(C_LOC(xp(1)), INT(np, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
-                                      INT(kernel, C_INT), C_LOC(w1(1)), C_LOC(w2(1)), C_LOC(w3(1)), ppmg_prec_f32, ppmg_mem_host)
+                                      INT(kernel, C_INT), C_LOC(w1(1)), C_LOC(w2(1)), C_LOC(w3(1)), ppmg_prec_f32, &
+                                          ppmg_mem_host)
     CALL ppmg_report('ppm_rmsh_comp_weights', info)
   END SUBROUTINE cw_core_s
 
@@ -263,7 +264,8 @@ CONTAINS
     INTEGER, INTENT(IN) :: np, topo_id, mesh_id, kernel
     INTEGER, INTENT(OUT) :: info
     info = ppmg_rmsh_comp_weights_reg(C_LOC(xp(1)), INT(np, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
-                                      INT(kernel, C_INT), C_LOC(w1(1)), C_LOC(w2(1)), C_LOC(w3(1)), ppmg_prec_f64, ppmg_mem_host)
+                                      INT(kernel, C_INT), C_LOC(w1(1)), C_LOC(w2(1)), C_LOC(w3(1)), ppmg_prec_f64, &
+                                          ppmg_mem_host)
     CALL ppmg_report('ppm_rmsh_comp_weights', info)
   END SUBROUTINE cw_core_d
 
@@ -352,7 +354,8 @@ MODULE ppm_module_rmsh_remesh
   END INTERFACE
 CONTAINS
 
-  SUBROUTINE ppm_rmsh_remesh_ss_2d(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, wx2_user, wx3_user)
+  SUBROUTINE ppm_rmsh_remesh_ss_2d(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, &
+      wx2_user, wx3_user)
     INTEGER, PARAMETER :: MK = ppm_kind_single
     REAL(MK), DIMENSION(:,:), POINTER :: xp
     REAL(MK), DIMENSION(:), POINTER :: up
@@ -394,7 +397,8 @@ CONTAINS
                        Np, lda, topo_id, mesh_id, kernel, cg, info)
   END SUBROUTINE ppm_rmsh_remesh_ss_2d
 
-  SUBROUTINE ppm_rmsh_remesh_ds_2d(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, wx2_user, wx3_user)
+  SUBROUTINE ppm_rmsh_remesh_ds_2d(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, &
+      wx2_user, wx3_user)
     INTEGER, PARAMETER :: MK = ppm_kind_double
     REAL(MK), DIMENSION(:,:), POINTER :: xp
     REAL(MK), DIMENSION(:), POINTER :: up
@@ -436,7 +440,8 @@ CONTAINS
                        Np, lda, topo_id, mesh_id, kernel, cg, info)
   END SUBROUTINE ppm_rmsh_remesh_ds_2d
 
-  SUBROUTINE ppm_rmsh_remesh_sv_2d(xp, Np, up, lda, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, wx2_user, wx3_user)
+  SUBROUTINE ppm_rmsh_remesh_sv_2d(xp, Np, up, lda, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, &
+      wx2_user, wx3_user)
     INTEGER, PARAMETER :: MK = ppm_kind_single
     REAL(MK), DIMENSION(:,:), POINTER :: xp
     REAL(MK), DIMENSION(:,:), POINTER :: up
@@ -478,7 +483,8 @@ CONTAINS
                        Np, lda, topo_id, mesh_id, kernel, cg, info)
   END SUBROUTINE ppm_rmsh_remesh_sv_2d
 
-  SUBROUTINE ppm_rmsh_remesh_dv_2d(xp, Np, up, lda, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, wx2_user, wx3_user)
+  SUBROUTINE ppm_rmsh_remesh_dv_2d(xp, Np, up, lda, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, &
+      wx2_user, wx3_user)
     INTEGER, PARAMETER :: MK = ppm_kind_double
     REAL(MK), DIMENSION(:,:), POINTER :: xp
     REAL(MK), DIMENSION(:,:), POINTER :: up
@@ -520,7 +526,8 @@ CONTAINS
                        Np, lda, topo_id, mesh_id, kernel, cg, info)
   END SUBROUTINE ppm_rmsh_remesh_dv_2d
 
-  SUBROUTINE ppm_rmsh_remesh_ss_3d(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, wx2_user, wx3_user)
+  SUBROUTINE ppm_rmsh_remesh_ss_3d(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, &
+      wx2_user, wx3_user)
     INTEGER, PARAMETER :: MK = ppm_kind_single
     REAL(MK), DIMENSION(:,:), POINTER :: xp
     REAL(MK), DIMENSION(:), POINTER :: up
@@ -562,7 +569,8 @@ CONTAINS
                        Np, lda, topo_id, mesh_id, kernel, cg, info)
   END SUBROUTINE ppm_rmsh_remesh_ss_3d
 
-  SUBROUTINE ppm_rmsh_remesh_ds_3d(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, wx2_user, wx3_user)
+  SUBROUTINE ppm_rmsh_remesh_ds_3d(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, &
+      wx2_user, wx3_user)
     INTEGER, PARAMETER :: MK = ppm_kind_double
     REAL(MK), DIMENSION(:,:), POINTER :: xp
     REAL(MK), DIMENSION(:), POINTER :: up
@@ -604,7 +612,8 @@ CONTAINS
                        Np, lda, topo_id, mesh_id, kernel, cg, info)
   END SUBROUTINE ppm_rmsh_remesh_ds_3d
 
-  SUBROUTINE ppm_rmsh_remesh_sv_3d(xp, Np, up, lda, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, wx2_user, wx3_user)
+  SUBROUTINE ppm_rmsh_remesh_sv_3d(xp, Np, up, lda, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, &
+      wx2_user, wx3_user)
     INTEGER, PARAMETER :: MK = ppm_kind_single
     REAL(MK), DIMENSION(:,:), POINTER :: xp
     REAL(MK), DIMENSION(:,:), POINTER :: up
@@ -646,7 +655,8 @@ CONTAINS
                        Np, lda, topo_id, mesh_id, kernel, cg, info)
   END SUBROUTINE ppm_rmsh_remesh_sv_3d
 
-  SUBROUTINE ppm_rmsh_remesh_dv_3d(xp, Np, up, lda, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, wx2_user, wx3_user)
+  SUBROUTINE ppm_rmsh_remesh_dv_3d(xp, Np, up, lda, topo_id, mesh_id, kernel, ghostsize, info, field_up, wx1_user, &
+      wx2_user, wx3_user)
     INTEGER, PARAMETER :: MK = ppm_kind_double
     REAL(MK), DIMENSION(:,:), POINTER :: xp
     REAL(MK), DIMENSION(:,:), POINTER :: up
@@ -689,3 +699,1661 @@ CONTAINS
   END SUBROUTINE ppm_rmsh_remesh_dv_3d
 
 END MODULE ppm_module_rmsh_remesh
+
+
+MODULE ppm_module_rmsh_create_part
+  USE ISO_C_BINDING
+  USE ppm_module_ppmg
+  USE ppmg_fcore
+  IMPLICIT NONE
+  PRIVATE
+  PUBLIC :: ppm_rmsh_create_part
+  INTERFACE ppm_rmsh_create_part
+    MODULE PROCEDURE ppm_rmsh_create_part_sss_2d
+    MODULE PROCEDURE ppm_rmsh_create_part_ssv_2d
+    MODULE PROCEDURE ppm_rmsh_create_part_svs_2d
+    MODULE PROCEDURE ppm_rmsh_create_part_svv_2d
+    MODULE PROCEDURE ppm_rmsh_create_part_dss_2d
+    MODULE PROCEDURE ppm_rmsh_create_part_dsv_2d
+    MODULE PROCEDURE ppm_rmsh_create_part_dvs_2d
+    MODULE PROCEDURE ppm_rmsh_create_part_dvv_2d
+    MODULE PROCEDURE ppm_rmsh_create_part_sss_3d
+    MODULE PROCEDURE ppm_rmsh_create_part_ssv_3d
+    MODULE PROCEDURE ppm_rmsh_create_part_svs_3d
+    MODULE PROCEDURE ppm_rmsh_create_part_svv_3d
+    MODULE PROCEDURE ppm_rmsh_create_part_dss_3d
+    MODULE PROCEDURE ppm_rmsh_create_part_dsv_3d
+    MODULE PROCEDURE ppm_rmsh_create_part_dvs_3d
+    MODULE PROCEDURE ppm_rmsh_create_part_dvv_3d
+  END INTERFACE
+CONTAINS
+
+  SUBROUTINE cp_core_s(field, fieldw, slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, maxnp, n, &
+                       pxp, pup, pwp, info)
+    REAL(C_FLOAT), TARGET, INTENT(IN) :: field(*), fieldw(*)
+    LOGICAL, INTENT(IN) :: slave, reset
+    INTEGER, INTENT(IN) :: lda, lda2, topo_id, mesh_id, maxnp
+    INTEGER(C_INT), INTENT(IN) :: cg(3)
+    REAL(C_DOUBLE), INTENT(IN) :: cut(2)
+    TYPE(C_PTR), INTENT(IN) :: cwp, pxp, pup, pwp
+    INTEGER, INTENT(OUT) :: n, info
+    INTEGER(C_INT) :: cn, rs
+    TYPE(C_PTR) :: pfw
+    REAL(C_DOUBLE) :: c2(2)
+    pfw = C_NULL_PTR
+    IF (slave) pfw = C_LOC(fieldw(1))
+    rs = 0
+    IF (reset) rs = 1
+    c2 = cut
+    info = ppmg_rmsh_create_part_reg(C_LOC(field(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), cg, &
+                                     c2, cwp, rs, INT(maxnp, C_INT), cn, pxp, pup, pfw, INT(lda2, C_INT), pwp, &
+                                     ppmg_prec_f32, ppmg_mem_host)
+    n = cn
+    CALL ppmg_report('ppm_rmsh_create_part', info)
+  END SUBROUTINE cp_core_s
+
+
+  SUBROUTINE cp_core_d(field, fieldw, slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, maxnp, n, &
+                       pxp, pup, pwp, info)
+    REAL(C_DOUBLE), TARGET, INTENT(IN) :: field(*), fieldw(*)
+    LOGICAL, INTENT(IN) :: slave, reset
+    INTEGER, INTENT(IN) :: lda, lda2, topo_id, mesh_id, maxnp
+    INTEGER(C_INT), INTENT(IN) :: cg(3)
+    REAL(C_DOUBLE), INTENT(IN) :: cut(2)
+    TYPE(C_PTR), INTENT(IN) :: cwp, pxp, pup, pwp
+    INTEGER, INTENT(OUT) :: n, info
+    INTEGER(C_INT) :: cn, rs
+    TYPE(C_PTR) :: pfw
+    REAL(C_DOUBLE) :: c2(2)
+    pfw = C_NULL_PTR
+    IF (slave) pfw = C_LOC(fieldw(1))
+    rs = 0
+    IF (reset) rs = 1
+    c2 = cut
+    info = ppmg_rmsh_create_part_reg(C_LOC(field(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), cg, &
+                                     c2, cwp, rs, INT(maxnp, C_INT), cn, pxp, pup, pfw, INT(lda2, C_INT), pwp, &
+                                     ppmg_prec_f64, ppmg_mem_host)
+    n = cn
+    CALL ppmg_report('ppm_rmsh_create_part', info)
+  END SUBROUTINE cp_core_d
+
+
+  SUBROUTINE ppm_rmsh_create_part_sss_2d(xp, np, up, field_up, topo_id, mesh_id, cutoff, info, resetpos, field_wp, wp)
+    INTEGER, PARAMETER :: MK = ppm_kind_single
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:), POINTER :: up
+    REAL(MK), DIMENSION(:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), DIMENSION(:,:,:), POINTER, OPTIONAL :: field_wp
+    REAL(MK), DIMENSION(:), POINTER, OPTIONAL :: wp
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda = 1
+    INTEGER, PARAMETER :: lda2 = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = PRESENT(field_wp)
+    CALL ppmg_box(caller, topo_id, (/0, 0/), 2, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 1)
+    lo(1) = LBOUND(field_up, 1)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 2)
+    lo(2) = LBOUND(field_up, 2)
+    hi(2) = hi(2) + cg(2)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .FALSE., lda, 2, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .FALSE., lda2, 2, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (slave) THEN
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_wp(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_up(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(2, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1))
+    IF (slave) pwp = C_LOC(wp(1))
+    IF (slave) THEN
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_wp(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_up(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_sss_2d
+
+  SUBROUTINE ppm_rmsh_create_part_ssv_2d(xp, np, up, field_up, topo_id, mesh_id, cutoff, info, resetpos, field_wp, wp, &
+      lda2)
+    INTEGER, PARAMETER :: MK = ppm_kind_single
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:), POINTER :: up
+    REAL(MK), DIMENSION(:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_wp
+    REAL(MK), DIMENSION(:,:), POINTER :: wp
+    INTEGER, INTENT(IN) :: lda2
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = .TRUE.
+    CALL ppmg_box(caller, topo_id, (/0, 0/), 2, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 1)
+    lo(1) = LBOUND(field_up, 1)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 2)
+    lo(2) = LBOUND(field_up, 2)
+    hi(2) = hi(2) + cg(2)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .FALSE., lda, 2, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .TRUE., lda2, 2, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (slave) THEN
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), 1), slave, lda, &
+          lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_up(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(lda2, n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(2, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1))
+    IF (slave) pwp = C_LOC(wp(1, 1))
+    IF (slave) THEN
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), 1), slave, lda, &
+          lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_up(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_ssv_2d
+
+  SUBROUTINE ppm_rmsh_create_part_svs_2d(xp, np, up, lda, field_up, topo_id, mesh_id, cutoff, info, resetpos, &
+      cutoff_weights, field_wp, wp)
+    INTEGER, PARAMETER :: MK = ppm_kind_single
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:,:), POINTER :: up
+    INTEGER, INTENT(IN) :: lda
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), OPTIONAL, DIMENSION(:), POINTER :: cutoff_weights
+    REAL(MK), DIMENSION(:,:,:), POINTER, OPTIONAL :: field_wp
+    REAL(MK), DIMENSION(:), POINTER, OPTIONAL :: wp
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda2 = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = PRESENT(field_wp)
+    CALL ppmg_box(caller, topo_id, (/0, 0/), 2, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 2)
+    lo(1) = LBOUND(field_up, 2)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 3)
+    lo(2) = LBOUND(field_up, 3)
+    hi(2) = hi(2) + cg(2)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .TRUE., lda, 2, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .FALSE., lda2, 2, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (PRESENT(cutoff_weights)) THEN
+      IF (.NOT. ASSOCIATED(cutoff_weights)) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights is not allocated', info)
+        RETURN
+      END IF
+      IF (SIZE(cutoff_weights) < lda) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights too short', info)
+        RETURN
+      END IF
+      IF (ANY(cutoff_weights(1:lda) < 0.0_MK) .OR. ANY(cutoff_weights(1:lda) > 1.0_MK)) THEN
+        CALL ppmg_argfail(caller, '0 <= cutoff_weights <= 1 is violated.', info)
+        RETURN
+      END IF
+      ALLOCATE(cw(lda))
+      cw = REAL(cutoff_weights(1:lda), C_DOUBLE)
+      cwp = C_LOC(cw(1))
+    END IF
+    IF (slave) THEN
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_wp(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, &
+          lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(lda, n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(2, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1, 1))
+    IF (slave) pwp = C_LOC(wp(1))
+    IF (slave) THEN
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_wp(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, &
+          lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_svs_2d
+
+  SUBROUTINE ppm_rmsh_create_part_svv_2d(xp, np, up, lda, field_up, topo_id, mesh_id, cutoff, info, resetpos, &
+      cutoff_weights, field_wp, wp, lda2)
+    INTEGER, PARAMETER :: MK = ppm_kind_single
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:,:), POINTER :: up
+    INTEGER, INTENT(IN) :: lda
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), OPTIONAL, DIMENSION(:), POINTER :: cutoff_weights
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_wp
+    REAL(MK), DIMENSION(:,:), POINTER :: wp
+    INTEGER, INTENT(IN) :: lda2
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = .TRUE.
+    CALL ppmg_box(caller, topo_id, (/0, 0/), 2, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 2)
+    lo(1) = LBOUND(field_up, 2)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 3)
+    lo(2) = LBOUND(field_up, 3)
+    hi(2) = hi(2) + cg(2)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .TRUE., lda, 2, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .TRUE., lda2, 2, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (PRESENT(cutoff_weights)) THEN
+      IF (.NOT. ASSOCIATED(cutoff_weights)) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights is not allocated', info)
+        RETURN
+      END IF
+      IF (SIZE(cutoff_weights) < lda) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights too short', info)
+        RETURN
+      END IF
+      IF (ANY(cutoff_weights(1:lda) < 0.0_MK) .OR. ANY(cutoff_weights(1:lda) > 1.0_MK)) THEN
+        CALL ppmg_argfail(caller, '0 <= cutoff_weights <= 1 is violated.', info)
+        RETURN
+      END IF
+      ALLOCATE(cw(lda))
+      cw = REAL(cutoff_weights(1:lda), C_DOUBLE)
+      cwp = C_LOC(cw(1))
+    END IF
+    IF (slave) THEN
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(lda, n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(lda2, n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(2, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1, 1))
+    IF (slave) pwp = C_LOC(wp(1, 1))
+    IF (slave) THEN
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_svv_2d
+
+  SUBROUTINE ppm_rmsh_create_part_dss_2d(xp, np, up, field_up, topo_id, mesh_id, cutoff, info, resetpos, field_wp, wp)
+    INTEGER, PARAMETER :: MK = ppm_kind_double
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:), POINTER :: up
+    REAL(MK), DIMENSION(:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), DIMENSION(:,:,:), POINTER, OPTIONAL :: field_wp
+    REAL(MK), DIMENSION(:), POINTER, OPTIONAL :: wp
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda = 1
+    INTEGER, PARAMETER :: lda2 = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = PRESENT(field_wp)
+    CALL ppmg_box(caller, topo_id, (/0, 0/), 2, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 1)
+    lo(1) = LBOUND(field_up, 1)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 2)
+    lo(2) = LBOUND(field_up, 2)
+    hi(2) = hi(2) + cg(2)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .FALSE., lda, 2, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .FALSE., lda2, 2, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (slave) THEN
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_wp(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_up(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(2, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1))
+    IF (slave) pwp = C_LOC(wp(1))
+    IF (slave) THEN
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_wp(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_up(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_dss_2d
+
+  SUBROUTINE ppm_rmsh_create_part_dsv_2d(xp, np, up, field_up, topo_id, mesh_id, cutoff, info, resetpos, field_wp, wp, &
+      lda2)
+    INTEGER, PARAMETER :: MK = ppm_kind_double
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:), POINTER :: up
+    REAL(MK), DIMENSION(:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_wp
+    REAL(MK), DIMENSION(:,:), POINTER :: wp
+    INTEGER, INTENT(IN) :: lda2
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = .TRUE.
+    CALL ppmg_box(caller, topo_id, (/0, 0/), 2, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 1)
+    lo(1) = LBOUND(field_up, 1)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 2)
+    lo(2) = LBOUND(field_up, 2)
+    hi(2) = hi(2) + cg(2)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .FALSE., lda, 2, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .TRUE., lda2, 2, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (slave) THEN
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), 1), slave, lda, &
+          lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_up(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(lda2, n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(2, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1))
+    IF (slave) pwp = C_LOC(wp(1, 1))
+    IF (slave) THEN
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), 1), slave, lda, &
+          lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), 1), field_up(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, lda2, &
+          topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_dsv_2d
+
+  SUBROUTINE ppm_rmsh_create_part_dvs_2d(xp, np, up, lda, field_up, topo_id, mesh_id, cutoff, info, resetpos, &
+      cutoff_weights, field_wp, wp)
+    INTEGER, PARAMETER :: MK = ppm_kind_double
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:,:), POINTER :: up
+    INTEGER, INTENT(IN) :: lda
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), OPTIONAL, DIMENSION(:), POINTER :: cutoff_weights
+    REAL(MK), DIMENSION(:,:,:), POINTER, OPTIONAL :: field_wp
+    REAL(MK), DIMENSION(:), POINTER, OPTIONAL :: wp
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda2 = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = PRESENT(field_wp)
+    CALL ppmg_box(caller, topo_id, (/0, 0/), 2, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 2)
+    lo(1) = LBOUND(field_up, 2)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 3)
+    lo(2) = LBOUND(field_up, 3)
+    hi(2) = hi(2) + cg(2)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .TRUE., lda, 2, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .FALSE., lda2, 2, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (PRESENT(cutoff_weights)) THEN
+      IF (.NOT. ASSOCIATED(cutoff_weights)) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights is not allocated', info)
+        RETURN
+      END IF
+      IF (SIZE(cutoff_weights) < lda) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights too short', info)
+        RETURN
+      END IF
+      IF (ANY(cutoff_weights(1:lda) < 0.0_MK) .OR. ANY(cutoff_weights(1:lda) > 1.0_MK)) THEN
+        CALL ppmg_argfail(caller, '0 <= cutoff_weights <= 1 is violated.', info)
+        RETURN
+      END IF
+      ALLOCATE(cw(lda))
+      cw = REAL(cutoff_weights(1:lda), C_DOUBLE)
+      cwp = C_LOC(cw(1))
+    END IF
+    IF (slave) THEN
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_wp(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, &
+          lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(lda, n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(2, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1, 1))
+    IF (slave) pwp = C_LOC(wp(1))
+    IF (slave) THEN
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_wp(lo(1):hi(1), lo(2):hi(2), 1), slave, lda, &
+          lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_dvs_2d
+
+  SUBROUTINE ppm_rmsh_create_part_dvv_2d(xp, np, up, lda, field_up, topo_id, mesh_id, cutoff, info, resetpos, &
+      cutoff_weights, field_wp, wp, lda2)
+    INTEGER, PARAMETER :: MK = ppm_kind_double
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:,:), POINTER :: up
+    INTEGER, INTENT(IN) :: lda
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), OPTIONAL, DIMENSION(:), POINTER :: cutoff_weights
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_wp
+    REAL(MK), DIMENSION(:,:), POINTER :: wp
+    INTEGER, INTENT(IN) :: lda2
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = .TRUE.
+    CALL ppmg_box(caller, topo_id, (/0, 0/), 2, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 2)
+    lo(1) = LBOUND(field_up, 2)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 3)
+    lo(2) = LBOUND(field_up, 3)
+    hi(2) = hi(2) + cg(2)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .TRUE., lda, 2, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .TRUE., lda2, 2, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (PRESENT(cutoff_weights)) THEN
+      IF (.NOT. ASSOCIATED(cutoff_weights)) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights is not allocated', info)
+        RETURN
+      END IF
+      IF (SIZE(cutoff_weights) < lda) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights too short', info)
+        RETURN
+      END IF
+      IF (ANY(cutoff_weights(1:lda) < 0.0_MK) .OR. ANY(cutoff_weights(1:lda) > 1.0_MK)) THEN
+        CALL ppmg_argfail(caller, '0 <= cutoff_weights <= 1 is violated.', info)
+        RETURN
+      END IF
+      ALLOCATE(cw(lda))
+      cw = REAL(cutoff_weights(1:lda), C_DOUBLE)
+      cwp = C_LOC(cw(1))
+    END IF
+    IF (slave) THEN
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(lda, n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(lda2, n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(2, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1, 1))
+    IF (slave) pwp = C_LOC(wp(1, 1))
+    IF (slave) THEN
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), field_up(1:lda, lo(1):hi(1), lo(2):hi(2), 1), &
+          slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_dvv_2d
+
+  SUBROUTINE ppm_rmsh_create_part_sss_3d(xp, np, up, field_up, topo_id, mesh_id, cutoff, info, resetpos, field_wp, wp)
+    INTEGER, PARAMETER :: MK = ppm_kind_single
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:), POINTER :: up
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), DIMENSION(:,:,:,:), POINTER, OPTIONAL :: field_wp
+    REAL(MK), DIMENSION(:), POINTER, OPTIONAL :: wp
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda = 1
+    INTEGER, PARAMETER :: lda2 = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = PRESENT(field_wp)
+    CALL ppmg_box(caller, topo_id, (/0, 0, 0/), 3, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 1)
+    lo(1) = LBOUND(field_up, 1)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 2)
+    lo(2) = LBOUND(field_up, 2)
+    hi(2) = hi(2) + cg(2)
+    cg(3) = 1 - LBOUND(field_up, 3)
+    lo(3) = LBOUND(field_up, 3)
+    hi(3) = hi(3) + cg(3)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .FALSE., lda, 3, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .FALSE., lda2, 3, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (slave) THEN
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(3, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1))
+    IF (slave) pwp = C_LOC(wp(1))
+    IF (slave) THEN
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_sss_3d
+
+  SUBROUTINE ppm_rmsh_create_part_ssv_3d(xp, np, up, field_up, topo_id, mesh_id, cutoff, info, resetpos, field_wp, wp, &
+      lda2)
+    INTEGER, PARAMETER :: MK = ppm_kind_single
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:), POINTER :: up
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), DIMENSION(:,:,:,:,:), POINTER :: field_wp
+    REAL(MK), DIMENSION(:,:), POINTER :: wp
+    INTEGER, INTENT(IN) :: lda2
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = .TRUE.
+    CALL ppmg_box(caller, topo_id, (/0, 0, 0/), 3, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 1)
+    lo(1) = LBOUND(field_up, 1)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 2)
+    lo(2) = LBOUND(field_up, 2)
+    hi(2) = hi(2) + cg(2)
+    cg(3) = 1 - LBOUND(field_up, 3)
+    lo(3) = LBOUND(field_up, 3)
+    hi(3) = hi(3) + cg(3)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .FALSE., lda, 3, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .TRUE., lda2, 3, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (slave) THEN
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(lda2, n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(3, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1))
+    IF (slave) pwp = C_LOC(wp(1, 1))
+    IF (slave) THEN
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_s(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_ssv_3d
+
+  SUBROUTINE ppm_rmsh_create_part_svs_3d(xp, np, up, lda, field_up, topo_id, mesh_id, cutoff, info, resetpos, &
+      cutoff_weights, field_wp, wp)
+    INTEGER, PARAMETER :: MK = ppm_kind_single
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:,:), POINTER :: up
+    INTEGER, INTENT(IN) :: lda
+    REAL(MK), DIMENSION(:,:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), OPTIONAL, DIMENSION(:), POINTER :: cutoff_weights
+    REAL(MK), DIMENSION(:,:,:,:), POINTER, OPTIONAL :: field_wp
+    REAL(MK), DIMENSION(:), POINTER, OPTIONAL :: wp
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda2 = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = PRESENT(field_wp)
+    CALL ppmg_box(caller, topo_id, (/0, 0, 0/), 3, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 2)
+    lo(1) = LBOUND(field_up, 2)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 3)
+    lo(2) = LBOUND(field_up, 3)
+    hi(2) = hi(2) + cg(2)
+    cg(3) = 1 - LBOUND(field_up, 4)
+    lo(3) = LBOUND(field_up, 4)
+    hi(3) = hi(3) + cg(3)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .TRUE., lda, 3, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .FALSE., lda2, 3, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (PRESENT(cutoff_weights)) THEN
+      IF (.NOT. ASSOCIATED(cutoff_weights)) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights is not allocated', info)
+        RETURN
+      END IF
+      IF (SIZE(cutoff_weights) < lda) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights too short', info)
+        RETURN
+      END IF
+      IF (ANY(cutoff_weights(1:lda) < 0.0_MK) .OR. ANY(cutoff_weights(1:lda) > 1.0_MK)) THEN
+        CALL ppmg_argfail(caller, '0 <= cutoff_weights <= 1 is violated.', info)
+        RETURN
+      END IF
+      ALLOCATE(cw(lda))
+      cw = REAL(cutoff_weights(1:lda), C_DOUBLE)
+      cwp = C_LOC(cw(1))
+    END IF
+    IF (slave) THEN
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(1:lda, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(lda, n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(3, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1, 1))
+    IF (slave) pwp = C_LOC(wp(1))
+    IF (slave) THEN
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(1:lda, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_svs_3d
+
+  SUBROUTINE ppm_rmsh_create_part_svv_3d(xp, np, up, lda, field_up, topo_id, mesh_id, cutoff, info, resetpos, &
+      cutoff_weights, field_wp, wp, lda2)
+    INTEGER, PARAMETER :: MK = ppm_kind_single
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:,:), POINTER :: up
+    INTEGER, INTENT(IN) :: lda
+    REAL(MK), DIMENSION(:,:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), OPTIONAL, DIMENSION(:), POINTER :: cutoff_weights
+    REAL(MK), DIMENSION(:,:,:,:,:), POINTER :: field_wp
+    REAL(MK), DIMENSION(:,:), POINTER :: wp
+    INTEGER, INTENT(IN) :: lda2
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = .TRUE.
+    CALL ppmg_box(caller, topo_id, (/0, 0, 0/), 3, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 2)
+    lo(1) = LBOUND(field_up, 2)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 3)
+    lo(2) = LBOUND(field_up, 3)
+    hi(2) = hi(2) + cg(2)
+    cg(3) = 1 - LBOUND(field_up, 4)
+    lo(3) = LBOUND(field_up, 4)
+    hi(3) = hi(3) + cg(3)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .TRUE., lda, 3, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .TRUE., lda2, 3, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (PRESENT(cutoff_weights)) THEN
+      IF (.NOT. ASSOCIATED(cutoff_weights)) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights is not allocated', info)
+        RETURN
+      END IF
+      IF (SIZE(cutoff_weights) < lda) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights too short', info)
+        RETURN
+      END IF
+      IF (ANY(cutoff_weights(1:lda) < 0.0_MK) .OR. ANY(cutoff_weights(1:lda) > 1.0_MK)) THEN
+        CALL ppmg_argfail(caller, '0 <= cutoff_weights <= 1 is violated.', info)
+        RETURN
+      END IF
+      ALLOCATE(cw(lda))
+      cw = REAL(cutoff_weights(1:lda), C_DOUBLE)
+      cwp = C_LOC(cw(1))
+    END IF
+    IF (slave) THEN
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(1:lda2, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(1:lda, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(lda, n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(lda2, n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(3, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1, 1))
+    IF (slave) pwp = C_LOC(wp(1, 1))
+    IF (slave) THEN
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(1:lda2, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_s(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(1:lda, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_svv_3d
+
+  SUBROUTINE ppm_rmsh_create_part_dss_3d(xp, np, up, field_up, topo_id, mesh_id, cutoff, info, resetpos, field_wp, wp)
+    INTEGER, PARAMETER :: MK = ppm_kind_double
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:), POINTER :: up
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), DIMENSION(:,:,:,:), POINTER, OPTIONAL :: field_wp
+    REAL(MK), DIMENSION(:), POINTER, OPTIONAL :: wp
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda = 1
+    INTEGER, PARAMETER :: lda2 = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = PRESENT(field_wp)
+    CALL ppmg_box(caller, topo_id, (/0, 0, 0/), 3, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 1)
+    lo(1) = LBOUND(field_up, 1)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 2)
+    lo(2) = LBOUND(field_up, 2)
+    hi(2) = hi(2) + cg(2)
+    cg(3) = 1 - LBOUND(field_up, 3)
+    lo(3) = LBOUND(field_up, 3)
+    hi(3) = hi(3) + cg(3)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .FALSE., lda, 3, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .FALSE., lda2, 3, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (slave) THEN
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(3, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1))
+    IF (slave) pwp = C_LOC(wp(1))
+    IF (slave) THEN
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_dss_3d
+
+  SUBROUTINE ppm_rmsh_create_part_dsv_3d(xp, np, up, field_up, topo_id, mesh_id, cutoff, info, resetpos, field_wp, wp, &
+      lda2)
+    INTEGER, PARAMETER :: MK = ppm_kind_double
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:), POINTER :: up
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), DIMENSION(:,:,:,:,:), POINTER :: field_wp
+    REAL(MK), DIMENSION(:,:), POINTER :: wp
+    INTEGER, INTENT(IN) :: lda2
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = .TRUE.
+    CALL ppmg_box(caller, topo_id, (/0, 0, 0/), 3, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 1)
+    lo(1) = LBOUND(field_up, 1)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 2)
+    lo(2) = LBOUND(field_up, 2)
+    hi(2) = hi(2) + cg(2)
+    cg(3) = 1 - LBOUND(field_up, 3)
+    lo(3) = LBOUND(field_up, 3)
+    hi(3) = hi(3) + cg(3)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .FALSE., lda, 3, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .TRUE., lda2, 3, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (slave) THEN
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(lda2, n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(3, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1))
+    IF (slave) pwp = C_LOC(wp(1, 1))
+    IF (slave) THEN
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(1:lda2, lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_d(field_up(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_dsv_3d
+
+  SUBROUTINE ppm_rmsh_create_part_dvs_3d(xp, np, up, lda, field_up, topo_id, mesh_id, cutoff, info, resetpos, &
+      cutoff_weights, field_wp, wp)
+    INTEGER, PARAMETER :: MK = ppm_kind_double
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:,:), POINTER :: up
+    INTEGER, INTENT(IN) :: lda
+    REAL(MK), DIMENSION(:,:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), OPTIONAL, DIMENSION(:), POINTER :: cutoff_weights
+    REAL(MK), DIMENSION(:,:,:,:), POINTER, OPTIONAL :: field_wp
+    REAL(MK), DIMENSION(:), POINTER, OPTIONAL :: wp
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    INTEGER, PARAMETER :: lda2 = 1
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = PRESENT(field_wp)
+    CALL ppmg_box(caller, topo_id, (/0, 0, 0/), 3, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 2)
+    lo(1) = LBOUND(field_up, 2)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 3)
+    lo(2) = LBOUND(field_up, 3)
+    hi(2) = hi(2) + cg(2)
+    cg(3) = 1 - LBOUND(field_up, 4)
+    lo(3) = LBOUND(field_up, 4)
+    hi(3) = hi(3) + cg(3)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .TRUE., lda, 3, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .FALSE., lda2, 3, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (PRESENT(cutoff_weights)) THEN
+      IF (.NOT. ASSOCIATED(cutoff_weights)) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights is not allocated', info)
+        RETURN
+      END IF
+      IF (SIZE(cutoff_weights) < lda) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights too short', info)
+        RETURN
+      END IF
+      IF (ANY(cutoff_weights(1:lda) < 0.0_MK) .OR. ANY(cutoff_weights(1:lda) > 1.0_MK)) THEN
+        CALL ppmg_argfail(caller, '0 <= cutoff_weights <= 1 is violated.', info)
+        RETURN
+      END IF
+      ALLOCATE(cw(lda))
+      cw = REAL(cutoff_weights(1:lda), C_DOUBLE)
+      cwp = C_LOC(cw(1))
+    END IF
+    IF (slave) THEN
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(1:lda, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(lda, n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(3, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1, 1))
+    IF (slave) pwp = C_LOC(wp(1))
+    IF (slave) THEN
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(lo(1):hi(1), lo(2):hi(2), &
+          lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(1:lda, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_dvs_3d
+
+  SUBROUTINE ppm_rmsh_create_part_dvv_3d(xp, np, up, lda, field_up, topo_id, mesh_id, cutoff, info, resetpos, &
+      cutoff_weights, field_wp, wp, lda2)
+    INTEGER, PARAMETER :: MK = ppm_kind_double
+    REAL(MK), DIMENSION(:,:), POINTER :: xp
+    INTEGER, INTENT(OUT) :: np
+    REAL(MK), DIMENSION(:,:), POINTER :: up
+    INTEGER, INTENT(IN) :: lda
+    REAL(MK), DIMENSION(:,:,:,:,:), POINTER :: field_up
+    INTEGER, INTENT(IN) :: topo_id, mesh_id
+    REAL(MK), DIMENSION(2), INTENT(IN) :: cutoff
+    INTEGER, INTENT(OUT) :: info
+    LOGICAL, OPTIONAL, INTENT(IN) :: resetpos
+    REAL(MK), OPTIONAL, DIMENSION(:), POINTER :: cutoff_weights
+    REAL(MK), DIMENSION(:,:,:,:,:), POINTER :: field_wp
+    REAL(MK), DIMENSION(:,:), POINTER :: wp
+    INTEGER, INTENT(IN) :: lda2
+    INTEGER :: lo(3), hi(3), n, i
+    INTEGER(C_INT) :: cg(3)
+    LOGICAL :: reset, slave
+    REAL(C_DOUBLE) :: cut(2)
+    REAL(C_DOUBLE), DIMENSION(:), ALLOCATABLE, TARGET :: cw
+    TYPE(C_PTR) :: cwp, pxp, pup, pwp
+    CHARACTER(LEN=*), PARAMETER :: caller = 'ppm_rmsh_create_part'
+    np = 0
+    reset = .FALSE.
+    IF (PRESENT(resetpos)) reset = resetpos
+    slave = .TRUE.
+    CALL ppmg_box(caller, topo_id, (/0, 0, 0/), 3, lo, hi, cg, info)
+    IF (info /= 0) RETURN
+    IF (.NOT. ASSOCIATED(field_up)) THEN
+      CALL ppmg_argfail(caller, 'field_up is not associated', info)
+      RETURN
+    END IF
+    cg(1) = 1 - LBOUND(field_up, 2)
+    lo(1) = LBOUND(field_up, 2)
+    hi(1) = hi(1) + cg(1)
+    cg(2) = 1 - LBOUND(field_up, 3)
+    lo(2) = LBOUND(field_up, 3)
+    hi(2) = hi(2) + cg(2)
+    cg(3) = 1 - LBOUND(field_up, 4)
+    lo(3) = LBOUND(field_up, 4)
+    hi(3) = hi(3) + cg(3)
+    CALL ppmg_field_check(caller, LBOUND(field_up), UBOUND(field_up), .TRUE., lda, 3, lo, hi, info)
+    IF (info /= 0) RETURN
+    IF (slave) THEN
+      IF (.NOT. ASSOCIATED(field_wp)) THEN
+        CALL ppmg_argfail(caller, 'field_wp is not associated', info)
+        RETURN
+      END IF
+      CALL ppmg_field_check(caller, LBOUND(field_wp), UBOUND(field_wp), .TRUE., lda2, 3, lo, hi, info)
+      IF (info /= 0) RETURN
+    END IF
+    cut = REAL(cutoff, C_DOUBLE)
+    cwp = C_NULL_PTR
+    IF (PRESENT(cutoff_weights)) THEN
+      IF (.NOT. ASSOCIATED(cutoff_weights)) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights is not allocated', info)
+        RETURN
+      END IF
+      IF (SIZE(cutoff_weights) < lda) THEN
+        CALL ppmg_argfail(caller, 'cutoff_weights too short', info)
+        RETURN
+      END IF
+      IF (ANY(cutoff_weights(1:lda) < 0.0_MK) .OR. ANY(cutoff_weights(1:lda) > 1.0_MK)) THEN
+        CALL ppmg_argfail(caller, '0 <= cutoff_weights <= 1 is violated.', info)
+        RETURN
+      END IF
+      ALLOCATE(cw(lda))
+      cw = REAL(cutoff_weights(1:lda), C_DOUBLE)
+      cwp = C_LOC(cw(1))
+    END IF
+    IF (slave) THEN
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(1:lda2, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    ELSE
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(1:lda, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, 0, n, &
+                     C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, info)
+    END IF
+    IF (info /= 0) RETURN
+    IF (ASSOCIATED(up)) DEALLOCATE(up)
+    ALLOCATE(up(lda, n))
+    IF (slave) THEN
+      IF (ASSOCIATED(wp)) DEALLOCATE(wp)
+      ALLOCATE(wp(lda2, n))
+    END IF
+    IF (reset) THEN
+      IF (ASSOCIATED(xp)) DEALLOCATE(xp)
+      ALLOCATE(xp(3, n))
+    END IF
+    np = n
+    IF (n == 0) RETURN
+    pxp = C_NULL_PTR
+    pwp = C_NULL_PTR
+    IF (reset) pxp = C_LOC(xp(1, 1))
+    pup = C_LOC(up(1, 1))
+    IF (slave) pwp = C_LOC(wp(1, 1))
+    IF (slave) THEN
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_wp(1:lda2, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    ELSE
+      CALL cp_core_d(field_up(1:lda, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3), 1), field_up(1:lda, lo(1):hi(1), &
+          lo(2):hi(2), lo(3):hi(3), 1), slave, lda, lda2, topo_id, mesh_id, cg, cut, cwp, reset, n, i, &
+                     pxp, pup, pwp, info)
+    END IF
+  END SUBROUTINE ppm_rmsh_create_part_dvv_3d
+
+END MODULE ppm_module_rmsh_create_part
+
+MODULE ppm_module_rmsh
+  USE ppm_module_rmsh_comp_weights, ONLY: ppm_rmsh_comp_weights
+  USE ppm_module_rmsh_remesh
+  USE ppm_module_rmsh_create_part
+END MODULE ppm_module_rmsh

@@ -1,0 +1,153 @@
+!-------------------------------------------------------------------------------
+! ppmg_fortran_check -- a PPM client program written against the reference's
+! module and routine names, linked against libppm_fortran.a + libppmg.so.
+! Reads particles from an input stream file, runs the interpolation and
+! remeshing routines through the drop-in modules and writes every result to
+! an output stream file, which the test suite checks.
+!
+!   ppmg_fortran_check <input> <output>
+!
+! Input (stream, native endian): nm, np, lda, kernel, ghost (int32); then
+! xp(3,np), up(lda,np), xq(3,np) (float64).
+!-------------------------------------------------------------------------------
+PROGRAM ppmg_fortran_check
+  USE ppm_module_data
+  USE ppm_module_init
+  USE ppm_module_finalize
+  USE ppm_module_mktopo
+  USE ppm_module_interp_p2m
+  USE ppm_module_interp_m2p
+  USE ppm_module_rmsh
+  USE ppm_module_util_rank
+  USE ppm_module_impose_part_bc
+  IMPLICIT NONE
+  INTEGER, PARAMETER :: MK = ppm_kind_double
+  CHARACTER(LEN=1024) :: fin, fout
+  INTEGER :: nm, np, lda, kernel, g, info, topo, mesh, topo2, mesh2, decomp, nsubs, nsublist, npn, i
+  INTEGER, DIMENSION(3) :: nmv, gs
+  INTEGER, DIMENSION(6) :: bc, ngl
+  REAL(MK), DIMENSION(3) :: minp, maxp
+  REAL(MK), DIMENSION(:,:), POINTER :: xp => NULL(), up => NULL(), xq => NULL(), uq => NULL()
+  REAL(MK), DIMENSION(:,:), POINTER :: xn => NULL(), un => NULL(), wn => NULL(), x2 => NULL()
+  REAL(MK), DIMENSION(:,:,:,:,:), POINTER :: field => NULL(), field2 => NULL(), fbig => NULL()
+  REAL(MK), DIMENSION(:,:,:), POINTER :: f2d => NULL()
+  REAL(MK), DIMENSION(:), POINTER :: u1 => NULL()
+  REAL(ppm_kind_single), DIMENSION(:,:), POINTER :: xs => NULL()
+  REAL(ppm_kind_single), DIMENSION(:), POINTER :: us => NULL()
+  REAL(ppm_kind_single), DIMENSION(:,:,:,:), POINTER :: fs => NULL()
+  REAL(MK), DIMENSION(:,:), POINTER :: min_sub => NULL(), max_sub => NULL()
+  REAL(MK), DIMENSION(:), POINTER :: cost => NULL()
+  INTEGER, DIMENSION(:,:), POINTER :: istart => NULL(), ndata => NULL()
+  INTEGER, DIMENSION(:), POINTER :: sub2proc => NULL(), isublist => NULL(), lpdx => NULL(), lhbx => NULL()
+  REAL(MK), DIMENSION(2) :: cutoff
+
+  CALL GET_COMMAND_ARGUMENT(1, fin)
+  CALL GET_COMMAND_ARGUMENT(2, fout)
+  OPEN(10, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+  READ(10) nm, np, lda, kernel, g
+  ALLOCATE(xp(3, np), up(lda, np), xq(3, np), uq(lda, np))
+  READ(10) xp, up, xq
+  CLOSE(10)
+  OPEN(20, FILE=TRIM(fout), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+
+  CALL ppm_init(3, ppm_kind_double, -15, 0, 1, info)
+  CALL check(info, 'ppm_init')
+  nmv = nm
+  gs = g
+  bc = ppm_param_bcdef_periodic
+  minp = 0.0_MK
+  maxp = 1.0_MK
+  decomp = ppm_param_decomp_cuboid
+  CALL ppm_mktopo(xp, np, nmv, decomp, ppm_param_assign_internal, minp, maxp, bc, gs, topo, mesh, &
+                  min_sub, max_sub, cost, sub2proc, nsubs, isublist, nsublist, istart, ndata, info)
+  CALL check(info, 'ppm_mktopo')
+
+  ! 1. P2M, vector double (dv_3d)
+  ALLOCATE(field(lda, 1-g:nm+g, 1-g:nm+g, 1-g:nm+g, 1))
+  CALL ppm_interp_p2m(xp, np, up, lda, topo, mesh, kernel, gs, field, info)
+  CALL check(info, 'ppm_interp_p2m dv_3d')
+  WRITE(20) field
+
+  ! 2. M2P from that field at xq
+  CALL ppm_interp_m2p(xq, np, uq, lda, topo, mesh, kernel, gs, field, info)
+  CALL check(info, 'ppm_interp_m2p dv_3d')
+  WRITE(20) uq
+
+  ! 3. the same P2M into a field with wider bounds than the box (staged copy)
+  ALLOCATE(fbig(lda+1, -g:nm+g+1, -g:nm+g+1, -g:nm+g+1, 1))
+  fbig = -7.0_MK
+  CALL ppm_interp_p2m(xp, np, up, lda, topo, mesh, kernel, gs, fbig, info)
+  CALL check(info, 'ppm_interp_p2m dv_3d (wide field)')
+  WRITE(20) fbig
+
+  ! 4. comp_weights + remesh (field allocated by remesh)
+  CALL ppm_rmsh_comp_weights(xp, np, topo, mesh, kernel, info)
+  CALL check(info, 'ppm_rmsh_comp_weights')
+  CALL ppm_rmsh_remesh(xp, np, up, lda, topo, mesh, kernel, gs, info, field2)
+  CALL check(info, 'ppm_rmsh_remesh')
+  WRITE(20) LBOUND(field2), UBOUND(field2)
+  WRITE(20) field2
+
+  ! 5. create_part with a slave field (svv form): master = P2M field, slave = remeshed field
+  cutoff(1) = 0.05_MK * MAXVAL(ABS(field))
+  cutoff(2) = 0.6_MK * MAXVAL(ABS(field)) * SQRT(REAL(lda, MK))
+  CALL ppm_rmsh_create_part(xn, npn, un, lda, field, topo, mesh, cutoff, info, .TRUE., field_wp=field2, wp=wn, &
+                            lda2=lda)
+  CALL check(info, 'ppm_rmsh_create_part dvv_3d')
+  WRITE(20) npn
+  WRITE(20) cutoff
+  WRITE(20) xn, un, wn
+
+  ! 6. util_rank3d on the mesh cells with one ghost layer each side
+  ngl = 1
+  CALL ppm_util_rank3d(xp, np, minp, maxp, nmv - 1, ngl, lpdx, lhbx, info)
+  CALL check(info, 'ppm_util_rank3d')
+  WRITE(20) SIZE(lpdx), SIZE(lhbx)
+  WRITE(20) lpdx, lhbx
+
+  ! 7. P2M single precision scalar (ss_3d) of component 1
+  ALLOCATE(xs(3, np), us(np), fs(1-g:nm+g, 1-g:nm+g, 1-g:nm+g, 1))
+  xs = REAL(xp, ppm_kind_single)
+  us = REAL(up(1, :), ppm_kind_single)
+  CALL ppm_interp_p2m(xs, np, us, topo, mesh, kernel, gs, fs, info)
+  CALL check(info, 'ppm_interp_p2m ss_3d')
+  WRITE(20) fs
+
+  ! 8. impose_part_bc on shifted positions
+  ALLOCATE(x2(3, np))
+  x2 = xp + 0.75_MK
+  CALL ppm_impose_part_bc(x2, np, topo, info)
+  CALL check(info, 'ppm_impose_part_bc')
+  WRITE(20) x2
+
+  ! 9. 2D topology, P2M double scalar (ds_2d) of the xy projection
+  CALL ppm_mktopo(xp, np, nmv(1:2), decomp, ppm_param_assign_internal, minp(1:2), maxp(1:2), bc(1:4), gs(1:2), &
+                  topo2, mesh2, min_sub, max_sub, cost, sub2proc, nsubs, isublist, nsublist, istart, ndata, info)
+  CALL check(info, 'ppm_mktopo 2d')
+  ALLOCATE(f2d(1-g:nm+g, 1-g:nm+g, 1), u1(np))
+  u1 = up(1, :)
+  CALL ppm_interp_p2m(xp(1:2, :), np, u1, topo2, mesh2, kernel, gs(1:2), f2d, info)
+  CALL check(info, 'ppm_interp_p2m ds_2d')
+  WRITE(20) f2d
+
+  ! 10. error path: a particle outside the domain is the reference's fatal 405
+  xp(2, 3) = 1.5_MK
+  CALL ppm_interp_p2m(xp, np, up, lda, topo, mesh, kernel, gs, field, info)
+  WRITE(20) info
+
+  CLOSE(20)
+  CALL ppm_finalize(info)
+  PRINT '(A)', 'ppmg_fortran_check: done'
+
+CONTAINS
+
+  SUBROUTINE check(info, what)
+    INTEGER, INTENT(IN) :: info
+    CHARACTER(LEN=*), INTENT(IN) :: what
+    IF (info /= 0) THEN
+      PRINT '(A,A,A,I0)', 'ppmg_fortran_check: ', what, ' failed, info = ', info
+      STOP 2
+    END IF
+  END SUBROUTINE check
+
+END PROGRAM ppmg_fortran_check

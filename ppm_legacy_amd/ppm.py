@@ -170,8 +170,10 @@ def ppm_impose_part_bc(xp, Npart, topo_id):
     return 0
 
 
-def ppm_rmsh_create_part(field_up, topo_id, mesh_id, cutoff, ghostsize, resetpos=False, cutoff_weights=None):
-    """Returns (info, xp, up) of the new particles (device tensors)."""
+def ppm_rmsh_create_part(field_up, topo_id, mesh_id, cutoff, ghostsize, resetpos=False, cutoff_weights=None,
+                         field_wp=None):
+    """Returns (info, xp, up) of the new particles (device tensors), or
+    (info, xp, up, wp) when field_wp is given; xp is None unless resetpos."""
     mesh = _topos.get(int(topo_id))
     if mesh is None:
         return -2, None, None
@@ -180,8 +182,14 @@ def ppm_rmsh_create_part(field_up, topo_id, mesh_id, cutoff, ghostsize, resetpos
         f = f.unsqueeze(-1)
     prec = PREC_F32 if f.dtype == torch.float32 else PREC_F64
     lo = [-int(g) for g in ghostsize]
+    w = None
+    if field_wp is not None:
+        w = _device_tensor(field_wp)
+        if w.dim() == mesh.dim:
+            w = w.unsqueeze(-1)
+        w = w.contiguous()
     try:
-        xp, up = core.create_part(mesh, f.contiguous(), lo, cutoff, cutoff_weights, resetpos, prec)
+        res = core.create_part(mesh, f.contiguous(), lo, cutoff, cutoff_weights, resetpos, prec, slave=w)
     except L.PpmgError as e:
-        return e.info, None, None
-    return 0, xp, up
+        return (e.info, None, None) if w is None else (e.info, None, None, None)
+    return (0,) + tuple(res)

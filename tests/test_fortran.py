@@ -1,0 +1,142 @@
+"""The Fortran drop-in: a client program written against the reference's
+module and routine names (ppm_init, ppm_mktopo, ppm_interp_p2m/m2p,
+ppm_rmsh_comp_weights/remesh/create_part, ppm_util_rank3d,
+ppm_impose_part_bc) runs on the GPU library and is checked against the
+oracle.  The program is ppm_legacy_amd/fortran/ppmg_fortran_check.f90, built
+by __graft_entry__.build() with flang."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "ppm_legacy_amd", "ppmg_fortran_check")
+BSP2, MP4 = 1, 2
+
+
+class _Reader:
+    def __init__(self, path):
+        self.b = open(path, "rb").read()
+        self.o = 0
+
+    def take(self, dtype, count):
+        a = np.frombuffer(self.b, dtype=dtype, count=count, offset=self.o)
+        self.o += a.nbytes
+        return a.copy()
+
+    def done(self):
+        return self.o == len(self.b)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _run(tmp_path, nm, np_, lda, kernel, g, seed):
+    rng = np.random.default_rng(seed)
+    xp = rng.uniform(0.0, 1.0, size=(np_, 3))
+    up = rng.normal(size=(np_, lda))
+    xq = rng.uniform(0.0, 1.0, size=(np_, 3))
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        f.write(np.array([nm, np_, lda, kernel, g], np.int32).tobytes())
+        f.write(xp.tobytes())       # (np,3) C order == xp(3,np) Fortran order
+        f.write(up.tobytes())
+        f.write(xq.tobytes())
+    print(f"ppmg_fortran_check nm={nm} np={np_} lda={lda} kernel={kernel} g={g} seed={seed}", flush=True)
+    r = subprocess.run([EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return xp, up, xq, _Reader(fout)
+
+
+def test_fortran_exe_fails_loudly_without_device_or_runs(tmp_path):
+    """CPU side: the program exists when flang built it; with no device its
+    ppm_init reports the device error and stops with a non-zero code."""
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran driver not built (flang absent)")
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("covered by the GPU test")
+    fin = tmp_path / "in.bin"
+    with open(fin, "wb") as f:
+        f.write(np.array([5, 1, 1, 2, 2], np.int32).tobytes())
+        f.write(np.zeros(7).tobytes())
+    r = subprocess.run([EXE, str(fin), str(tmp_path / "o.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "ppm_init" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+def test_fortran_dropin_against_oracle(tmp_path, oracle, kernel):
+    if not os.path.exists(EXE):
+        pytest.fail("Fortran driver not built: run __graft_entry__.build() with flang present")
+    nm, np_, lda, g = 13, 3000, 3, 2
+    xp, up, xq, rd = _run(tmp_path, nm, np_, lda, kernel, g, seed=40 + kernel)
+    n = nm + 2 * g
+    t = oracle.topo(3, [nm] * 3)
+
+    # 1. P2M dv_3d
+    field = rd.take(np.float64, lda * n ** 3).reshape(n, n, n, lda)
+    _, fref = oracle.p2m(t, kernel, xp, up, [g] * 3)
+    assert _rel(field, fref) < 1e-12
+    # 2. M2P dv_3d
+    uq = rd.take(np.float64, lda * np_).reshape(np_, lda)
+    _, qref = oracle.m2p(t, kernel, xq, fref, [g] * 3)
+    assert _rel(uq, qref) < 1e-12
+    # 3. wider caller array: the box is written, everything else untouched
+    nb = n + 2
+    fbig = rd.take(np.float64, (lda + 1) * nb ** 3).reshape(nb, nb, nb, lda + 1)
+    assert np.array_equal(fbig[1:-1, 1:-1, 1:-1, :lda], field)
+    outside = np.ones(fbig.shape, bool)
+    outside[1:-1, 1:-1, 1:-1, :lda] = False
+    assert np.all(fbig[outside] == -7.0)
+    # 4. comp_weights + remesh; remesh allocated field_up(1:lda, 1-g:nm+g, ..., 1:1)
+    lb = rd.take(np.int32, 5)
+    ub = rd.take(np.int32, 5)
+    assert list(lb) == [1, 1 - g, 1 - g, 1 - g, 1] and list(ub) == [lda, nm + g, nm + g, nm + g, 1]
+    field2 = rd.take(np.float64, lda * n ** 3).reshape(n, n, n, lda)
+    info, w = oracle.comp_weights(t, kernel, xp)
+    assert info == 0
+    info, rref = oracle.remesh(t, kernel, xp, up, [g] * 3, w)
+    assert info == 0
+    assert _rel(field2, rref) < 1e-12
+    # 5. create_part (svv) with field2 as slave: same selection as the oracle on
+    #    the GPU field; master and slave values at the selected nodes
+    npn = int(rd.take(np.int32, 1)[0])
+    cutoff = rd.take(np.float64, 2)
+    xn = rd.take(np.float64, 3 * npn).reshape(npn, 3)
+    un = rd.take(np.float64, lda * npn).reshape(npn, lda)
+    wn = rd.take(np.float64, lda * npn).reshape(npn, lda)
+    both = np.concatenate([field, field2], axis=-1)
+    info, xref, cref = oracle.create_part(t, both, [g] * 3, cutoff, cutoff_weights=np.array([1.0] * lda + [0.0] * lda))
+    assert info == 0 and npn == len(xref) and npn > 0
+    assert np.array_equal(xn, xref)
+    assert np.array_equal(un, cref[:, :lda])
+    assert np.array_equal(wn, cref[:, lda:])
+    # 6. util_rank3d
+    nl = int(rd.take(np.int32, 1)[0])
+    nh = int(rd.take(np.int32, 1)[0])
+    lpdx = rd.take(np.int32, nl)
+    lhbx = rd.take(np.int32, nh)
+    lref, href, ic, _, _ = oracle.rank(xp, [0.0] * 3, [1.0] * 3, [nm - 1] * 3, [1] * 6)
+    assert nl == ic and np.array_equal(lpdx, lref) and np.array_equal(lhbx, href)
+    # 7. P2M ss_3d (single precision; the oracle's float sums in another order)
+    fs = rd.take(np.float32, n ** 3).reshape(n, n, n)
+    _, sref = oracle.p2m(t, kernel, xp.astype(np.float32), up[:, 0].astype(np.float32), [g] * 3, dtype=np.float32)
+    assert _rel(fs, sref[..., 0]) < 1e-5
+    # 8. impose_part_bc
+    x2 = rd.take(np.float64, 3 * np_).reshape(np_, 3)
+    assert np.array_equal(x2, oracle.impose_part_bc(t, xp + 0.75))
+    # 9. 2D P2M ds_2d
+    f2d = rd.take(np.float64, n * n).reshape(n, n)
+    t2 = oracle.topo(2, [nm] * 2)
+    _, r2 = oracle.p2m(t2, kernel, np.ascontiguousarray(xp[:, :2]), up[:, 0], [g] * 2)
+    assert _rel(f2d, r2[..., 0]) < 1e-12
+    # 10. particle outside the domain: the reference's fatal error
+    assert int(rd.take(np.int32, 1)[0]) == -1
+    assert rd.done()

@@ -381,6 +381,36 @@ def test_create_part(core, oracle):
     assert np.array_equal(up.cpu().numpy(), uref)
 
 
+def test_create_part_slave_field(core, oracle):
+    """field_wp / wp (ppm_rmsh_create_part.f:626-632): the slave values at the
+    selected nodes.  The slave is an affine image of the master with its own
+    ghost width and component count, so wp must equal that image of up, and
+    the positions must match the node lattice of the oracle."""
+    N = 9
+    t = oracle.topo(3, [N + 1] * 3)
+    f = np.random.default_rng(16).normal(size=oracle.field_shape(t, 3, [1, 1, 1]))
+    cut = [0.5, 1.5]
+    info, xref, uref = oracle.create_part(t, f, [1, 1, 1], cut)
+    assert info == 0 and len(uref) > 0
+    mesh = core.Mesh(3, N + 1)
+    # slave: 2 components, ghost 2, values (2*u0 + 1, -u2) at every stored node
+    inner = f[1:-1, 1:-1, 1:-1]
+    s = np.zeros((N + 5, N + 5, N + 5, 2))
+    s[2:-2, 2:-2, 2:-2, 0] = 2.0 * inner[..., 0] + 1.0
+    s[2:-2, 2:-2, 2:-2, 1] = -inner[..., 2]
+    xp, up, wp = core.create_part(mesh, _dev(f, torch.float64), [-1] * 3, cut, slave=_dev(s, torch.float64),
+                                  slave_lo=[-2] * 3)
+    assert np.array_equal(xp.cpu().numpy(), xref)
+    assert np.array_equal(up.cpu().numpy(), uref)
+    w = wp.cpu().numpy()
+    assert np.array_equal(w[:, 0], 2.0 * uref[:, 0] + 1.0)
+    assert np.array_equal(w[:, 1], -uref[:, 2])
+    # without resetpos there are no positions, as the reference leaves xp alone
+    xn, un, wn = core.create_part(mesh, _dev(f, torch.float64), [-1] * 3, cut, resetpos=False,
+                                  slave=_dev(s, torch.float64), slave_lo=[-2] * 3)
+    assert xn is None and np.array_equal(un.cpu().numpy(), uref) and np.array_equal(wn.cpu().numpy(), w)
+
+
 def test_impose_part_bc(core, oracle):
     t = oracle.topo(3, [9, 9, 9], [-1.0, 0.0, 0.0], [1.0, 1.0, 2.0])
     rng = np.random.default_rng(12)
