@@ -79,6 +79,65 @@ __device__ __forceinline__ T scaled_coord(T x, T mn, T dxi) {
     return (x - mn) * dxi;
 }
 
+// ------------------------------------------------------ z-chunk choice --
+// Workgroups of `kernel` (launched with `threads` threads) resident on the
+// whole device at once, from the occupancy API (cached by the caller).
+template <typename K>
+inline int resident_workgroups(K kernel, int threads) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess) return 1;
+    return cus * per_cu > 0 ? cus * per_cu : 1;
+}
+
+// Cell planes per workgroup for the z-marching tile kernels.  A workgroup
+// spends ~(tz + halo) plane steps; the launch takes ceil(workgroups / slots)
+// such rounds.  Picks the chunk with the fewest plane steps overall (larger
+// chunk on ties), so small meshes still fill the device and large ones march
+// long chunks with little halo overhead.
+inline int pick_z_chunk(int64_t tiles_xy, int nz, int slots, int halo) {
+    int best = 32;
+    int64_t best_cost = -1;
+    for (int tz = 8; tz <= 128; tz *= 2) {
+        int64_t nwg = tiles_xy * (int64_t)((nz + tz - 1) / tz);
+        int64_t rounds = (nwg + slots - 1) / slots;
+        int64_t cost = rounds * (int64_t)((tz < nz ? tz : nz) + halo);
+        if (best_cost < 0 || cost <= best_cost) {
+            best_cost = cost;
+            best = tz;
+        }
+    }
+    return best;
+}
+
+// ------------------------------------------------------ DPP row shifts --
+// Move a value K lanes up (row_shr: lane l reads lane l-K) or down (row_shl:
+// lane l reads lane l+K) inside each 16-lane row; bound_ctrl makes lanes whose
+// source lies outside the row read 0.  Doubles move as two dwords.
+template <int K>
+__device__ __forceinline__ double row_shr(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x110 + K, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x110 + K, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+template <int K>
+__device__ __forceinline__ float row_shr(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + K, 0xF, 0xF, true));
+}
+template <int K>
+__device__ __forceinline__ double row_shl(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x100 + K, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x100 + K, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+template <int K>
+__device__ __forceinline__ float row_shl(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + K, 0xF, 0xF, true));
+}
+
 // ---------------------------------------------------------------- weights --
 // M'4 on the nodes INT(x0)-1 .. INT(x0)+2 (0-based) in the 3D unrolled form
 // of ppm_interp_p2m.f:824-853 (same polynomials as ppm_interp_m2p.f).

@@ -1,128 +1,138 @@
-// ppmg_m2p3d.hip -- 3D M'4 M2P fast path for gfx950: tiled z-march gather.
+// ppmg_m2p3d.hip -- 3D M'4 M2P fast path for gfx950: tiled z-march gather
+// with DPP row exchange.
 //
 // Replaces the gather loop of ppm_interp_m2p (ppm_interp_m2p.f:895-1240) for
 // fully periodic meshes stored in the compact layout; the periodic ghost fill
 // the reference gets from ppm_map_field_ghost (ghost_get) is the index wrap of
 // the node planes loaded around the tile.
 //
-// A workgroup (4 wavefronts) owns the particles of a 13 x 13 cell tile in x,y
-// over 32 cell planes in z.  The particles are binned first (stable counting
-// sort), so the particles of one cell row of the tile are contiguous.  The
-// 16 x 16 nodes those cells read (13 + 3) are kept for 4 node planes in an
-// LDS ring: cell plane cz reads node planes cz-1 .. cz+2; afterwards plane
-// cz-1 is replaced by plane cz+3, loaded with coalesced row reads.  Every node
-// of the field is therefore read from HBM (16/13)^2 times, and every particle
-// gathers its 4x4x4 stencil from LDS.  Output order is the caller's particle
-// order (written through the permutation).
+// A workgroup (4 wavefronts, 16 rows of 16 lanes) owns the particles of a
+// 13 x 16 cell tile in x,y over 32 cell planes in z.  The 16 x 19 nodes those
+// cells read are kept for 4 node planes in an LDS ring (component-major, rows
+// of 16 nodes): cell plane cz reads node planes cz-1 .. cz+2, after which
+// plane cz-1 is replaced by plane cz+3, loaded with coalesced reads.
+//
+// Lane mapping: lane ex of row `row` stands for node column tx0-1+ex and for
+// cell (tx0+ex-1, ty0+row); lanes 1..13 carry the particles of their cell, one
+// particle per round (particles are binned, so a cell's particles are
+// contiguous in the permutation).  Instead of every particle reading its
+// 4x4x4 stencil (192 scattered LDS reads for lda = 3), every lane reads ONE
+// 4x4 (y,z) column of its node column -- 16 lanes read 16 consecutive words,
+// free of bank conflicts -- and contracts it with the (y,z) weights of the
+// four particles whose x-stencil covers that column; those weights come from
+// lanes ex+1, ex, ex-1, ex-2 by DPP row shifts, and the four partial sums go
+// back to their particles by the opposite shifts, where the x weights finish
+// the sum:
+//   u(P) = wx0(P) C0(P-1) + wx1(P) C1(P) + wx2(P) C2(P+1) + wx3(P) C3(P+2),
+//   Ck(L) = sum_{oy,oz} wy_oy(L+1-k) wz_oz(L+1-k) f(L, oy, oz).
+// Output order is the caller's particle order (written through the
+// permutation).
+#include <atomic>
+
 #include "ppmg_internal.h"
 #include "ppmg_interp.h"
 #include "ppmg_sort.h"
 
 namespace ppmg {
 
-constexpr int M3_TT = 13;                 // owned cells per tile edge (x, y)
-constexpr int M3_TE = 16;                 // nodes per tile edge (TT + 3)
-constexpr int M3_TZ = 32;                 // cell planes per z chunk
-constexpr int M3_PITCH = 20;              // ring row pitch in nodes (16 used; padding spreads LDS banks)
-constexpr int M3_PL = M3_PITCH * M3_TE;   // node slots per ring plane and component
-constexpr int M3_THREADS = 256;
+constexpr int M3_TX = 13;               // owned cells per tile along x (16 lanes = 13 + 3 node columns)
+constexpr int M3_TY = 16;               // owned cell rows per tile (one 16-lane row each)
+constexpr int M3_RY = M3_TY + 3;        // node rows in the ring
+constexpr int M3_PL = M3_RY * 16;       // node slots per ring plane and component
+constexpr int M3_THREADS = 16 * M3_TY;  // 256
 
-template <int LDA, typename PosT, typename ValT>
+template <int LDA, typename PosT, typename ValT, typename AccT>
 __global__ void __launch_bounds__(M3_THREADS)
 k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
-          const ValT* __restrict__ field, ValT* __restrict__ up, int lda, int c0) {
-    using Acc = typename AccOf<ValT>::type;
-    __shared__ ValT ring[4 * M3_PL * LDA];
-    __shared__ int row_first[M3_TT];
-    __shared__ int row_pref[M3_TT + 1];
+          const ValT* __restrict__ field, ValT* __restrict__ up, int lda, int c0, int tz) {
+    __shared__ ValT ring[4 * LDA * M3_PL];   // [plane][c][ry][nx]
     const int tid = threadIdx.x;
+    const int ex = tid & 15, row = tid >> 4;
     const int Nx = m.N[0], Ny = m.N[1], Nz = m.N[2];
-    const int tx0 = blockIdx.x * M3_TT, ty0 = blockIdx.y * M3_TT, z0 = blockIdx.z * M3_TZ;
-    const int z1 = min(z0 + M3_TZ, Nz);
-    const int nxo = min(M3_TT, Nx - tx0), nyo = min(M3_TT, Ny - ty0);
+    const int tx0 = blockIdx.x * M3_TX, ty0 = blockIdx.y * M3_TY, z0 = blockIdx.z * tz;
+    const int z1 = min(z0 + tz, Nz);
+    const int nxo = min(M3_TX, Nx - tx0), nyo = min(M3_TY, Ny - ty0);
     const int64_t plane = (int64_t)Nx * Ny;
 
     auto load_plane = [&](int n) {
-        ValT* dst = ring + (n & 3) * (M3_PL * LDA);
+        ValT* dst = ring + (n & 3) * (LDA * M3_PL);
         int nz = n % Nz;
         if (nz < 0) nz += Nz;
-        // coalesced in the field (component fastest), component-major in the ring
-        for (int i = tid; i < M3_TE * M3_TE * LDA; i += M3_THREADS) {
-            int c = i % LDA;
-            int node = i / LDA;
-            int nx = node & (M3_TE - 1), ny = node / M3_TE;
-            int gx = tx0 - 1 + nx;
-            int gy = ty0 - 1 + ny;
-            gx %= Nx; if (gx < 0) gx += Nx;
-            gy %= Ny; if (gy < 0) gy += Ny;
-            dst[c * M3_PL + nx + M3_PITCH * ny] = field[((int64_t)nz * plane + (int64_t)gy * Nx + gx) * lda + c0 + c];
+        // component fastest in the field (coalesced), component-major in the ring
+        for (int i = tid; i < M3_PL * LDA; i += M3_THREADS) {
+            const int c = i % LDA;
+            const int node = i / LDA;
+            const int nx = node & 15, ry = node >> 4;
+            int gx = (tx0 - 1 + nx) % Nx;
+            if (gx < 0) gx += Nx;
+            int gy = (ty0 - 1 + ry) % Ny;
+            if (gy < 0) gy += Ny;
+            dst[c * M3_PL + node] = field[((int64_t)nz * plane + (int64_t)gy * Nx + gx) * lda + c0 + c];
         }
     };
     for (int n = z0 - 1; n <= z0 + 2; ++n) load_plane(n);
     __syncthreads();
 
+    const bool plane_lane = ex >= 1 && ex <= nxo && row < nyo;   // lanes that carry particles
+    const int cx = tx0 + ex - 1, cy = ty0 + row;
+
     for (int cz = z0; cz < z1; ++cz) {
-        if (tid < nyo) {
-            int key = (int)((int64_t)cz * plane + (int64_t)(ty0 + tid) * Nx + tx0);
-            int b = start[key];
-            row_first[tid] = b;
-            row_pref[tid + 1] = start[key + nxo] - b;
+        int first = 0, cnt = 0;
+        if (plane_lane) {
+            const int key = (int)((int64_t)cz * plane + (int64_t)cy * Nx + cx);
+            first = start[key];
+            cnt = start[key + 1] - first;
         }
-        __syncthreads();
-        if (tid == 0) {
-            row_pref[0] = 0;
-            for (int r = 0; r < nyo; ++r) row_pref[r + 1] += row_pref[r];
-        }
-        __syncthreads();
-        const int total = row_pref[nyo];
-        for (int i = tid; i < total; i += M3_THREADS) {
-            int r = 0;
-            while (r + 1 < nyo && row_pref[r + 1] <= i) ++r;
-            const int s = row_first[r] + (i - row_pref[r]);
-            const int p = perm[s];
-            const PosT* x = xp + (int64_t)p * 3;
-            ValT a[3][4];
-            int cell[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
+        for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
+            const bool has = cnt > r;
+            int p = 0;
+            AccT wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, wz[4] = {0, 0, 0, 0};
+            if (has) {
+                p = perm[first + r];
+                const PosT* x = xp + (int64_t)p * 3;
                 int ci;
-                PosT frac = cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
-                mp4_unrolled<ValT>((ValT)frac, a[d]);
-                if (ci >= m.N[d]) ci -= m.N[d];
-                cell[d] = ci;
+                mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci), wx);
+                mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci), wy);
+                mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[2], (PosT)m.minv[2], (PosT)m.dxi[2], ci), wz);
             }
-            const int lx = cell[0] - tx0, ly = cell[1] - ty0;   // in [0, 13): node lx + ox of the ring
-            if ((unsigned)lx >= (unsigned)nxo || (unsigned)ly >= (unsigned)nyo) continue;   // not binned here
-            // one partial sum per z offset: four independent FMA chains per component
-            Acc acc[4][LDA];
+            // (y,z) weights of the particles of lanes ex+1 (k=0), ex (1), ex-1 (2), ex-2 (3)
+            AccT a[4][4], b[4][4];
 #pragma unroll
-            for (int oz = 0; oz < 4; ++oz)
+            for (int o = 0; o < 4; ++o) {
+                a[0][o] = row_shl<1>(wy[o]); b[0][o] = row_shl<1>(wz[o]);
+                a[1][o] = wy[o];             b[1][o] = wz[o];
+                a[2][o] = row_shr<1>(wy[o]); b[2][o] = row_shr<1>(wz[o]);
+                a[3][o] = row_shr<2>(wy[o]); b[3][o] = row_shr<2>(wz[o]);
+            }
+            // column partial sums Ck = sum_oz b_k[oz] * sum_oy a_k[oy] f(oz, oy), one
+            // 4-node (y) slice of the column live at a time
+            AccT C[4][LDA];
 #pragma unroll
-                for (int c = 0; c < LDA; ++c) acc[oz][c].init();
+            for (int c = 0; c < LDA; ++c) {
 #pragma unroll
-            for (int ox = 0; ox < 4; ++ox) {
+                for (int k = 0; k < 4; ++k) C[k][c] = AccT(0);
 #pragma unroll
-                for (int oy = 0; oy < 4; ++oy) {
-                    const ValT wxy = a[0][ox] * a[1][oy];
-                    const int node = (lx + ox) + M3_PITCH * (ly + oy);
+                for (int oz = 0; oz < 4; ++oz) {
+                    const ValT* f = ring + ((cz - 1 + oz) & 3) * (LDA * M3_PL) + c * M3_PL + row * 16 + ex;
+                    const AccT f0 = (AccT)f[0], f1 = (AccT)f[16], f2 = (AccT)f[32], f3 = (AccT)f[48];
 #pragma unroll
-                    for (int oz = 0; oz < 4; ++oz) {
-                        const ValT w = wxy * a[2][oz];
-                        const ValT* f = ring + ((cz - 1 + oz) & 3) * (M3_PL * LDA) + node;
-#pragma unroll
-                        for (int c = 0; c < LDA; ++c) acc[oz][c].add(w * f[c * M3_PL]);
+                    for (int k = 0; k < 4; ++k) {
+                        AccT t = a[k][0] * f0;
+                        t += a[k][1] * f1;
+                        t += a[k][2] * f2;
+                        t += a[k][3] * f3;
+                        C[k][c] += b[k][oz] * t;
                     }
                 }
             }
 #pragma unroll
             for (int c = 0; c < LDA; ++c) {
-                acc[0][c].merge(acc[1][c]);
-                acc[2][c].merge(acc[3][c]);
-                acc[0][c].merge(acc[2][c]);
+                AccT u = wx[0] * row_shr<1>(C[0][c]);
+                u += wx[1] * C[1][c];
+                u += wx[2] * row_shl<1>(C[2][c]);
+                u += wx[3] * row_shl<2>(C[3][c]);
+                if (has) up[(int64_t)p * lda + c0 + c] = (ValT)u;
             }
-            ValT* u = up + (int64_t)p * lda + c0;
-#pragma unroll
-            for (int c = 0; c < LDA; ++c) u[c] = (ValT)acc[0][c].value();
         }
         __syncthreads();
         if (cz + 1 < z1) load_plane(cz + 3);
@@ -139,15 +149,26 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
     return true;
 }
 
-template <typename PosT, typename ValT>
+// One launch of the gather for components c0 .. c0+LL-1; the z chunk comes
+// from the resident workgroup count (each chunk loads 3 extra node planes).
+template <int LL, typename PosT, typename ValT, typename AccT>
+static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda, int c0, const ValT* field, ValT* up,
+                      hipStream_t s) {
+    static std::atomic<int> slots{0};
+    if (slots.load() == 0) slots.store(resident_workgroups(k_m3_tile<LL, PosT, ValT, AccT>, M3_THREADS));
+    const int tx = (m.N[0] + M3_TX - 1) / M3_TX, ty = (m.N[1] + M3_TY - 1) / M3_TY;
+    const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 1);
+    dim3 grid((unsigned)tx, (unsigned)ty, (unsigned)((m.N[2] + tz - 1) / tz));
+    hipLaunchKernelGGL((k_m3_tile<LL, PosT, ValT, AccT>), grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
+                       plan->perm.as<int>(), xp, field, up, lda, c0, tz);
+}
+
+template <typename PosT, typename ValT, typename AccT>
 static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda, const ValT* field, ValT* up,
                        hipStream_t s) {
-    dim3 grid((unsigned)((m.N[0] + M3_TT - 1) / M3_TT), (unsigned)((m.N[1] + M3_TT - 1) / M3_TT),
-              (unsigned)((m.N[2] + M3_TZ - 1) / M3_TZ));
     for (int c0 = 0; c0 < lda; c0 += 4) {
         int L = lda - c0 < 4 ? lda - c0 : 4;
-#define TL(LL) hipLaunchKernelGGL((k_m3_tile<LL, PosT, ValT>), grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(), \
-                                  plan->perm.as<int>(), xp, field, up, lda, c0)
+#define TL(LL) launch_m3<LL, PosT, ValT, AccT>(plan, m, xp, lda, c0, field, up, s)
         if (L == 1) TL(1);
         else if (L == 2) TL(2);
         else if (L == 3) TL(3);
@@ -161,17 +182,25 @@ static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int ld
 // Bins the particles into the plan, then gathers.  Returns 1 (nothing done)
 // when some particle lies outside the sub, so the caller can use the general
 // path, which reads such particles' stencils with bounds checks.
+//
+// Arithmetic: FP64 in every form.  On gfx950 (packed FP32 off) an FP64 FMA
+// issues at the FP32 rate, so the FP32-compensated form keeps FP32 only where
+// it saves bytes (field in HBM and LDS, output) and sums its 64 products in
+// FP64 registers -- more accurate than compensated FP32 at no extra cost.
 int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* field,
                void* up, hipStream_t s) {
     int64_t nbad = 0;
     if (bin_particles(plan, m, prec, np, xp, s, &nbad)) return PPMG_ERROR_FATAL;
     if (nbad > 0) return 1;
     mark(plan, s);
+    if (np <= 0) return 0;
     if (prec == PPMG_PREC_F64)
-        return m2p3d_typed<double, double>(plan, m, (const double*)xp, lda, (const double*)field, (double*)up, s);
+        return m2p3d_typed<double, double, double>(plan, m, (const double*)xp, lda, (const double*)field,
+                                                   (double*)up, s);
     if (prec == PPMG_PREC_F32C)
-        return m2p3d_typed<double, float>(plan, m, (const double*)xp, lda, (const float*)field, (float*)up, s);
-    return m2p3d_typed<float, float>(plan, m, (const float*)xp, lda, (const float*)field, (float*)up, s);
+        return m2p3d_typed<double, float, double>(plan, m, (const double*)xp, lda, (const float*)field, (float*)up,
+                                                  s);
+    return m2p3d_typed<float, float, double>(plan, m, (const float*)xp, lda, (const float*)field, (float*)up, s);
 }
 
 }  // namespace ppmg

@@ -18,6 +18,8 @@
 // fixed function of the binning, so results are bitwise reproducible with no
 // global float atomics.  The periodic fold happens through the index wrap of
 // the cells around the tile.
+#include <atomic>
+
 #include "ppmg_internal.h"
 #include "ppmg_interp.h"
 #include "ppmg_sort.h"
@@ -26,7 +28,6 @@ namespace ppmg {
 
 constexpr int P3_TT = 13;   // owned nodes per tile edge (x, y)
 constexpr int P3_TE = 16;   // cells per tile edge incl. halo (TT + 3)
-constexpr int P3_TZ = 32;   // owned node planes per z chunk
 constexpr int P3_PL = P3_TT * P3_TT;
 
 // LDS accumulator of one node component.  Every form accumulates in an FP64
@@ -44,38 +45,6 @@ template <typename ValT> struct LdsAcc {
     __device__ static __forceinline__ ValT value(const slot_t& a) { return (ValT)a; }
 };
 
-// Sorted particle records: fractional cell offsets (INT truncation as in
-// ppm_interp_p2m.f:800-822, in the position precision) and strengths.
-template <typename PosT, typename FracT, typename ValT>
-__global__ void __launch_bounds__(256)
-k_p3_sorted(const int* __restrict__ perm, const PosT* __restrict__ xp, const ValT* __restrict__ up, int lda,
-            int64_t np, DevMesh m, FracT* __restrict__ fr, ValT* __restrict__ us) {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= np) return;
-    int p = perm[s];
-    const PosT* x = xp + (int64_t)p * 3;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        int i;
-        fr[s * 3 + d] = (FracT)cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], i);
-    }
-    for (int c = 0; c < lda; ++c) us[s * lda + c] = up[(int64_t)p * lda + c];
-}
-
-// Shift a value one to three lanes up inside each 16-lane row (DPP row_shr,
-// bound_ctrl: lanes shifted in from outside the row read 0).
-template <int K>
-__device__ __forceinline__ double row_shr(double v) {
-    int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x110 + K, 0xF, 0xF, true);
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x110 + K, 0xF, 0xF, true);
-    return __hiloint2double(hi, lo);
-}
-template <int K>
-__device__ __forceinline__ float row_shr(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + K, 0xF, 0xF, true));
-}
-
 // Tiled deposit.  Lane = one cell of the 16 x 16 window: ex = lane & 15 along
 // x, so each 16-lane DPP row is a run of 16 consecutive cells at fixed y.  In
 // round r every lane forms the contributions of the r-th particle of its cell
@@ -85,18 +54,18 @@ __device__ __forceinline__ float row_shr(float v) {
 // so each lane adds ONE value per (y, z offset, component) to the ring -- 48
 // LDS atomics per particle for lda = 3 instead of 192.  The ring is stored
 // component-major (SoA) so a row's 16 lanes hit consecutive 8-byte words.
-template <int LDA, typename ValT, typename FracT>
+template <int LDA, typename PosT, typename ValT>
 __global__ void __launch_bounds__(64)
-k_p3_tile(DevMesh m, const int* __restrict__ start, const FracT* __restrict__ fr, const ValT* __restrict__ us,
-          int lda, int c0, ValT* __restrict__ field) {
+k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
+          const ValT* __restrict__ up, int lda, int c0, int tz, ValT* __restrict__ field) {
     using A = LdsAcc<ValT>;
     using slot_t = typename A::slot_t;
     constexpr int PLANE = P3_PL * LDA;            // slots per ring plane: [c][node]
     __shared__ slot_t ring[4 * PLANE];
     const int lane = threadIdx.x;
     const int Nx = m.N[0], Ny = m.N[1], Nz = m.N[2];
-    const int tx0 = blockIdx.x * P3_TT, ty0 = blockIdx.y * P3_TT, z0 = blockIdx.z * P3_TZ;
-    const int z1 = min(z0 + P3_TZ, Nz);
+    const int tx0 = blockIdx.x * P3_TT, ty0 = blockIdx.y * P3_TT, z0 = blockIdx.z * tz;
+    const int z1 = min(z0 + tz, Nz);
     const int nxo = min(P3_TT, Nx - tx0), nyo = min(P3_TT, Ny - ty0);
     for (int i = lane; i < 4 * PLANE; i += 64) A::zero(ring[i]);
 
@@ -129,13 +98,25 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const FracT* __restrict__ fr
             const int ly0 = ey4[q] - 3;              // node row of stencil offset oy = 0
             for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
                 const bool has = cnt > r;
-                const int sp = has ? first + r : 0;
+                // the particle's offsets (INT truncation as ppm_interp_p2m.f:800-822, in the
+                // position precision) and strengths, read through the permutation
                 ValT wx[4], wy[4], wz[4], u[LDA];
-                mp4_unrolled<ValT>((ValT)fr[3 * (int64_t)sp], wx);
-                mp4_unrolled<ValT>((ValT)fr[3 * (int64_t)sp + 1], wy);
-                mp4_unrolled<ValT>((ValT)fr[3 * (int64_t)sp + 2], wz);
+                ValT fr[3] = {ValT(0), ValT(0), ValT(0)};
 #pragma unroll
-                for (int c = 0; c < LDA; ++c) u[c] = has ? us[(int64_t)sp * lda + c0 + c] : ValT(0);
+                for (int c = 0; c < LDA; ++c) u[c] = ValT(0);
+                if (has) {
+                    const int p = perm[first + r];
+                    const PosT* x = xp + (int64_t)p * 3;
+                    int ci;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        fr[d] = (ValT)cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
+#pragma unroll
+                    for (int c = 0; c < LDA; ++c) u[c] = up[(int64_t)p * lda + c0 + c];
+                }
+                mp4_unrolled<ValT>(fr[0], wx);
+                mp4_unrolled<ValT>(fr[1], wy);
+                mp4_unrolled<ValT>(fr[2], wz);
 #pragma unroll
                 for (int oy = 0; oy < 4; ++oy) {
                     const int ly = ly0 + oy;
@@ -147,11 +128,13 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const FracT* __restrict__ fr
                         const ValT wyz = wy[oy] * wz[oz];
 #pragma unroll
                         for (int c = 0; c < LDA; ++c) {
+                            // Horner form of the row sum: after three one-lane shifts
+                            // v(ex) = sum_k wx_k(ex-k) t(ex-k), the node ex-3 total
                             const ValT t = wyz * u[c];
-                            ValT v = wx[0] * t;
-                            v += row_shr<1>(wx[1] * t);
-                            v += row_shr<2>(wx[2] * t);
-                            v += row_shr<3>(wx[3] * t);
+                            ValT v = wx[3] * t;
+                            v = wx[2] * t + row_shr<1>(v);
+                            v = wx[1] * t + row_shr<1>(v);
+                            v = wx[0] * t + row_shr<1>(v);
                             if (own) A::add(pl + c * P3_PL + node, v);
                         }
                     }
@@ -177,11 +160,6 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const FracT* __restrict__ fr
     }
 }
 
-static inline unsigned nblk(int64_t n, int b) {
-    int64_t g = (n + b - 1) / b;
-    return (unsigned)(g > 0 ? g : 1);
-}
-
 bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg) {
     if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return false;
     for (int d = 0; d < 3; ++d) {
@@ -192,26 +170,29 @@ bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGe
     return true;
 }
 
-template <typename PosT, typename FracT, typename ValT>
+// One launch of the deposit for components c0 .. c0+LL-1.  The z chunk is
+// chosen from the device's resident workgroup count (queried once per
+// instantiation): each chunk marches 3 extra cell planes.
+template <int LL, typename PosT, typename ValT>
+static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, int lda, int c0, ValT* field,
+                      hipStream_t s) {
+    static std::atomic<int> slots{0};
+    if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT>, 64));
+    const int tx = (m.N[0] + P3_TT - 1) / P3_TT, ty = (m.N[1] + P3_TT - 1) / P3_TT;
+    const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 3);
+    dim3 grid((unsigned)tx, (unsigned)ty, (unsigned)((m.N[2] + tz - 1) / tz));
+    hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT>), grid, dim3(64), 0, s, m, plan->start.as<int>(),
+                       plan->perm.as<int>(), xp, up, lda, c0, tz, field);
+}
+
+template <typename PosT, typename ValT>
 static int p2m3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT* xp, int lda, const ValT* up,
                        ValT* field, hipStream_t s) {
-    if (plan->wsorted.ensure(sizeof(FracT) * 3 * (size_t)(np > 0 ? np : 1)) ||
-        plan->usorted.ensure(sizeof(ValT) * (size_t)lda * (size_t)(np > 0 ? np : 1)))
-        return PPMG_ERROR_FATAL;
-    FracT* fr = plan->wsorted.as<FracT>();
-    ValT* us = plan->usorted.as<ValT>();
-    if (np > 0) {
-        hipLaunchKernelGGL((k_p3_sorted<PosT, FracT, ValT>), dim3(nblk(np, 256)), dim3(256), 0, s, plan->perm.as<int>(),
-                           xp, up, lda, np, m, fr, us);
-        PPMG_HIP(hipGetLastError());
-    }
+    (void)np;
     mark(plan, s);
-    dim3 grid((unsigned)((m.N[0] + P3_TT - 1) / P3_TT), (unsigned)((m.N[1] + P3_TT - 1) / P3_TT),
-              (unsigned)((m.N[2] + P3_TZ - 1) / P3_TZ));
-    const int* start = plan->start.as<int>();
     for (int c0 = 0; c0 < lda; c0 += 4) {
         int L = lda - c0 < 4 ? lda - c0 : 4;
-#define TL(LL) hipLaunchKernelGGL((k_p3_tile<LL, ValT, FracT>), grid, dim3(64), 0, s, m, start, fr, us, lda, c0, field)
+#define TL(LL) launch_p3<LL, PosT, ValT>(plan, m, xp, up, lda, c0, field, s)
         if (L == 1) TL(1);
         else if (L == 2) TL(2);
         else if (L == 3) TL(3);
@@ -226,12 +207,10 @@ static int p2m3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT
 int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
                void* field, hipStream_t s) {
     if (prec == PPMG_PREC_F64)
-        return p2m3d_typed<double, double, double>(plan, m, np, (const double*)xp, lda, (const double*)up,
-                                                   (double*)field, s);
+        return p2m3d_typed<double, double>(plan, m, np, (const double*)xp, lda, (const double*)up, (double*)field, s);
     if (prec == PPMG_PREC_F32C)
-        return p2m3d_typed<double, float, float>(plan, m, np, (const double*)xp, lda, (const float*)up, (float*)field,
-                                                 s);
-    return p2m3d_typed<float, float, float>(plan, m, np, (const float*)xp, lda, (const float*)up, (float*)field, s);
+        return p2m3d_typed<double, float>(plan, m, np, (const double*)xp, lda, (const float*)up, (float*)field, s);
+    return p2m3d_typed<float, float>(plan, m, np, (const float*)xp, lda, (const float*)up, (float*)field, s);
 }
 
 }  // namespace ppmg
