@@ -54,42 +54,97 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const int nxo = min(M3_TX, Nx - tx0), nyo = min(M3_TY, Ny - ty0);
     const int64_t plane = (int64_t)Nx * Ny;
 
-    auto load_plane = [&](int n) {
-        ValT* dst = ring + (n & 3) * (LDA * M3_PL);
+    // Ring planes move global -> registers -> LDS so that the global reads of
+    // plane cz+3 are in flight while plane cz is computed.
+    constexpr int NLD = (M3_PL * LDA + M3_THREADS - 1) / M3_THREADS;
+    ValT stage[NLD];
+    auto fetch_plane = [&](int n) {
         int nz = n % Nz;
         if (nz < 0) nz += Nz;
         // component fastest in the field (coalesced), component-major in the ring
-        for (int i = tid; i < M3_PL * LDA; i += M3_THREADS) {
-            const int c = i % LDA;
-            const int node = i / LDA;
-            const int nx = node & 15, ry = node >> 4;
-            int gx = (tx0 - 1 + nx) % Nx;
-            if (gx < 0) gx += Nx;
-            int gy = (ty0 - 1 + ry) % Ny;
-            if (gy < 0) gy += Ny;
-            dst[c * M3_PL + node] = field[((int64_t)nz * plane + (int64_t)gy * Nx + gx) * lda + c0 + c];
+#pragma unroll
+        for (int j = 0; j < NLD; ++j) {
+            const int i = tid + j * M3_THREADS;
+            if (i < M3_PL * LDA) {
+                const int c = i % LDA;
+                const int node = i / LDA;
+                const int nx = node & 15, ry = node >> 4;
+                int gx = (tx0 - 1 + nx) % Nx;
+                if (gx < 0) gx += Nx;
+                int gy = (ty0 - 1 + ry) % Ny;
+                if (gy < 0) gy += Ny;
+                stage[j] = field[((int64_t)nz * plane + (int64_t)gy * Nx + gx) * lda + c0 + c];
+            }
         }
     };
-    for (int n = z0 - 1; n <= z0 + 2; ++n) load_plane(n);
-    __syncthreads();
+    auto store_plane = [&](int n) {
+        ValT* dst = ring + (n & 3) * (LDA * M3_PL);
+#pragma unroll
+        for (int j = 0; j < NLD; ++j) {
+            const int i = tid + j * M3_THREADS;
+            if (i < M3_PL * LDA) dst[(i % LDA) * M3_PL + i / LDA] = stage[j];
+        }
+    };
+    for (int n = z0 - 1; n <= z0 + 2; ++n) {
+        fetch_plane(n);
+        store_plane(n);
+    }
 
     const bool plane_lane = ex >= 1 && ex <= nxo && row < nyo;   // lanes that carry particles
     const int cx = tx0 + ex - 1, cy = ty0 + row;
+    const int key0 = (int)((int64_t)cy * Nx + cx);
+    // Cell heads run one plane ahead of the particle data, which runs one
+    // plane ahead of the computation.
+    auto heads = [&](int z, int& f, int& n) {
+        f = 0;
+        n = 0;
+        if (plane_lane && z < z1) {
+            const int key = (int)((int64_t)z * plane) + key0;
+            f = start[key];
+            n = start[key + 1] - f;
+        }
+    };
+    auto particle = [&](int f, int n, int& p, PosT (&x)[3]) {
+        p = 0;
+        x[0] = x[1] = x[2] = PosT(0);
+        if (n > 0) {
+            p = perm[f];
+            x[0] = xp[(int64_t)p * 3];
+            x[1] = xp[(int64_t)p * 3 + 1];
+            x[2] = xp[(int64_t)p * 3 + 2];
+        }
+    };
+    int first_n, cnt_n, first_nn, cnt_nn, p_n;
+    PosT x_n[3];
+    heads(z0, first_n, cnt_n);
+    heads(z0 + 1, first_nn, cnt_nn);
+    particle(first_n, cnt_n, p_n, x_n);
+    __syncthreads();
 
     for (int cz = z0; cz < z1; ++cz) {
-        int first = 0, cnt = 0;
-        if (plane_lane) {
-            const int key = (int)((int64_t)cz * plane + (int64_t)cy * Nx + cx);
-            first = start[key];
-            cnt = start[key + 1] - first;
-        }
+        const int first = first_n, cnt = cnt_n, p0 = p_n;
+        const PosT x0[3] = {x_n[0], x_n[1], x_n[2]};
+        first_n = first_nn;
+        cnt_n = cnt_nn;
+        particle(first_n, cnt_n, p_n, x_n);
+        heads(cz + 2, first_nn, cnt_nn);
+        const bool more = cz + 1 < z1;
+        if (more) fetch_plane(cz + 3);
         for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
             const bool has = cnt > r;
             int p = 0;
             AccT wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, wz[4] = {0, 0, 0, 0};
             if (has) {
-                p = perm[first + r];
-                const PosT* x = xp + (int64_t)p * 3;
+                PosT x[3];
+                if (r == 0) {
+                    p = p0;
+                    x[0] = x0[0]; x[1] = x0[1]; x[2] = x0[2];
+                } else {
+                    p = perm[first + r];
+                    x[0] = xp[(int64_t)p * 3];
+                    x[1] = xp[(int64_t)p * 3 + 1];
+                    x[2] = xp[(int64_t)p * 3 + 2];
+                }
                 int ci;
                 mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci), wx);
                 mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci), wy);
@@ -134,8 +189,8 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                 if (has) up[(int64_t)p * lda + c0 + c] = (ValT)u;
             }
         }
-        __syncthreads();
-        if (cz + 1 < z1) load_plane(cz + 3);
+        __syncthreads();   // every row is done with plane cz-1: its slot takes plane cz+3
+        if (more) store_plane(cz + 3);
         __syncthreads();
     }
 }
