@@ -45,15 +45,40 @@ template <typename ValT> struct LdsAcc {
     __device__ static __forceinline__ ValT value(const slot_t& a) { return (ValT)a; }
 };
 
+// Data of the particle K lanes below (row_shr by K; K = 0: this lane's own):
+// G[c] = wx_K * u_c, and its y and z weights.
+template <int K, typename ValT, int LDA>
+__device__ __forceinline__ void shift_in(const ValT (&wx)[4], const ValT (&wy)[4], const ValT (&wz)[4],
+                                         const ValT (&u)[LDA], ValT (&G)[LDA], ValT (&WY)[4], ValT (&WZ)[4]) {
+#pragma unroll
+    for (int c = 0; c < LDA; ++c) {
+        const ValT g = wx[K] * u[c];
+        if constexpr (K == 0) G[c] = g;
+        else G[c] = row_shr<K>(g);
+    }
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        if constexpr (K == 0) {
+            WY[o] = wy[o];
+            WZ[o] = wz[o];
+        } else {
+            WY[o] = row_shr<K>(wy[o]);
+            WZ[o] = row_shr<K>(wz[o]);
+        }
+    }
+}
+
 // Tiled deposit.  Lane = one cell of the 16 x 16 window: ex = lane & 15 along
 // x, so each 16-lane DPP row is a run of 16 consecutive cells at fixed y.  In
-// round r every lane forms the contributions of the r-th particle of its cell
-// (zero when it has none); the four x-offsets that land on one node come from
-// four neighbouring lanes, and are summed across the row with DPP shifts:
-//   node(ex - 3) = v0(ex) + v1(ex-1) + v2(ex-2) + v3(ex-3)
-// so each lane adds ONE value per (y, z offset, component) to the ring -- 48
-// LDS atomics per particle for lda = 3 instead of 192.  The ring is stored
-// component-major (SoA) so a row's 16 lanes hit consecutive 8-byte words.
+// round r every lane takes the r-th particle of its cell (zero strength when
+// it has none).  Lane ex finishes node ex-3 of its row, which receives x-offset
+// k from the particle of lane ex-k:
+//   node(ex - 3) += sum_k wx_k(ex-k) u(ex-k) wy_oy(ex-k) wz_oz(ex-k)
+// The 11 inputs of each neighbour (wx_k u, wy, wz) come over by DPP row
+// shifts, the 16 (y,z) sums are formed locally, and each lane adds ONE value
+// per (y, z offset, component) to the ring -- 48 LDS atomics per particle for
+// lda = 3 instead of 192.  The ring is stored component-major (SoA) so a row's
+// 16 lanes hit consecutive 8-byte words.
 template <int LDA, typename PosT, typename ValT>
 __global__ void __launch_bounds__(64)
 k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
@@ -86,37 +111,79 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     __syncthreads();
     const int64_t plane_cells = (int64_t)Nx * Ny;
 
+    // Software pipeline over the (plane, chunk) steps: the cell heads of the
+    // step after next and the first particle of the next step are loaded while
+    // the current step computes.
+    auto heads = [&](int cz, int q, int& f, int& n) {
+        f = 0;
+        n = 0;
+        if (cz <= z1) {
+            int czw = cz % Nz;
+            if (czw < 0) czw += Nz;
+            const int key = (int)(plane_cells * czw + keyxy[q]);
+            f = start[key];
+            n = start[key + 1] - f;
+        }
+    };
+    auto particle = [&](int f, int n, ValT (&fr)[3], ValT (&u)[LDA]) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) fr[d] = ValT(0);
+#pragma unroll
+        for (int c = 0; c < LDA; ++c) u[c] = ValT(0);
+        if (n > 0) {
+            // offsets (INT truncation as ppm_interp_p2m.f:800-822, in the position
+            // precision) and strengths, read through the permutation
+            const int p = perm[f];
+            const PosT* x = xp + (int64_t)p * 3;
+            int ci;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) fr[d] = (ValT)cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
+#pragma unroll
+            for (int c = 0; c < LDA; ++c) u[c] = up[(int64_t)p * lda + c0 + c];
+        }
+    };
+    int f1, n1, f2, n2;
+    ValT fr1[3], u1[LDA];
+    heads(z0 - 2, 0, f1, n1);
+    heads(z0 - 2, 1, f2, n2);
+    particle(f1, n1, fr1, u1);
+
     for (int cz = z0 - 2; cz <= z1; ++cz) {
-        int czw = cz % Nz;
-        if (czw < 0) czw += Nz;
-        const int64_t kz = plane_cells * czw;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int key = (int)(kz + keyxy[q]);
-            const int first = start[key];
-            const int cnt = start[key + 1] - first;
+            const int first = f1, cnt = n1;
+            ValT fr0[3], u0[LDA];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) fr0[d] = fr1[d];
+#pragma unroll
+            for (int c = 0; c < LDA; ++c) u0[c] = u1[c];
+            f1 = f2;
+            n1 = n2;
+            particle(f1, n1, fr1, u1);
+            if (q < 2) heads(cz, q + 2, f2, n2);
+            else heads(cz + 1, q - 2, f2, n2);
             const int ly0 = ey4[q] - 3;              // node row of stencil offset oy = 0
             for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
-                const bool has = cnt > r;
-                // the particle's offsets (INT truncation as ppm_interp_p2m.f:800-822, in the
-                // position precision) and strengths, read through the permutation
-                ValT wx[4], wy[4], wz[4], u[LDA];
-                ValT fr[3] = {ValT(0), ValT(0), ValT(0)};
+                ValT wx[4], wy[4], wz[4], u[LDA], fr[3];
+                if (r == 0) {
 #pragma unroll
-                for (int c = 0; c < LDA; ++c) u[c] = ValT(0);
-                if (has) {
-                    const int p = perm[first + r];
-                    const PosT* x = xp + (int64_t)p * 3;
-                    int ci;
+                    for (int d = 0; d < 3; ++d) fr[d] = fr0[d];
 #pragma unroll
-                    for (int d = 0; d < 3; ++d)
-                        fr[d] = (ValT)cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
-#pragma unroll
-                    for (int c = 0; c < LDA; ++c) u[c] = up[(int64_t)p * lda + c0 + c];
+                    for (int c = 0; c < LDA; ++c) u[c] = u0[c];
+                } else {
+                    particle(first + r, cnt - r, fr, u);
                 }
                 mp4_unrolled<ValT>(fr[0], wx);
                 mp4_unrolled<ValT>(fr[1], wy);
                 mp4_unrolled<ValT>(fr[2], wz);
+                // Lane ex finishes node ex-3, which takes x-offset k from the particle
+                // of lane ex-k: fetch that particle's wx_k * u, wy and wz by DPP
+                // (k = 1..3), then form the 16 (y,z) sums locally.
+                ValT G[4][LDA], WY[4][4], WZ[4][4];
+                shift_in<0>(wx, wy, wz, u, G[0], WY[0], WZ[0]);
+                shift_in<1>(wx, wy, wz, u, G[1], WY[1], WZ[1]);
+                shift_in<2>(wx, wy, wz, u, G[2], WY[2], WZ[2]);
+                shift_in<3>(wx, wy, wz, u, G[3], WY[3], WZ[3]);
 #pragma unroll
                 for (int oy = 0; oy < 4; ++oy) {
                     const int ly = ly0 + oy;
@@ -125,17 +192,16 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
 #pragma unroll
                     for (int oz = 0; oz < 4; ++oz) {
                         slot_t* pl = ring + ((cz - 1 + oz) & 3) * PLANE;
-                        const ValT wyz = wy[oy] * wz[oz];
+                        ValT v[LDA];
 #pragma unroll
-                        for (int c = 0; c < LDA; ++c) {
-                            // Horner form of the row sum: after three one-lane shifts
-                            // v(ex) = sum_k wx_k(ex-k) t(ex-k), the node ex-3 total
-                            const ValT t = wyz * u[c];
-                            ValT v = wx[3] * t;
-                            v = wx[2] * t + row_shr<1>(v);
-                            v = wx[1] * t + row_shr<1>(v);
-                            v = wx[0] * t + row_shr<1>(v);
-                            if (own) A::add(pl + c * P3_PL + node, v);
+                        for (int k = 0; k < 4; ++k) {
+                            const ValT w = WY[k][oy] * WZ[k][oz];
+#pragma unroll
+                            for (int c = 0; c < LDA; ++c) v[c] = k == 0 ? w * G[k][c] : v[c] + w * G[k][c];
+                        }
+                        if (own) {
+#pragma unroll
+                            for (int c = 0; c < LDA; ++c) A::add(pl + c * P3_PL + node, v[c]);
                         }
                     }
                 }
