@@ -231,6 +231,7 @@ struct ppmg_plan {
     ppmg::DevBuf key, slot, count, start, perm, wsorted, usorted, scan_tmp, err, misc, misc2;
     int64_t np = 0;
     int64_t ncells = 0;
+    bool sorted_input = false;   // last binning found the input already in cell order
     int dim = 0, kernel = 0, prec = 0;
     ppmg::DevMesh mesh{};
     bool have_forward = false;

@@ -140,15 +140,11 @@ int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, int64_t tota
 }
 
 // ------------------------------------------------------------ binning ---
-// Cell key per particle with the index arithmetic of ppm_interp_p2m.f:800-822
-// (x0 = (x - min)*dxi, INT truncation) and the sub test of :535-561.
+// Cell key of one particle with the index arithmetic of ppm_interp_p2m.f:
+// 800-822 (x0 = (x - min)*dxi, INT truncation) and the sub test of :535-561;
+// -1 when the particle lies in no cell of the sub.
 template <int DIM, typename PosT>
-__global__ void __launch_bounds__(256) k_bin(const PosT* __restrict__ xp, int64_t np, DevMesh m,
-                                             int* __restrict__ key, int* __restrict__ slot,
-                                             int* __restrict__ count, int* __restrict__ err) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    const PosT* x = xp + p * DIM;
+__device__ __forceinline__ int cell_key(const PosT* __restrict__ x, const DevMesh& m) {
     bool ok = true;
     int cell[3] = {0, 0, 0};
 #pragma unroll
@@ -164,15 +160,92 @@ __global__ void __launch_bounds__(256) k_bin(const PosT* __restrict__ xp, int64_
         if (i < 0 || i >= m.nbin[d]) ok = false;
         cell[d] = i;
     }
-    if (!ok) {
-        atomicAdd(err, 1);
-        key[p] = -1;
-        return;
-    }
-    int k = cell[0] + m.nbin[0] * (cell[1] + m.nbin[1] * cell[2]);
-    key[p] = k;
-    slot[p] = atomicAdd(&count[k], 1);
+    return ok ? cell[0] + m.nbin[0] * (cell[1] + m.nbin[1] * cell[2]) : -1;
 }
+
+// Keys of all particles.  err[0] counts the particles in no cell; err[1] is
+// set to 1 when a descent key[p] < key[p-1] exists (none: the input is already
+// in cell order, e.g. particles made by create_part or a lattice).  Inside a
+// wavefront the neighbours' keys come by shuffles; descents across wavefront
+// boundaries are checked afterwards by k_wave_seams.  The flag is read before
+// it is written, so the many wavefronts of unsorted input do not queue on one
+// word.
+//   SLOT (unsorted input expected): take each particle's arrival slot in its
+//     cell by an atomic count.
+//   !SLOT (input expected in cell order): write the identity permutation and,
+//     at the first particle of each run of equal keys inside the wavefront,
+//     the run length as the cell's count (k_wave_seams then rewrites the
+//     counts of runs that cross a wavefront seam); valid only if no descent is
+//     found (else the caller clears the counts and counts again).
+template <int DIM, typename PosT, bool SLOT>
+__global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64_t np, DevMesh m,
+                                              int* __restrict__ key, int* __restrict__ slot,
+                                              int* __restrict__ count, int* __restrict__ perm,
+                                              int* __restrict__ err) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool live = p < np;
+    const int k = live ? cell_key<DIM, PosT>(xp + p * DIM, m) : -1;
+    // neighbours' keys by shuffles while every lane is still active
+    const int left = __shfl_up(k, 1);
+    const int right = __shfl_down(k, 1);
+    const bool bad = live && k < 0;
+    const bool desc = live && lane > 0 && k < left;
+    const uint64_t bb = __ballot(bad), bd = __ballot(desc);
+    // lanes whose successor in the wavefront has the same key
+    const uint64_t same_next = __ballot(live && lane < 63 && right == k);
+    if (lane == 0) {
+        if (bb) atomicAdd(&err[0], (int)__popcll(bb));
+        if (bd && *(volatile int*)&err[1] == 0) err[1] = 1;
+    }
+    if (!live) return;
+    key[p] = k;
+    if (k < 0) return;
+    if (SLOT) {
+        slot[p] = atomicAdd(&count[k], 1);
+    } else {
+        perm[p] = (int)p;
+        if (lane > 0 && left == k) return;            // not the first of its run in this wavefront
+        // run length inside the wavefront: 1 + the unbroken ones of same_next from this lane on
+        const uint64_t tail = ~(same_next >> lane);
+        count[k] = 1 + (tail ? __builtin_ctzll(tail) : 64 - lane);
+    }
+}
+
+// Wavefront seams b = 64, 128, ...: descents key[b] < key[b-1], and (count
+// non-null, input expected in cell order) the full length of every run of
+// equal keys that crosses a seam, replacing the per-wavefront lengths; a run
+// crossing several seams gets the same value from each.
+__global__ void __launch_bounds__(256) k_wave_seams(const int* __restrict__ key, int64_t np, int* __restrict__ err,
+                                                    int* __restrict__ count) {
+    const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x + 1) * 64;
+    bool desc = false;
+    if (b < np) {
+        const int k = key[b], kl = key[b - 1];
+        desc = k < kl;
+        if (count && k == kl && k >= 0) {
+            int64_t s0 = b - 1, e = b + 1;
+            while (s0 > 0 && key[s0 - 1] == k) --s0;
+            while (e < np && key[e] == k) ++e;
+            count[k] = (int)(e - s0);
+        }
+    }
+    if (__ballot(desc) && (threadIdx.x & 63) == 0 && *(volatile int*)&err[1] == 0) err[1] = 1;
+}
+
+// Unsorted input: arrival slot of every particle in its cell.
+__global__ void __launch_bounds__(256) k_count(const int* __restrict__ key, int64_t np, int* __restrict__ slot,
+                                               int* __restrict__ count) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    slot[p] = atomicAdd(&count[key[p]], 1);
+}
+
+__global__ void __launch_bounds__(256) k_identity(int64_t np, int* __restrict__ perm) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) perm[p] = (int)p;
+}
+
 
 // Cell key with the arithmetic of ppm_util_rank3d.f:414-475 (FLOOR, ghost
 // cells, upper-boundary corrections, -1 for particles in no box).
@@ -276,39 +349,60 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
         return PPMG_ERROR_FATAL;
     if (fill_zero(plan->count.p, sizeof(int) * (size_t)ncells, s) || fill_zero(plan->err.p, sizeof(int) * 4, s))
         return PPMG_ERROR_FATAL;
+    // The plan remembers whether its last input was in cell order; a plan fed
+    // the same kind of particles each call (advected particles, or particles
+    // fresh from create_part) predicts right and does each step once.
+    const bool predict_sorted = plan->sorted_input;
     if (np > 0) {
         unsigned g = nblk(np, 256);
+        int* key = plan->key.as<int>();
+        int* slot = plan->slot.as<int>();
+        int* count = plan->count.as<int>();
+        int* err = plan->err.as<int>();
+#define KEYS(D, T, S) hipLaunchKernelGGL((k_keys<D, T, S>), dim3(g), dim3(256), 0, s, (const T*)xp, np, m, key, slot, \
+                                         count, plan->perm.as<int>(), err)
+        const bool single = prec == PPMG_PREC_F32;
         if (m.dim == 3) {
-            if (prec == PPMG_PREC_F32)
-                hipLaunchKernelGGL((k_bin<3, float>), dim3(g), dim3(256), 0, s, (const float*)xp, np, m,
-                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), plan->err.as<int>());
-            else
-                hipLaunchKernelGGL((k_bin<3, double>), dim3(g), dim3(256), 0, s, (const double*)xp, np, m,
-                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), plan->err.as<int>());
+            if (single) { if (predict_sorted) KEYS(3, float, false); else KEYS(3, float, true); }
+            else { if (predict_sorted) KEYS(3, double, false); else KEYS(3, double, true); }
         } else {
-            if (prec == PPMG_PREC_F32)
-                hipLaunchKernelGGL((k_bin<2, float>), dim3(g), dim3(256), 0, s, (const float*)xp, np, m,
-                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), plan->err.as<int>());
-            else
-                hipLaunchKernelGGL((k_bin<2, double>), dim3(g), dim3(256), 0, s, (const double*)xp, np, m,
-                                   plan->key.as<int>(), plan->slot.as<int>(), plan->count.as<int>(), plan->err.as<int>());
+            if (single) { if (predict_sorted) KEYS(2, float, false); else KEYS(2, float, true); }
+            else { if (predict_sorted) KEYS(2, double, false); else KEYS(2, double, true); }
+        }
+#undef KEYS
+        const int64_t seams = (np - 1) / 64;
+        if (seams > 0)
+            hipLaunchKernelGGL(k_wave_seams, dim3(nblk(seams, 256)), dim3(256), 0, s, key, np, err,
+                               predict_sorted ? count : nullptr);
+        PPMG_HIP(hipGetLastError());
+    }
+    int herr[2] = {0, 0};
+    PPMG_HIP(hipMemcpyAsync(herr, plan->err.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    PPMG_HIP(hipStreamSynchronize(s));
+    *nbad = herr[0];
+    if (herr[0]) return 0;
+    const bool sorted = herr[1] == 0;
+    if (np > 0) {
+        if (sorted && !predict_sorted) {
+            hipLaunchKernelGGL(k_identity, dim3(nblk(np, 256)), dim3(256), 0, s, np, plan->perm.as<int>());
+        } else if (!sorted && predict_sorted) {
+            // the run lengths written under the wrong prediction are not counts
+            if (fill_zero(plan->count.p, sizeof(int) * (size_t)ncells, s)) return PPMG_ERROR_FATAL;
+            hipLaunchKernelGGL(k_count, dim3(nblk(np, 256)), dim3(256), 0, s, plan->key.as<int>(), np,
+                               plan->slot.as<int>(), plan->count.as<int>());
         }
         PPMG_HIP(hipGetLastError());
     }
-    int hbad = 0;
-    PPMG_HIP(hipMemcpyAsync(&hbad, plan->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    PPMG_HIP(hipStreamSynchronize(s));
-    *nbad = hbad;
-    if (hbad) return 0;
     if (exclusive_scan(plan->count.as<int>(), ncells, plan->start.as<int>(), plan->scan_tmp, np, s))
         return PPMG_ERROR_FATAL;
-    if (np > 0) {
+    if (np > 0 && !sorted) {
         hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 256)), dim3(256), 0, s, plan->key.as<int>(),
                            plan->slot.as<int>(), plan->start.as<int>(), np, plan->perm.as<int>());
         hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, plan->start.as<int>(),
                            ncells, plan->perm.as<int>());
         PPMG_HIP(hipGetLastError());
     }
+    plan->sorted_input = sorted;
     plan->np = np;
     plan->ncells = ncells;
     return 0;

@@ -84,6 +84,37 @@ def test_p2m_config1_64cubed_fp64(core, oracle):
     assert np.array_equal(start.cpu().numpy(), np.concatenate([[0], np.cumsum(np.bincount(ref_key, minlength=N ** 3))]))
 
 
+def test_binning_sorted_and_unsorted_input(core, oracle):
+    """The binning has two paths: input already in cell order (identity
+    permutation, counts from runs of equal keys) and any other order (atomic
+    slots + per-cell stabilisation); a plan predicts the path from its last
+    input.  Every (predicted, actual) pair -- one plan fed sorted, sorted,
+    shuffled, shuffled, sorted input -- must give numpy's stable argsort, the
+    exact cell heads and the oracle's field, on clustered particles with empty
+    and crowded cells."""
+    N, g = 24, [2, 2, 2]
+    rng = np.random.default_rng(77)
+    base = np.concatenate([rng.normal(0.3, 0.02, size=(3000, 3)) % 1.0, rng.uniform(0, 1, size=(1000, 3))])
+    t = oracle.topo(3, [N + 1] * 3)
+    mesh = core.Mesh(3, N + 1)
+    plan = core.Plan()
+    for order in ["cell_sorted", "cell_sorted", "shuffled", "shuffled", "cell_sorted"]:
+        cells = (base * N).astype(np.int64)
+        key = cells[:, 0] + N * (cells[:, 1] + N * cells[:, 2])
+        o = np.argsort(key, kind="stable") if order == "cell_sorted" else rng.permutation(len(base))
+        xp, key = base[o], key[o]
+        up = rng.normal(size=(len(xp), 3))
+        field = torch.zeros(mesh.padded_shape(3, g), dtype=torch.float64, device="cuda")
+        core.p2m(mesh, _dev(xp, torch.float64), _dev(up, torch.float64), field, [-2] * 3, MP4, F64, plan)
+        k, perm, start = plan.sort_view()
+        assert np.array_equal(k.cpu().numpy(), key), order
+        assert np.array_equal(perm.cpu().numpy(), np.argsort(key, kind="stable")), order
+        assert np.array_equal(start.cpu().numpy(),
+                              np.concatenate([[0], np.cumsum(np.bincount(key, minlength=N ** 3))])), order
+        _, ref = oracle.p2m(t, MP4, xp, up, g)
+        assert _rel(field.cpu().numpy(), ref) < 1e-12, order
+
+
 # ----------------------------------------------------------- variants --
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
