@@ -111,6 +111,19 @@ inline int pick_z_chunk(int64_t tiles_xy, int nz, int slots, int halo) {
     return best;
 }
 
+// ----------------------------------------------------- XCD tile order --
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8), each with its own L2.  Tile kernels use a 1D grid and map
+// workgroup b to tile (b % 8) * q + min(b % 8, r) + b / 8 (q, r: tiles per
+// XCD and remainder), so each XCD walks a contiguous block of tiles in x-fastest
+// order and the halos that neighbouring tiles share are served from one L2.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t ntiles) {
+    const int64_t x = b % NUM_XCD, j = b / NUM_XCD;
+    const int64_t q = ntiles / NUM_XCD, r = ntiles % NUM_XCD;
+    return x * q + (x < r ? x : r) + j;
+}
+
 // ------------------------------------------------------ DPP row shifts --
 // Move a value K lanes up (row_shr: lane l reads lane l-K) or down (row_shl:
 // lane l reads lane l+K) inside each 16-lane row; bound_ctrl makes lanes whose

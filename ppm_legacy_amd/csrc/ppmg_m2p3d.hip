@@ -49,7 +49,10 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const int tid = threadIdx.x;
     const int ex = tid & 15, row = tid >> 4;
     const int Nx = m.N[0], Ny = m.N[1], Nz = m.N[2];
-    const int tx0 = blockIdx.x * M3_TX, ty0 = blockIdx.y * M3_TY, z0 = blockIdx.z * tz;
+    const int ntx = (Nx + M3_TX - 1) / M3_TX, nty = (Ny + M3_TY - 1) / M3_TY;
+    const int64_t tile = xcd_tile(blockIdx.x, (int64_t)gridDim.x);
+    const int tx0 = (int)(tile % ntx) * M3_TX, ty0 = (int)((tile / ntx) % nty) * M3_TY;
+    const int z0 = (int)(tile / ((int64_t)ntx * nty)) * tz;
     const int z1 = min(z0 + tz, Nz);
     const int nxo = min(M3_TX, Nx - tx0), nyo = min(M3_TY, Ny - ty0);
     const int64_t plane = (int64_t)Nx * Ny;
@@ -213,7 +216,7 @@ static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda
     if (slots.load() == 0) slots.store(resident_workgroups(k_m3_tile<LL, PosT, ValT, AccT>, M3_THREADS));
     const int tx = (m.N[0] + M3_TX - 1) / M3_TX, ty = (m.N[1] + M3_TY - 1) / M3_TY;
     const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 1);
-    dim3 grid((unsigned)tx, (unsigned)ty, (unsigned)((m.N[2] + tz - 1) / tz));
+    dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
     hipLaunchKernelGGL((k_m3_tile<LL, PosT, ValT, AccT>), grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
                        plan->perm.as<int>(), xp, field, up, lda, c0, tz);
 }

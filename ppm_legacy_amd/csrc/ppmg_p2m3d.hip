@@ -89,7 +89,10 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     __shared__ slot_t ring[4 * PLANE];
     const int lane = threadIdx.x;
     const int Nx = m.N[0], Ny = m.N[1], Nz = m.N[2];
-    const int tx0 = blockIdx.x * P3_TT, ty0 = blockIdx.y * P3_TT, z0 = blockIdx.z * tz;
+    const int ntx = (Nx + P3_TT - 1) / P3_TT, nty = (Ny + P3_TT - 1) / P3_TT;
+    const int64_t tile = xcd_tile(blockIdx.x, (int64_t)gridDim.x);
+    const int tx0 = (int)(tile % ntx) * P3_TT, ty0 = (int)((tile / ntx) % nty) * P3_TT;
+    const int z0 = (int)(tile / ((int64_t)ntx * nty)) * tz;
     const int z1 = min(z0 + tz, Nz);
     const int nxo = min(P3_TT, Nx - tx0), nyo = min(P3_TT, Ny - ty0);
     for (int i = lane; i < 4 * PLANE; i += 64) A::zero(ring[i]);
@@ -246,7 +249,7 @@ static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const V
     if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT>, 64));
     const int tx = (m.N[0] + P3_TT - 1) / P3_TT, ty = (m.N[1] + P3_TT - 1) / P3_TT;
     const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 3);
-    dim3 grid((unsigned)tx, (unsigned)ty, (unsigned)((m.N[2] + tz - 1) / tz));
+    dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
     hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT>), grid, dim3(64), 0, s, m, plan->start.as<int>(),
                        plan->perm.as<int>(), xp, up, lda, c0, tz, field);
 }
