@@ -129,6 +129,22 @@ def cpu_baseline(oracle_mod, core, wl, torch, n, lda, seed):
                       f"median of {reps} cycles, {best:.3f} s/cycle"}
 
 
+def measured_traffic(kernel, N, lda):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (copy-calibrated), when they were taken on this workload."""
+    path = os.path.join(ROOT, "profiles", "roofline_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    w = t.get("workload", {})
+    k = t.get("kernels", {}).get(kernel)
+    if k is None or w.get("cells") != N or w.get("lda") != lda or w.get("dtype") != "f64":
+        return None, None
+    return k["traffic_bytes"], t.get("source")
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -185,7 +201,9 @@ def main():
 
     # dominant kernel of the FP64 cycle, priced with the algorithmic bytes of its pass
     dom = max(("deposit", "m2p"), key=lambda k: ph64[k])
+    dom_kernel = "k_p3_tile" if dom == "deposit" else "k_m3_tile"
     achieved = b64_pass / (ph64[dom] * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(dom_kernel, N, lda)
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -213,9 +231,9 @@ def main():
                    "cells": N, "particles": np_, "lda": lda, "kernel": "M'4", "seed": args.seed},
         "cycle_hbm_frac": 2 * b64_pass / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "phases_ms": ph64,
-        "roofline": {"bound": "hbm", "kernel": "k_p3_tile" if dom == "deposit" else "k_m3_tile",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": b64_pass, "kernel_ms": ph64[dom], "traffic_source": traffic_src},
         "fp32c": res or None,
         "cpu_baseline": cpu,
     }
