@@ -41,8 +41,11 @@ constexpr int M3_RY = M3_TY + 3;        // node rows in the ring
 constexpr int M3_PL = M3_RY * 16;       // node slots per ring plane and component
 constexpr int M3_THREADS = 16 * M3_TY;  // 256
 
-template <int LDA, typename PosT, typename ValT, typename AccT>
-__global__ void __launch_bounds__(M3_THREADS)
+// WPE: minimum wavefronts per SIMD the register allocation must allow.  The
+// FP64 ring runs 2 (no spills); the FP32 ring (FP32c and FP32 forms) runs 3,
+// at the cost of a few spills reloaded once per plane, outside the particle loop.
+template <int LDA, typename PosT, typename ValT, typename AccT, int WPE>
+__global__ void __launch_bounds__(M3_THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
           const ValT* __restrict__ field, ValT* __restrict__ up, int lda, int c0, int tz) {
     __shared__ ValT ring[4 * LDA * M3_PL];   // [plane][c][ry][nx]
@@ -162,13 +165,11 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                 a[2][o] = row_shr<1>(wy[o]); b[2][o] = row_shr<1>(wz[o]);
                 a[3][o] = row_shr<2>(wy[o]); b[3][o] = row_shr<2>(wz[o]);
             }
-            // column partial sums Ck = sum_oz b_k[oz] * sum_oy a_k[oy] f(oz, oy), one
-            // 4-node (y) slice of the column live at a time
-            AccT C[4][LDA];
+            // per component: column partial sums Ck = sum_oz b_k[oz] * sum_oy a_k[oy] f(oz, oy)
+            // (one 4-node y slice of the column live at a time), then the particle's sum
 #pragma unroll
             for (int c = 0; c < LDA; ++c) {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) C[k][c] = AccT(0);
+                AccT C[4] = {AccT(0), AccT(0), AccT(0), AccT(0)};
 #pragma unroll
                 for (int oz = 0; oz < 4; ++oz) {
                     const ValT* f = ring + ((cz - 1 + oz) & 3) * (LDA * M3_PL) + c * M3_PL + row * 16 + ex;
@@ -179,16 +180,13 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                         t += a[k][1] * f1;
                         t += a[k][2] * f2;
                         t += a[k][3] * f3;
-                        C[k][c] += b[k][oz] * t;
+                        C[k] += b[k][oz] * t;
                     }
                 }
-            }
-#pragma unroll
-            for (int c = 0; c < LDA; ++c) {
-                AccT u = wx[0] * row_shr<1>(C[0][c]);
-                u += wx[1] * C[1][c];
-                u += wx[2] * row_shl<1>(C[2][c]);
-                u += wx[3] * row_shl<2>(C[3][c]);
+                AccT u = wx[0] * row_shr<1>(C[0]);
+                u += wx[1] * C[1];
+                u += wx[2] * row_shl<1>(C[2]);
+                u += wx[3] * row_shl<2>(C[3]);
                 if (has) up[(int64_t)p * lda + c0 + c] = (ValT)u;
             }
         }
@@ -212,12 +210,13 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
 template <int LL, typename PosT, typename ValT, typename AccT>
 static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda, int c0, const ValT* field, ValT* up,
                       hipStream_t s) {
+    auto kernel = k_m3_tile<LL, PosT, ValT, AccT, sizeof(ValT) == 4 ? 3 : 1>;
     static std::atomic<int> slots{0};
-    if (slots.load() == 0) slots.store(resident_workgroups(k_m3_tile<LL, PosT, ValT, AccT>, M3_THREADS));
+    if (slots.load() == 0) slots.store(resident_workgroups(kernel, M3_THREADS));
     const int tx = (m.N[0] + M3_TX - 1) / M3_TX, ty = (m.N[1] + M3_TY - 1) / M3_TY;
     const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 1);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
-    hipLaunchKernelGGL((k_m3_tile<LL, PosT, ValT, AccT>), grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
+    hipLaunchKernelGGL(kernel, grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
                        plan->perm.as<int>(), xp, field, up, lda, c0, tz);
 }
 
