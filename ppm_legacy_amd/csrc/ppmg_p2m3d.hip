@@ -5,14 +5,15 @@
 // meshes written in the compact layout (unique nodes only).
 //
 // Work decomposition (one wavefront per workgroup):
-//   * a workgroup owns a 13 x 13 node tile in x,y and 32 node planes in z;
+//   * a workgroup owns a 13 x 9 node tile in x,y and a chunk of node planes in z;
 //   * it marches the cell planes cz = z0-2 .. z1 (the cells whose M'4 stencil
 //     touches its planes) and keeps a ring of 4 node planes in LDS: cell plane
 //     cz deposits into node planes cz-1 .. cz+2, after which node plane cz-1
 //     is complete and is written out with coalesced row stores;
-//   * the 16 x 16 cells whose stencils touch the tile (13 + 3 halo) are taken
-//     64 at a time, one cell per lane; each lane walks its cell's particles
-//     (binned, in stable order) and adds the owned part of the 4x4x4 stencil.
+//   * the 16 x 12 cells whose stencils touch the tile (13 + 3 and 9 + 3 halo)
+//     are taken 64 at a time, one cell per lane; each lane walks its cell's
+//     particles (binned, in stable order) and adds the owned part of the
+//     4x4x4 stencil.
 // Lanes of one instruction hold distinct cells, hence write distinct nodes,
 // and one wavefront owns the ring: the order of every node's additions is a
 // fixed function of the binning, so results are bitwise reproducible with no
@@ -26,9 +27,8 @@
 
 namespace ppmg {
 
-constexpr int P3_TT = 13;   // owned nodes per tile edge (x, y)
-constexpr int P3_TE = 16;   // cells per tile edge incl. halo (TT + 3)
-constexpr int P3_PL = P3_TT * P3_TT;
+constexpr int P3_TT = 13;   // owned nodes per tile along x (16-lane DPP row = 13 + 3 halo cells)
+constexpr int P3_TY = 9;    // owned node rows per tile along y (TY + 3 cell rows = 3 chunks of 4)
 
 // LDS accumulator of one node component.  Every form accumulates in an FP64
 // slot with ds_add_f64 (fire-and-forget, no read-modify-write latency chain):
@@ -68,8 +68,12 @@ __device__ __forceinline__ void shift_in(const ValT (&wx)[4], const ValT (&wy)[4
     }
 }
 
-// Tiled deposit.  Lane = one cell of the 16 x 16 window: ex = lane & 15 along
-// x, so each 16-lane DPP row is a run of 16 consecutive cells at fixed y.  In
+// Tiled deposit.  Lane = one cell of the 16 x (TY+3) window: ex = lane & 15
+// along x, so each 16-lane DPP row is a run of 16 consecutive cells at fixed
+// y; the window's cell rows are taken 4 at a time (one per DPP row).  The tile
+// is 13 x 9 nodes: a smaller ring (4 planes x 117 nodes) leaves room for more
+// resident workgroups, which this latency-bound kernel needs more than it
+// needs the lower halo share of a taller tile.  In
 // round r every lane takes the r-th particle of its cell (zero strength when
 // it has none).  Lane ex finishes node ex-3 of its row, which receives x-offset
 // k from the particle of lane ex-k:
@@ -79,22 +83,25 @@ __device__ __forceinline__ void shift_in(const ValT (&wx)[4], const ValT (&wy)[4
 // per (y, z offset, component) to the ring -- 48 LDS atomics per particle for
 // lda = 3 instead of 192.  The ring is stored component-major (SoA) so a row's
 // 16 lanes hit consecutive 8-byte words.
-template <int LDA, typename PosT, typename ValT>
+template <int LDA, typename PosT, typename ValT, int TY>
 __global__ void __launch_bounds__(64)
 k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
           const ValT* __restrict__ up, int lda, int c0, int tz, ValT* __restrict__ field) {
     using A = LdsAcc<ValT>;
     using slot_t = typename A::slot_t;
+    constexpr int P3_PL = P3_TT * TY;              // nodes per ring plane
+    constexpr int NQ = (TY + 3) / 4;               // chunks of 4 cell rows per plane
+    static_assert((TY + 3) % 4 == 0, "tile height + 3 must be a multiple of 4");
     constexpr int PLANE = P3_PL * LDA;            // slots per ring plane: [c][node]
     __shared__ slot_t ring[4 * PLANE];
     const int lane = threadIdx.x;
     const int Nx = m.N[0], Ny = m.N[1], Nz = m.N[2];
-    const int ntx = (Nx + P3_TT - 1) / P3_TT, nty = (Ny + P3_TT - 1) / P3_TT;
+    const int ntx = (Nx + P3_TT - 1) / P3_TT, nty = (Ny + TY - 1) / TY;
     const int64_t tile = xcd_tile(blockIdx.x, (int64_t)gridDim.x);
-    const int tx0 = (int)(tile % ntx) * P3_TT, ty0 = (int)((tile / ntx) % nty) * P3_TT;
+    const int tx0 = (int)(tile % ntx) * P3_TT, ty0 = (int)((tile / ntx) % nty) * TY;
     const int z0 = (int)(tile / ((int64_t)ntx * nty)) * tz;
     const int z1 = min(z0 + tz, Nz);
-    const int nxo = min(P3_TT, Nx - tx0), nyo = min(P3_TT, Ny - ty0);
+    const int nxo = min(P3_TT, Nx - tx0), nyo = min(TY, Ny - ty0);
     for (int i = lane; i < 4 * PLANE; i += 64) A::zero(ring[i]);
 
     const int ex = lane & 15, erow = lane >> 4;
@@ -102,9 +109,9 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const bool xown = lx >= 0 && lx < nxo;
     int cx = tx0 - 2 + ex;
     cx %= Nx; if (cx < 0) cx += Nx;
-    int keyxy[4], ey4[4];
+    int keyxy[NQ], ey4[NQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         int ey = q * 4 + erow;
         ey4[q] = ey;
         int cy = ty0 - 2 + ey;
@@ -153,7 +160,7 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
 
     for (int cz = z0 - 2; cz <= z1; ++cz) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             const int first = f1, cnt = n1;
             ValT fr0[3], u0[LDA];
 #pragma unroll
@@ -163,8 +170,8 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             f1 = f2;
             n1 = n2;
             particle(f1, n1, fr1, u1);
-            if (q < 2) heads(cz, q + 2, f2, n2);
-            else heads(cz + 1, q - 2, f2, n2);
+            if (q + 2 < NQ) heads(cz, q + 2, f2, n2);
+            else heads(cz + 1, q + 2 - NQ, f2, n2);
             const int ly0 = ey4[q] - 3;              // node row of stencil offset oy = 0
             for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
                 ValT wx[4], wy[4], wz[4], u[LDA], fr[3];
@@ -246,11 +253,11 @@ template <int LL, typename PosT, typename ValT>
 static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, int lda, int c0, ValT* field,
                       hipStream_t s) {
     static std::atomic<int> slots{0};
-    if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT>, 64));
-    const int tx = (m.N[0] + P3_TT - 1) / P3_TT, ty = (m.N[1] + P3_TT - 1) / P3_TT;
+    if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT, P3_TY>, 64));
+    const int tx = (m.N[0] + P3_TT - 1) / P3_TT, ty = (m.N[1] + P3_TY - 1) / P3_TY;
     const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 3);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
-    hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT>), grid, dim3(64), 0, s, m, plan->start.as<int>(),
+    hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT, P3_TY>), grid, dim3(64), 0, s, m, plan->start.as<int>(),
                        plan->perm.as<int>(), xp, up, lda, c0, tz, field);
 }
 
