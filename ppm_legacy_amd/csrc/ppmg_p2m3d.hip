@@ -45,27 +45,22 @@ template <typename ValT> struct LdsAcc {
     __device__ static __forceinline__ ValT value(const slot_t& a) { return (ValT)a; }
 };
 
-// Data of the particle K lanes below (row_shr by K; K = 0: this lane's own):
-// G[c] = wx_K * u_c, and its y and z weights.
+// Value of lane ex-K (row_shr by K; K = 0: this lane's own).
+template <int K, typename ValT>
+__device__ __forceinline__ ValT from_left(ValT v) {
+    if constexpr (K == 0) return v;
+    else return row_shr<K>(v);
+}
+
+// Data of the particle K lanes below: G[c] = wx_K * u_c and its z weights
+// (its y weights are fetched per y offset, to keep fewer registers live).
 template <int K, typename ValT, int LDA>
-__device__ __forceinline__ void shift_in(const ValT (&wx)[4], const ValT (&wy)[4], const ValT (&wz)[4],
-                                         const ValT (&u)[LDA], ValT (&G)[LDA], ValT (&WY)[4], ValT (&WZ)[4]) {
+__device__ __forceinline__ void shift_in(const ValT (&wx)[4], const ValT (&wz)[4], const ValT (&u)[LDA],
+                                         ValT (&G)[LDA], ValT (&WZ)[4]) {
 #pragma unroll
-    for (int c = 0; c < LDA; ++c) {
-        const ValT g = wx[K] * u[c];
-        if constexpr (K == 0) G[c] = g;
-        else G[c] = row_shr<K>(g);
-    }
+    for (int c = 0; c < LDA; ++c) G[c] = from_left<K>(wx[K] * u[c]);
 #pragma unroll
-    for (int o = 0; o < 4; ++o) {
-        if constexpr (K == 0) {
-            WY[o] = wy[o];
-            WZ[o] = wz[o];
-        } else {
-            WY[o] = row_shr<K>(wy[o]);
-            WZ[o] = row_shr<K>(wz[o]);
-        }
-    }
+    for (int o = 0; o < 4; ++o) WZ[o] = from_left<K>(wz[o]);
 }
 
 // Tiled deposit.  Lane = one cell of the 16 x (TY+3) window: ex = lane & 15
@@ -121,9 +116,8 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     __syncthreads();
     const int64_t plane_cells = (int64_t)Nx * Ny;
 
-    // Software pipeline over the (plane, chunk) steps: the cell heads of the
-    // step after next and the first particle of the next step are loaded while
-    // the current step computes.
+    // The cell heads of the step after next are loaded while the current
+    // (plane, chunk) step computes.
     auto heads = [&](int cz, int q, int& f, int& n) {
         f = 0;
         n = 0;
@@ -153,49 +147,36 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         }
     };
     int f1, n1, f2, n2;
-    ValT fr1[3], u1[LDA];
     heads(z0 - 2, 0, f1, n1);
     heads(z0 - 2, 1, f2, n2);
-    particle(f1, n1, fr1, u1);
 
     for (int cz = z0 - 2; cz <= z1; ++cz) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int first = f1, cnt = n1;
-            ValT fr0[3], u0[LDA];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) fr0[d] = fr1[d];
-#pragma unroll
-            for (int c = 0; c < LDA; ++c) u0[c] = u1[c];
             f1 = f2;
             n1 = n2;
-            particle(f1, n1, fr1, u1);
             if (q + 2 < NQ) heads(cz, q + 2, f2, n2);
             else heads(cz + 1, q + 2 - NQ, f2, n2);
             const int ly0 = ey4[q] - 3;              // node row of stencil offset oy = 0
             for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
                 ValT wx[4], wy[4], wz[4], u[LDA], fr[3];
-                if (r == 0) {
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) fr[d] = fr0[d];
-#pragma unroll
-                    for (int c = 0; c < LDA; ++c) u[c] = u0[c];
-                } else {
-                    particle(first + r, cnt - r, fr, u);
-                }
+                particle(first + r, cnt - r, fr, u);
                 mp4_unrolled<ValT>(fr[0], wx);
                 mp4_unrolled<ValT>(fr[1], wy);
                 mp4_unrolled<ValT>(fr[2], wz);
                 // Lane ex finishes node ex-3, which takes x-offset k from the particle
                 // of lane ex-k: fetch that particle's wx_k * u, wy and wz by DPP
                 // (k = 1..3), then form the 16 (y,z) sums locally.
-                ValT G[4][LDA], WY[4][4], WZ[4][4];
-                shift_in<0>(wx, wy, wz, u, G[0], WY[0], WZ[0]);
-                shift_in<1>(wx, wy, wz, u, G[1], WY[1], WZ[1]);
-                shift_in<2>(wx, wy, wz, u, G[2], WY[2], WZ[2]);
-                shift_in<3>(wx, wy, wz, u, G[3], WY[3], WZ[3]);
+                ValT G[4][LDA], WZ[4][4];
+                shift_in<0>(wx, wz, u, G[0], WZ[0]);
+                shift_in<1>(wx, wz, u, G[1], WZ[1]);
+                shift_in<2>(wx, wz, u, G[2], WZ[2]);
+                shift_in<3>(wx, wz, u, G[3], WZ[3]);
 #pragma unroll
                 for (int oy = 0; oy < 4; ++oy) {
+                    const ValT WY[4] = {from_left<0>(wy[oy]), from_left<1>(wy[oy]), from_left<2>(wy[oy]),
+                                        from_left<3>(wy[oy])};
                     const int ly = ly0 + oy;
                     const bool own = xown && ly >= 0 && ly < nyo;
                     const int node = lx + P3_TT * ly;
@@ -205,7 +186,7 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                         ValT v[LDA];
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
-                            const ValT w = WY[k][oy] * WZ[k][oz];
+                            const ValT w = WY[k] * WZ[k][oz];
 #pragma unroll
                             for (int c = 0; c < LDA; ++c) v[c] = k == 0 ? w * G[k][c] : v[c] + w * G[k][c];
                         }
