@@ -99,8 +99,7 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const bool plane_lane = ex >= 1 && ex <= nxo && row < nyo;   // lanes that carry particles
     const int cx = tx0 + ex - 1, cy = ty0 + row;
     const int key0 = (int)((int64_t)cy * Nx + cx);
-    // Cell heads run one plane ahead of the particle data, which runs one
-    // plane ahead of the computation.
+    // Cell heads run one plane ahead of the computation.
     auto heads = [&](int z, int& f, int& n) {
         f = 0;
         n = 0;
@@ -120,20 +119,13 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             x[2] = xp[(int64_t)p * 3 + 2];
         }
     };
-    int first_n, cnt_n, first_nn, cnt_nn, p_n;
-    PosT x_n[3];
+    int first_n, cnt_n;
     heads(z0, first_n, cnt_n);
-    heads(z0 + 1, first_nn, cnt_nn);
-    particle(first_n, cnt_n, p_n, x_n);
     __syncthreads();
 
     for (int cz = z0; cz < z1; ++cz) {
-        const int first = first_n, cnt = cnt_n, p0 = p_n;
-        const PosT x0[3] = {x_n[0], x_n[1], x_n[2]};
-        first_n = first_nn;
-        cnt_n = cnt_nn;
-        particle(first_n, cnt_n, p_n, x_n);
-        heads(cz + 2, first_nn, cnt_nn);
+        const int first = first_n, cnt = cnt_n;
+        heads(cz + 1, first_n, cnt_n);
         const bool more = cz + 1 < z1;
         if (more) fetch_plane(cz + 3);
         for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
@@ -142,15 +134,7 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             AccT wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, wz[4] = {0, 0, 0, 0};
             if (has) {
                 PosT x[3];
-                if (r == 0) {
-                    p = p0;
-                    x[0] = x0[0]; x[1] = x0[1]; x[2] = x0[2];
-                } else {
-                    p = perm[first + r];
-                    x[0] = xp[(int64_t)p * 3];
-                    x[1] = xp[(int64_t)p * 3 + 1];
-                    x[2] = xp[(int64_t)p * 3 + 2];
-                }
+                particle(first + r, cnt - r, p, x);
                 int ci;
                 mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci), wx);
                 mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci), wy);
