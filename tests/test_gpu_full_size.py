@@ -1,0 +1,195 @@
+"""Parity at the benchmark's own sizes.
+
+The C restatement runs with OpenMP on the host cores, so the remesh cycle at
+256^3 (16.8M particles) and the clustered 512^3 case (64M particles) are
+compared with it node for node.  The 128^3 adjoint is checked against central
+finite differences of the GPU P2M.  Size-independent properties are checked
+as well: mass conservation, bitwise repeatability, and the FP32-compensated
+form's error against FP64."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from ppm_legacy_amd import workloads as wl
+
+MP4 = 2
+F64, F32C = 0, 1
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def core():
+    from ppm_legacy_amd import core as c
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return c
+
+
+def _threads():
+    return max(1, min(len(os.sched_getaffinity(0)), 16))
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _advected_lattice(core, N, seed):
+    """The headline workload: jittered lattice advected at CFL 0.5 by an
+    8-mode divergence-free field, wrapped; vorticity from another field."""
+    mesh = core.Mesh(3, N + 1)
+    x = core.gen_jittered_lattice(N, seed)
+    v = core.gen_divfree_velocity(x, wl.fourier_modes(seed))
+    dt = 0.5 * (1.0 / N) / torch.sqrt((v * v).sum(1)).max().item()
+    x.add_(v, alpha=dt)
+    del v
+    core.impose_part_bc(mesh, x)
+    w = core.gen_divfree_velocity(x, wl.fourier_modes(seed + 1000))
+    w /= w.abs().max()
+    return mesh, x, w.contiguous()
+
+
+def test_config2_remesh_cycle_256_vs_oracle(core, oracle):
+    """256^3 remesh cycle (P2M of the advected particles, M2P onto a fresh
+    lattice), FP64 GPU vs the oracle node for node and particle for particle,
+    plus the FP32-compensated form's error against FP64."""
+    N = 256
+    print(f"config 2: {N}^3, {N ** 3} particles", flush=True)
+    mesh, x, w = _advected_lattice(core, N, 5)
+    xq = core.gen_jittered_lattice(N, 6)
+    field = torch.empty(mesh.compact_shape(3), dtype=torch.float64, device="cuda")
+    out = torch.empty_like(w)
+    core.p2m(mesh, x, w, field, [0, 0, 0], MP4, F64)
+    core.m2p(mesh, xq, field, [0, 0, 0], out, MP4, F64)
+    f64, o64 = field.cpu().numpy(), out.cpu().numpy()
+    # mass: the unique nodes carry the total strength of every component
+    tot = w.sum(0).cpu().numpy()
+    assert np.allclose(f64.reshape(-1, 3).sum(0), tot, rtol=0, atol=1e-9 * np.abs(w.cpu().numpy()).sum())
+    # bitwise repeatability of the deposit
+    field2 = torch.empty_like(field)
+    core.p2m(mesh, x, w, field2, [0, 0, 0], MP4, F64)
+    assert torch.equal(field, field2)
+    # the oracle at full size
+    t = oracle.topo(3, [N + 1] * 3)
+    xh, wh, xqh = x.cpu().numpy(), w.cpu().numpy(), xq.cpu().numpy()
+    info, fref = oracle.p2m_par(t, MP4, xh, wh, [2, 2, 2], _threads())
+    assert info == 0
+    assert _rel(f64, oracle.unique_nodes(fref, [2, 2, 2], 3)) < 1e-12
+    info, oref = oracle.m2p_par(t, MP4, xqh, fref, [2, 2, 2], _threads())
+    assert info == 0
+    assert _rel(o64, oref) < 1e-12
+    # FP32-compensated form: within 1e-6 of the FP64 field's maximum
+    f32 = torch.empty(mesh.compact_shape(3), dtype=torch.float32, device="cuda")
+    o32 = torch.empty(out.shape, dtype=torch.float32, device="cuda")
+    core.p2m(mesh, x, w.float(), f32, [0, 0, 0], MP4, F32C)
+    core.m2p(mesh, xq, f32, [0, 0, 0], o32, MP4, F32C)
+    assert _rel(f32.double().cpu().numpy(), f64) < 1e-6
+    assert _rel(o32.double().cpu().numpy(), o64) < 1e-6
+
+
+def test_config3_adjoint_128_fd(core):
+    """128^3 adjoint, 2.1M jittered particles, lda 1: L = 0.5 sum w m^2 with
+    w ~ N(0,1) per node.  Strength and position gradients by autograd against
+    central differences (step 1e-6 h) of the GPU P2M on 64 sampled particles,
+    formed from the field difference so that only the perturbed stencil
+    contributes rounding."""
+    from ppm_legacy_amd import autograd
+    N = 128
+    print(f"config 3: {N}^3 adjoint", flush=True)
+    mesh = core.Mesh(3, N + 1)
+    x0 = core.gen_jittered_lattice(N, 3)
+    u0 = core.gen_normals(N ** 3, 1, 3)
+    wn = core.gen_normals(N ** 3, 1, 33).reshape(N, N, N, 1)
+    x = x0.clone().requires_grad_(True)
+    u = u0.clone().requires_grad_(True)
+    m = autograd.p2m(x, u, mesh)
+    loss = 0.5 * (wn * m * m).sum()
+    loss.backward()
+    gx, gu = x.grad.cpu().numpy(), u.grad.cpu().numpy()
+    h = 1.0 / N
+    step = 1e-6 * h
+    field = torch.empty(mesh.compact_shape(1), dtype=torch.float64, device="cuda")
+    fp = torch.empty_like(field)
+    core.p2m(mesh, x0, u0, field, [0, 0, 0])
+    rng = np.random.default_rng(0)
+    sample = rng.choice(N ** 3, size=64, replace=False)
+    worst_x, worst_u = 0.0, 0.0
+    scale_x = np.abs(gx).max()
+    scale_u = np.abs(gu).max()
+    for p in sample:
+        p = int(p)
+        for d in range(3):
+            diffs = []
+            for sgn in (1.0, -1.0):
+                xs = x0.clone()
+                xs[p, d] += sgn * step
+                core.p2m(mesh, xs, u0, fp, [0, 0, 0])
+                diffs.append(fp - field)
+            # L(x+) - L(x-) = 0.5 sum w (m+ - m-)(m+ + m-), with m+- = m + diffs
+            dl = 0.5 * (wn * (diffs[0] - diffs[1]) * (2 * field + diffs[0] + diffs[1])).sum().item()
+            worst_x = max(worst_x, abs(dl / (2 * step) - gx[p, d]) / scale_x)
+        # strength: L is quadratic in u_p, so a unit step is exact up to rounding
+        us = u0.clone()
+        us[p, 0] += 1.0
+        core.p2m(mesh, x0, us, fp, [0, 0, 0])
+        dm = fp - field
+        us[p, 0] -= 2.0
+        core.p2m(mesh, x0, us, fp, [0, 0, 0])
+        dm2 = fp - field
+        dl = 0.5 * (wn * (dm - dm2) * (2 * field + dm + dm2)).sum().item()
+        worst_u = max(worst_u, abs(dl / 2.0 - gu[p, 0]) / scale_u)
+    assert worst_x < 1e-6, worst_x
+    assert worst_u < 1e-9, worst_u
+
+
+def test_config4_clustered_512_vs_oracle(core, oracle):
+    """512^3, 64M particles: 80% from 4 Gaussian blobs (sigma 0.05), 20%
+    uniform, 3 components.  A blob of 16.8M particles with sigma 0.05 puts
+    about 63 particles in a cell at its centre (the maximum comes out near
+    80).  FP64 P2M vs the
+    oracle on every node; M2P of that field at 1M of the particles; mass,
+    repeatability, and the FP32-compensated form against FP64."""
+    N, n = 512, 1 << 26
+    print(f"config 4: {N}^3, {n} clustered particles", flush=True)
+    # generated on the device: blob centres from the workload's Philox streams,
+    # Gaussian offsets from the Philox normal generator, uniform part from a
+    # seeded torch stream
+    centres = torch.from_numpy(np.stack([wl.uniforms(4, wl.STREAM_BLOB + d, 4)[0] for d in range(3)], 1)).cuda()
+    g = core.gen_normals(n, 3, 44)
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    uni = torch.rand((n, 3), dtype=torch.float64, device="cuda", generator=gen)
+    idx = torch.arange(n, device="cuda")
+    x = torch.where((idx < int(0.8 * n))[:, None], centres[idx % 4] + 0.05 * g, uni)
+    del g, uni, idx
+    x = x - torch.floor(x)
+    x = torch.where(x >= 1.0, x - 1.0, x).contiguous()
+    w = core.gen_normals(n, 3, 45)
+    xh, wh = x.cpu().numpy(), w.cpu().numpy()
+    mesh = core.Mesh(3, N + 1)
+    field = torch.empty(mesh.compact_shape(3), dtype=torch.float64, device="cuda")
+    plan = core.p2m(mesh, x, w, field, [0, 0, 0], MP4, F64)
+    _, _, start = plan.sort_view()
+    occ = torch.diff(start.long()).max().item()
+    print(f"max cell occupancy {occ}", flush=True)
+    assert occ > 50
+    field2 = torch.empty_like(field)
+    core.p2m(mesh, x, w, field2, [0, 0, 0], MP4, F64)
+    assert torch.equal(field, field2)
+    del field2
+    f64 = field.cpu().numpy()
+    assert np.allclose(f64.reshape(-1, 3).sum(0), wh.sum(0), rtol=0, atol=1e-9 * np.abs(wh).sum())
+    t = oracle.topo(3, [N + 1] * 3)
+    info, fref = oracle.p2m_par(t, MP4, xh, wh, [2, 2, 2], _threads())
+    assert info == 0
+    assert _rel(f64, oracle.unique_nodes(fref, [2, 2, 2], 3)) < 1e-12
+    sub = np.ascontiguousarray(xh[:: 64])
+    out = torch.empty((len(sub), 3), dtype=torch.float64, device="cuda")
+    core.m2p(mesh, torch.from_numpy(sub).cuda(), field, [0, 0, 0], out, MP4, F64)
+    info, oref = oracle.m2p_par(t, MP4, sub, fref, [2, 2, 2], _threads())
+    assert info == 0
+    assert _rel(out.cpu().numpy(), oref) < 1e-12
+    del fref
+    f32 = torch.empty(mesh.compact_shape(3), dtype=torch.float32, device="cuda")
+    core.p2m(mesh, x, w.float(), f32, [0, 0, 0], MP4, F32C)
+    assert _rel(f32.double().cpu().numpy(), f64) < 1e-6
