@@ -1,0 +1,251 @@
+// ppmg_adj3d.hip -- adjoint of the 3D M'4 P2M (no reference counterpart),
+// tiled for gfx950 on the structure of the M2P gather (ppmg_m2p3d.hip).
+//
+// For a loss L of the P2M field with cotangent g on the unique nodes:
+//   dL/du_p,c = sum_n W(x_n - x_p) g_n,c                     (M2P of g)
+//   dL/dx_p,d = sum_c u_p,c sum_n dW/dx_p,d (x_n - x_p) g_n,c
+// with dW/dx_d the M'4 derivative stencil along d times dxi_d.  The forward's
+// binning (permutation, cell heads) is reused.
+//
+// Layout and lane mapping as the M2P tile: a workgroup (4 wavefronts, 16
+// rows of 16 lanes) owns 13 x 16 cells in x,y and marches a z chunk; an LDS
+// ring holds 4 planes of g (16 x 19 nodes, component-major).  Lane ex of row
+// `row` reads the 4x4 (y,z) column of node column tx0-1+ex and contracts it,
+// for the particles of lanes ex+1, ex, ex-1, ex-2 (k = 0..3), three ways:
+//   C0_k = sum wy wz g,   Cy_k = sum dwy wz g,   Cz_k = sum wy dwz g
+// (the y and z weights of those particles arrive by DPP row shifts); the
+// partial sums return to their particles, where the x weights finish them:
+//   dL/du_c = sum_k wx_k C0_k,         dL/dx_0 = sum_c u_c sum_k dwx_k C0_k,
+//   dL/dx_1 = sum_c u_c sum_k wx_k Cy_k, dL/dx_2 = sum_c u_c sum_k wx_k Cz_k.
+// All sums are FP64 in registers; the FP32 forms keep FP32 only for g, u and
+// the outputs.
+#include <atomic>
+
+#include "ppmg_internal.h"
+#include "ppmg_interp.h"
+
+namespace ppmg {
+
+constexpr int A3_TX = 13;
+constexpr int A3_TY = 16;
+constexpr int A3_RY = A3_TY + 3;
+constexpr int A3_PL = A3_RY * 16;
+constexpr int A3_THREADS = 16 * A3_TY;
+
+template <int LDA, typename PosT, typename ValT, typename AccT>
+__global__ void __launch_bounds__(A3_THREADS)
+k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
+          const ValT* __restrict__ up, const ValT* __restrict__ g, ValT* __restrict__ grad_u,
+          ValT* __restrict__ grad_x, int lda, int c0, int tz, int accumulate_x) {
+    __shared__ ValT ring[4 * LDA * A3_PL];   // [plane][c][ry][nx]
+    const int tid = threadIdx.x;
+    const int ex = tid & 15, row = tid >> 4;
+    const int Nx = m.N[0], Ny = m.N[1], Nz = m.N[2];
+    const int ntx = (Nx + A3_TX - 1) / A3_TX, nty = (Ny + A3_TY - 1) / A3_TY;
+    const int64_t tile = xcd_tile(blockIdx.x, (int64_t)gridDim.x);
+    const int tx0 = (int)(tile % ntx) * A3_TX, ty0 = (int)((tile / ntx) % nty) * A3_TY;
+    const int z0 = (int)(tile / ((int64_t)ntx * nty)) * tz;
+    const int z1 = min(z0 + tz, Nz);
+    const int nxo = min(A3_TX, Nx - tx0), nyo = min(A3_TY, Ny - ty0);
+    const int64_t plane = (int64_t)Nx * Ny;
+
+    constexpr int NLD = (A3_PL * LDA + A3_THREADS - 1) / A3_THREADS;
+    ValT stage[NLD];
+    auto fetch_plane = [&](int n) {
+        int nz = n % Nz;
+        if (nz < 0) nz += Nz;
+#pragma unroll
+        for (int j = 0; j < NLD; ++j) {
+            const int i = tid + j * A3_THREADS;
+            if (i < A3_PL * LDA) {
+                const int c = i % LDA;
+                const int node = i / LDA;
+                const int nx = node & 15, ry = node >> 4;
+                int gx = (tx0 - 1 + nx) % Nx;
+                if (gx < 0) gx += Nx;
+                int gy = (ty0 - 1 + ry) % Ny;
+                if (gy < 0) gy += Ny;
+                stage[j] = g[((int64_t)nz * plane + (int64_t)gy * Nx + gx) * lda + c0 + c];
+            }
+        }
+    };
+    auto store_plane = [&](int n) {
+        ValT* dst = ring + (n & 3) * (LDA * A3_PL);
+#pragma unroll
+        for (int j = 0; j < NLD; ++j) {
+            const int i = tid + j * A3_THREADS;
+            if (i < A3_PL * LDA) dst[(i % LDA) * A3_PL + i / LDA] = stage[j];
+        }
+    };
+    for (int n = z0 - 1; n <= z0 + 2; ++n) {
+        fetch_plane(n);
+        store_plane(n);
+    }
+
+    const bool plane_lane = ex >= 1 && ex <= nxo && row < nyo;
+    const int cx = tx0 + ex - 1, cy = ty0 + row;
+    const int key0 = (int)((int64_t)cy * Nx + cx);
+    auto heads = [&](int z, int& f, int& n) {
+        f = 0;
+        n = 0;
+        if (plane_lane && z < z1) {
+            const int key = (int)((int64_t)z * plane) + key0;
+            f = start[key];
+            n = start[key + 1] - f;
+        }
+    };
+    int first_n, cnt_n;
+    heads(z0, first_n, cnt_n);
+    __syncthreads();
+
+    for (int cz = z0; cz < z1; ++cz) {
+        const int first = first_n, cnt = cnt_n;
+        heads(cz + 1, first_n, cnt_n);
+        const bool more = cz + 1 < z1;
+        if (more) fetch_plane(cz + 3);
+        for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
+            const bool has = cnt > r;
+            int p = 0;
+            AccT wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, wz[4] = {0, 0, 0, 0};
+            AccT dx[4] = {0, 0, 0, 0}, dy[4] = {0, 0, 0, 0}, dz[4] = {0, 0, 0, 0};
+            AccT u[LDA];
+#pragma unroll
+            for (int c = 0; c < LDA; ++c) u[c] = AccT(0);
+            if (has) {
+                p = perm[first + r];
+                const PosT* x = xp + (int64_t)p * 3;
+                int ci;
+                const AccT f0 = (AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci);
+                const AccT f1 = (AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci);
+                const AccT f2 = (AccT)cell_offset<PosT>(x[2], (PosT)m.minv[2], (PosT)m.dxi[2], ci);
+                mp4_unrolled<AccT>(f0, wx);
+                mp4_unrolled<AccT>(f1, wy);
+                mp4_unrolled<AccT>(f2, wz);
+                mp4_unrolled_deriv<AccT>(f0, dx);
+                mp4_unrolled_deriv<AccT>(f1, dy);
+                mp4_unrolled_deriv<AccT>(f2, dz);
+#pragma unroll
+                for (int o = 0; o < 4; ++o) {
+                    dx[o] *= (AccT)m.dxi[0];
+                    dy[o] *= (AccT)m.dxi[1];
+                    dz[o] *= (AccT)m.dxi[2];
+                }
+#pragma unroll
+                for (int c = 0; c < LDA; ++c) u[c] = (AccT)up[(int64_t)p * lda + c0 + c];
+            }
+            // y and z weights (and derivatives) of the particles of lanes ex+1, ex, ex-1, ex-2
+            AccT a[4][4], da[4][4], b[4][4], db[4][4];
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                a[0][o] = row_shl<1>(wy[o]); da[0][o] = row_shl<1>(dy[o]);
+                b[0][o] = row_shl<1>(wz[o]); db[0][o] = row_shl<1>(dz[o]);
+                a[1][o] = wy[o];             da[1][o] = dy[o];
+                b[1][o] = wz[o];             db[1][o] = dz[o];
+                a[2][o] = row_shr<1>(wy[o]); da[2][o] = row_shr<1>(dy[o]);
+                b[2][o] = row_shr<1>(wz[o]); db[2][o] = row_shr<1>(dz[o]);
+                a[3][o] = row_shr<2>(wy[o]); da[3][o] = row_shr<2>(dy[o]);
+                b[3][o] = row_shr<2>(wz[o]); db[3][o] = row_shr<2>(dz[o]);
+            }
+            AccT gx0 = AccT(0), gx1 = AccT(0), gx2 = AccT(0);
+#pragma unroll
+            for (int c = 0; c < LDA; ++c) {
+                AccT C0[4] = {0, 0, 0, 0}, Cy[4] = {0, 0, 0, 0}, Cz[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int oz = 0; oz < 4; ++oz) {
+                    const ValT* f = ring + ((cz - 1 + oz) & 3) * (LDA * A3_PL) + c * A3_PL + row * 16 + ex;
+                    const AccT g0 = (AccT)f[0], g1 = (AccT)f[16], g2 = (AccT)f[32], g3 = (AccT)f[48];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        AccT t = a[k][0] * g0;
+                        t += a[k][1] * g1;
+                        t += a[k][2] * g2;
+                        t += a[k][3] * g3;
+                        AccT td = da[k][0] * g0;
+                        td += da[k][1] * g1;
+                        td += da[k][2] * g2;
+                        td += da[k][3] * g3;
+                        C0[k] += b[k][oz] * t;
+                        Cz[k] += db[k][oz] * t;
+                        Cy[k] += b[k][oz] * td;
+                    }
+                }
+                // back to the particle lanes: C_k of lane P-1+k
+                const AccT s00 = row_shr<1>(C0[0]), s02 = row_shl<1>(C0[2]), s03 = row_shl<2>(C0[3]);
+                const AccT sy0 = row_shr<1>(Cy[0]), sy2 = row_shl<1>(Cy[2]), sy3 = row_shl<2>(Cy[3]);
+                const AccT sz0 = row_shr<1>(Cz[0]), sz2 = row_shl<1>(Cz[2]), sz3 = row_shl<2>(Cz[3]);
+                const AccT gu = wx[0] * s00 + wx[1] * C0[1] + wx[2] * s02 + wx[3] * s03;
+                gx0 += u[c] * (dx[0] * s00 + dx[1] * C0[1] + dx[2] * s02 + dx[3] * s03);
+                gx1 += u[c] * (wx[0] * sy0 + wx[1] * Cy[1] + wx[2] * sy2 + wx[3] * sy3);
+                gx2 += u[c] * (wx[0] * sz0 + wx[1] * Cz[1] + wx[2] * sz2 + wx[3] * sz3);
+                if (has && grad_u) grad_u[(int64_t)p * lda + c0 + c] = (ValT)gu;
+            }
+            if (has && grad_x) {
+                ValT* o = grad_x + (int64_t)p * 3;
+                if (accumulate_x) {
+                    o[0] = (ValT)((AccT)o[0] + gx0);
+                    o[1] = (ValT)((AccT)o[1] + gx1);
+                    o[2] = (ValT)((AccT)o[2] + gx2);
+                } else {
+                    o[0] = (ValT)gx0;
+                    o[1] = (ValT)gx1;
+                    o[2] = (ValT)gx2;
+                }
+            }
+        }
+        __syncthreads();   // every row is done with plane cz-1: its slot takes plane cz+3
+        if (more) store_plane(cz + 3);
+        __syncthreads();
+    }
+}
+
+template <int LL, typename PosT, typename ValT, typename AccT>
+static void launch_a3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const ValT* g, ValT* gu,
+                      ValT* gx, int lda, int c0, int acc, hipStream_t s) {
+    static std::atomic<int> slots{0};
+    if (slots.load() == 0) slots.store(resident_workgroups(k_a3_tile<LL, PosT, ValT, AccT>, A3_THREADS));
+    const int tx = (m.N[0] + A3_TX - 1) / A3_TX, ty = (m.N[1] + A3_TY - 1) / A3_TY;
+    const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 1);
+    dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
+    hipLaunchKernelGGL((k_a3_tile<LL, PosT, ValT, AccT>), grid, dim3(A3_THREADS), 0, s, m, plan->start.as<int>(),
+                       plan->perm.as<int>(), xp, up, g, gu, gx, lda, c0, tz, acc);
+}
+
+// One launch per component: the four weight sets (y, z and their
+// derivatives) of four neighbouring particles take most of the registers, and
+// one component at a time keeps the kernel at 3 wavefronts per SIMD; the
+// position gradient accumulates over the launches.
+template <typename PosT, typename ValT>
+static int adjoint3d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda, const ValT* up, const ValT* g,
+                           ValT* gu, ValT* gx, hipStream_t s) {
+    for (int c0 = 0; c0 < lda; ++c0) {
+        launch_a3<1, PosT, ValT, double>(plan, m, xp, up, g, gu, gx, lda, c0, c0 > 0 ? 1 : 0, s);
+        PPMG_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+bool adjoint3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
+    if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return false;
+    for (int d = 0; d < 3; ++d) {
+        if (!m.periodic[d] || m.N[d] < 4) return false;
+        if (fg.lo[d] != 0 || fg.ext[d] != m.N[d]) return false;
+    }
+    return true;
+}
+
+// Uses the forward's binning in the plan (every particle is inside the sub:
+// the forward P2M would have failed otherwise).
+int adjoint3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
+                   const void* g, void* gu, void* gx, hipStream_t s) {
+    if (np <= 0) return 0;
+    if (prec == PPMG_PREC_F64)
+        return adjoint3d_typed<double, double>(plan, m, (const double*)xp, lda, (const double*)up, (const double*)g,
+                                               (double*)gu, (double*)gx, s);
+    if (prec == PPMG_PREC_F32C)
+        return adjoint3d_typed<double, float>(plan, m, (const double*)xp, lda, (const float*)up, (const float*)g,
+                                              (float*)gu, (float*)gx, s);
+    return adjoint3d_typed<float, float>(plan, m, (const float*)xp, lda, (const float*)up, (const float*)g,
+                                         (float*)gu, (float*)gx, s);
+}
+
+}  // namespace ppmg

@@ -332,8 +332,16 @@ int ppmg_p2m_adjoint(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, co
                             "periodic adjoint needs the compact cotangent layout (lo = 0, ext = Nm - 1)");
     if (np == 0) return 0;
     mark(plan, as_stream(stream), true);
-    if (adjoint_general(plan, m, kernel, prec, np, xp, lda, up, fg, g->data, grad_up, grad_xp, as_stream(stream)))
+    // PPMG_GENERAL=1 forces the general per-particle path (used to cross-check the tiled one)
+    const char* gen = std::getenv("PPMG_GENERAL");
+    const bool force_general = gen && gen[0] && gen[0] != '0';
+    if (!force_general && adjoint3d_fast_applicable(m, kernel, fg)) {
+        if (adjoint3d_fast(plan, m, prec, np, xp, lda, up, g->data, grad_up, grad_xp, as_stream(stream)))
+            return g_status.info;
+    } else if (adjoint_general(plan, m, kernel, prec, np, xp, lda, up, fg, g->data, grad_up, grad_xp,
+                               as_stream(stream))) {
         return g_status.info;
+    }
     mark(plan, as_stream(stream));
     return 0;
 }

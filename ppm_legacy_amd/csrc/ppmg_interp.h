@@ -25,6 +25,10 @@ int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const vo
 
 // 3D M'4 tiled gather (ppmg_m2p3d.hip): bins the particles into the plan first;
 // returns 1 without touching up when a particle lies outside the sub.
+// Tiled 3D M'4 adjoint (ppmg_adj3d.hip), on the forward's binning.
+bool adjoint3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
+int adjoint3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
+                   const void* g, void* gu, void* gx, hipStream_t s);
 bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
 int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* field,
                void* up, hipStream_t s);
