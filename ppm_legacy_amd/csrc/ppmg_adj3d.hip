@@ -23,6 +23,7 @@
 
 #include "ppmg_internal.h"
 #include "ppmg_interp.h"
+#include "ppmg_sort.h"
 
 namespace ppmg {
 
@@ -112,7 +113,7 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
 #pragma unroll
             for (int c = 0; c < LDA; ++c) u[c] = AccT(0);
             if (has) {
-                p = perm[first + r];
+                p = perm ? perm[first + r] : first + r;
                 const PosT* x = xp + (int64_t)p * 3;
                 int ci;
                 const AccT f0 = (AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci);
@@ -207,7 +208,7 @@ static void launch_a3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const V
     const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 1);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
     hipLaunchKernelGGL((k_a3_tile<LL, PosT, ValT, AccT>), grid, dim3(A3_THREADS), 0, s, m, plan->start.as<int>(),
-                       plan->perm.as<int>(), xp, up, g, gu, gx, lda, c0, tz, acc);
+                       tile_perm(plan), xp, up, g, gu, gx, lda, c0, tz, acc);
 }
 
 // One launch per component: the four weight sets (y, z and their

@@ -183,7 +183,7 @@ int ppmg_plan_create(ppmg_plan** plan) {
 int ppmg_plan_destroy(ppmg_plan* plan) {
     if (!plan) return 0;
     DevBuf* bufs[] = {&plan->key, &plan->slot, &plan->count, &plan->start, &plan->perm, &plan->wsorted,
-                      &plan->usorted, &plan->scan_tmp, &plan->err, &plan->misc, &plan->misc2};
+                      &plan->usorted, &plan->scan_tmp, &plan->err, &plan->misc, &plan->misc2, &plan->gaps};
     for (DevBuf* b : bufs) b->release();
     if (plan->ev_init)
         for (auto& e : plan->ev) (void)hipEventDestroy(e);
@@ -214,6 +214,12 @@ int ppmg_plan_timings(ppmg_plan* plan, double* ms, int n) {
 int ppmg_plan_sort_view(const ppmg_plan* plan, const int** key, const int** perm, const int** start, int64_t* np,
                         int64_t* ncells) {
     if (!plan) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "plan is NULL");
+    if (perm && plan->perm_identity && !plan->perm_written) {
+        // the binning left the identity permutation implicit: write it now
+        ppmg_plan* w = const_cast<ppmg_plan*>(plan);
+        if (materialize_perm(w, w->bin_stream)) return g_status.info;
+        PPMG_HIP(hipStreamSynchronize(w->bin_stream));
+    }
     if (key) *key = plan->key.as<int>();
     if (perm) *perm = plan->perm.as<int>();
     if (start) *start = plan->start.as<int>();

@@ -241,10 +241,13 @@ struct DevBuf {
 
 // Opaque plan: scratch of the binning and the forward record for the adjoint.
 struct ppmg_plan {
-    ppmg::DevBuf key, slot, count, start, perm, wsorted, usorted, scan_tmp, err, misc, misc2;
+    ppmg::DevBuf key, slot, count, start, perm, wsorted, usorted, scan_tmp, err, misc, misc2, gaps;
     int64_t np = 0;
     int64_t ncells = 0;
     bool sorted_input = false;   // last binning found the input already in cell order
+    bool perm_identity = false;  // the permutation of the last binning is the identity ...
+    bool perm_written = true;    // ... and has (or has not yet) been written into `perm`
+    hipStream_t bin_stream = nullptr;   // stream of the last binning
     int dim = 0, kernel = 0, prec = 0;
     ppmg::DevMesh mesh{};
     bool have_forward = false;

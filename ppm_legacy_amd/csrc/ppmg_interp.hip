@@ -404,6 +404,7 @@ static int p2m_typed(ppmg_plan* plan, const DevMesh& m, int kernel, int64_t np, 
     if (plan->wsorted.ensure(wbytes) || plan->usorted.ensure(ubytes)) return PPMG_ERROR_FATAL;
     ValT* ws = plan->wsorted.as<ValT>();
     ValT* us = plan->usorted.as<ValT>();
+    if (materialize_perm(plan, s)) return PPMG_ERROR_FATAL;
     const int* perm = plan->perm.as<int>();
     const ValT* t1 = wtab ? wtab[0] : nullptr;
     const ValT* t2 = wtab ? wtab[1] : nullptr;
@@ -488,6 +489,7 @@ static int adjoint_typed(ppmg_plan* plan, const DevMesh& m, int kernel, int64_t 
     const int KS = kernel == PPMG_KERNEL_MP4 ? 4 : 2;
     if (np <= 0) return 0;
     unsigned gr = nblk(np, 256);
+    if (materialize_perm(plan, s)) return PPMG_ERROR_FATAL;
     const int* perm = plan->perm.as<int>();
     const int* key = plan->key.as<int>();
     for (int c0 = 0; c0 < lda; c0 += 4) {

@@ -113,7 +113,7 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         p = 0;
         x[0] = x[1] = x[2] = PosT(0);
         if (n > 0) {
-            p = perm[f];
+            p = perm ? perm[f] : f;
             x[0] = xp[(int64_t)p * 3];
             x[1] = xp[(int64_t)p * 3 + 1];
             x[2] = xp[(int64_t)p * 3 + 2];
@@ -201,7 +201,7 @@ static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda
     const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 1);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
     hipLaunchKernelGGL(kernel, grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
-                       plan->perm.as<int>(), xp, field, up, lda, c0, tz);
+                       tile_perm(plan), xp, field, up, lda, c0, tz);
 }
 
 template <typename PosT, typename ValT, typename AccT>

@@ -137,7 +137,7 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         if (n > 0) {
             // offsets (INT truncation as ppm_interp_p2m.f:800-822, in the position
             // precision) and strengths, read through the permutation
-            const int p = perm[f];
+            const int p = perm ? perm[f] : f;
             const PosT* x = xp + (int64_t)p * 3;
             int ci;
 #pragma unroll
@@ -239,7 +239,7 @@ static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const V
     const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 3);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
     hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT, P3_TY>), grid, dim3(64), 0, s, m, plan->start.as<int>(),
-                       plan->perm.as<int>(), xp, up, lda, c0, tz, field);
+                       tile_perm(plan), xp, up, lda, c0, tz, field);
 }
 
 template <typename PosT, typename ValT>

@@ -19,6 +19,12 @@ int fill_zero(void* p, size_t bytes, hipStream_t s);
 // Bins np particles into the plan (key, perm, start); *nbad = particles outside the sub.
 int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp,
                   hipStream_t s, int64_t* nbad);
+// The binning leaves the identity permutation implicit when the input is
+// already in cell order.  Tiled kernels take tile_perm(): nullptr then, and
+// they read particle f at sorted position f; other readers of plan->perm call
+// materialize_perm() first.
+inline const int* tile_perm(const ppmg_plan* plan) { return plan->perm_identity ? nullptr : plan->perm.as<int>(); }
+int materialize_perm(ppmg_plan* plan, hipStream_t s);
 int rank_particles(ppmg_plan* plan, const RankGeom& g, int prec, int64_t np, const void* xp,
                    int* lpdx, int* lhbx, int64_t* icount, hipStream_t s);
 

@@ -163,37 +163,55 @@ __device__ __forceinline__ int cell_key(const PosT* __restrict__ x, const DevMes
     return ok ? cell[0] + m.nbin[0] * (cell[1] + m.nbin[1] * cell[2]) : -1;
 }
 
+// Heads of the cells lo..hi (all empty but possibly the last) := v.  Short
+// runs are written by the lane itself; longer ones are queued for
+// k_fill_gaps in pieces of GAP_CHUNK cells (err[2] counts the entries; an
+// overflowing queue sends the caller to the counting path).
+constexpr int GAP_LOCAL = 32;
+constexpr int GAP_CHUNK = 65536;
+
+__device__ __forceinline__ void set_heads(int* __restrict__ start, int lo, int hi, int v, int* __restrict__ err,
+                                          int4* __restrict__ gaps, int gapcap) {
+    if (hi - lo < GAP_LOCAL) {
+        for (int c = lo; c <= hi; ++c) start[c] = v;
+        return;
+    }
+    for (int64_t a = lo; a <= hi; a += GAP_CHUNK) {
+        const int e = (int)(a + GAP_CHUNK - 1 < hi ? a + GAP_CHUNK - 1 : hi);
+        const int i = atomicAdd(&err[2], 1);
+        if (i < gapcap) gaps[i] = make_int4((int)a, e, v, 0);
+    }
+}
+
 // Keys of all particles.  err[0] counts the particles in no cell; err[1] is
 // set to 1 when a descent key[p] < key[p-1] exists (none: the input is already
-// in cell order, e.g. particles made by create_part or a lattice).  Inside a
-// wavefront the neighbours' keys come by shuffles; descents across wavefront
-// boundaries are checked afterwards by k_wave_seams.  The flag is read before
-// it is written, so the many wavefronts of unsorted input do not queue on one
-// word.
+// in cell order, e.g. particles made by create_part or a lattice).  Lanes get
+// their predecessor's key by a shuffle; lane 0 of each wavefront recomputes it
+// from the previous particle's position, so descents across wavefront seams
+// are caught here too.  The flag is read before it is written, so the many
+// wavefronts of unsorted input do not queue on one word.
 //   SLOT (unsorted input expected): take each particle's arrival slot in its
 //     cell by an atomic count.
-//   !SLOT (input expected in cell order): write the identity permutation and,
-//     at the first particle of each run of equal keys inside the wavefront,
-//     the run length as the cell's count (k_wave_seams then rewrites the
-//     counts of runs that cross a wavefront seam); valid only if no descent is
-//     found (else the caller clears the counts and counts again).
+//   !SLOT (input expected in cell order): the first particle of each run of
+//     equal keys writes its index as the head of its cell and of the empty
+//     cells before it (back to its predecessor's cell); the last particle
+//     closes the array with np.  No counts, no scan, and the permutation is
+//     the identity.  Valid only if no descent is found (else the caller counts
+//     from the keys).
 template <int DIM, typename PosT, bool SLOT>
-__global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64_t np, DevMesh m,
+__global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64_t np, DevMesh m, int64_t ncells,
                                               int* __restrict__ key, int* __restrict__ slot,
-                                              int* __restrict__ count, int* __restrict__ perm,
-                                              int* __restrict__ err) {
+                                              int* __restrict__ count, int* __restrict__ start,
+                                              int* __restrict__ err, int4* __restrict__ gaps, int gapcap) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool live = p < np;
     const int k = live ? cell_key<DIM, PosT>(xp + p * DIM, m) : -1;
-    // neighbours' keys by shuffles while every lane is still active
-    const int left = __shfl_up(k, 1);
-    const int right = __shfl_down(k, 1);
+    int left = __shfl_up(k, 1);
+    if (lane == 0) left = live && p > 0 ? cell_key<DIM, PosT>(xp + (p - 1) * DIM, m) : -1;
     const bool bad = live && k < 0;
-    const bool desc = live && lane > 0 && k < left;
+    const bool desc = live && p > 0 && k < left;
     const uint64_t bb = __ballot(bad), bd = __ballot(desc);
-    // lanes whose successor in the wavefront has the same key
-    const uint64_t same_next = __ballot(live && lane < 63 && right == k);
     if (lane == 0) {
         if (bb) atomicAdd(&err[0], (int)__popcll(bb));
         if (bd && *(volatile int*)&err[1] == 0) err[1] = 1;
@@ -203,34 +221,19 @@ __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64
     if (k < 0) return;
     if (SLOT) {
         slot[p] = atomicAdd(&count[k], 1);
-    } else {
-        perm[p] = (int)p;
-        if (lane > 0 && left == k) return;            // not the first of its run in this wavefront
-        // run length inside the wavefront: 1 + the unbroken ones of same_next from this lane on
-        const uint64_t tail = ~(same_next >> lane);
-        count[k] = 1 + (tail ? __builtin_ctzll(tail) : 64 - lane);
+        return;
     }
+    if (desc) return;
+    if (k != left) set_heads(start, left + 1, k, (int)p, err, gaps, gapcap);
+    if (p == np - 1) set_heads(start, k + 1, (int)ncells, (int)np, err, gaps, gapcap);
 }
 
-// Wavefront seams b = 64, 128, ...: descents key[b] < key[b-1], and (count
-// non-null, input expected in cell order) the full length of every run of
-// equal keys that crosses a seam, replacing the per-wavefront lengths; a run
-// crossing several seams gets the same value from each.
-__global__ void __launch_bounds__(256) k_wave_seams(const int* __restrict__ key, int64_t np, int* __restrict__ err,
-                                                    int* __restrict__ count) {
-    const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x + 1) * 64;
-    bool desc = false;
-    if (b < np) {
-        const int k = key[b], kl = key[b - 1];
-        desc = k < kl;
-        if (count && k == kl && k >= 0) {
-            int64_t s0 = b - 1, e = b + 1;
-            while (s0 > 0 && key[s0 - 1] == k) --s0;
-            while (e < np && key[e] == k) ++e;
-            count[k] = (int)(e - s0);
-        }
+// Long runs of empty cells queued by k_keys: one workgroup per entry.
+__global__ void __launch_bounds__(256) k_fill_gaps(const int4* __restrict__ gaps, int n, int* __restrict__ start) {
+    for (int g = blockIdx.x; g < n; g += gridDim.x) {
+        const int4 e = gaps[g];
+        for (int c = e.x + (int)threadIdx.x; c <= e.y; c += 256) start[c] = e.z;
     }
-    if (__ballot(desc) && (threadIdx.x & 63) == 0 && *(volatile int*)&err[1] == 0) err[1] = 1;
 }
 
 // Unsorted input: arrival slot of every particle in its cell.
@@ -340,27 +343,40 @@ int fill_zero(void* p, size_t bytes, hipStream_t s) {
 int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp,
                   hipStream_t s, int64_t* nbad) {
     int64_t ncells = (int64_t)m.nbin[0] * m.nbin[1] * m.nbin[2];
+    const int64_t gapcap = np / 16 + 4096;
     if (plan->key.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
         plan->slot.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
         plan->perm.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
         plan->count.ensure(sizeof(int) * (size_t)ncells) ||
         plan->start.ensure(sizeof(int) * (size_t)(ncells + 1)) ||
+        plan->gaps.ensure(sizeof(int4) * (size_t)gapcap) ||
         plan->err.ensure(sizeof(int) * 4))
         return PPMG_ERROR_FATAL;
-    if (fill_zero(plan->count.p, sizeof(int) * (size_t)ncells, s) || fill_zero(plan->err.p, sizeof(int) * 4, s))
-        return PPMG_ERROR_FATAL;
+    plan->bin_stream = s;
+    plan->np = np;
+    plan->ncells = ncells;
+    *nbad = 0;
+    if (np == 0) {
+        plan->perm_identity = true;
+        plan->perm_written = true;
+        return fill_zero(plan->start.p, sizeof(int) * (size_t)(ncells + 1), s);
+    }
+    if (fill_zero(plan->err.p, sizeof(int) * 4, s)) return PPMG_ERROR_FATAL;
     // The plan remembers whether its last input was in cell order; a plan fed
     // the same kind of particles each call (advected particles, or particles
     // fresh from create_part) predicts right and does each step once.
     const bool predict_sorted = plan->sorted_input;
-    if (np > 0) {
-        unsigned g = nblk(np, 256);
-        int* key = plan->key.as<int>();
-        int* slot = plan->slot.as<int>();
-        int* count = plan->count.as<int>();
-        int* err = plan->err.as<int>();
-#define KEYS(D, T, S) hipLaunchKernelGGL((k_keys<D, T, S>), dim3(g), dim3(256), 0, s, (const T*)xp, np, m, key, slot, \
-                                         count, plan->perm.as<int>(), err)
+    if (!predict_sorted && fill_zero(plan->count.p, sizeof(int) * (size_t)ncells, s)) return PPMG_ERROR_FATAL;
+    int* key = plan->key.as<int>();
+    int* slot = plan->slot.as<int>();
+    int* count = plan->count.as<int>();
+    int* start = plan->start.as<int>();
+    int* err = plan->err.as<int>();
+    int4* gaps = plan->gaps.as<int4>();
+    {
+        const unsigned g = nblk(np, 256);
+#define KEYS(D, T, S) hipLaunchKernelGGL((k_keys<D, T, S>), dim3(g), dim3(256), 0, s, (const T*)xp, np, m, ncells, key, \
+                                         slot, count, start, err, gaps, (int)gapcap)
         const bool single = prec == PPMG_PREC_F32;
         if (m.dim == 3) {
             if (single) { if (predict_sorted) KEYS(3, float, false); else KEYS(3, float, true); }
@@ -370,41 +386,47 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
             else { if (predict_sorted) KEYS(2, double, false); else KEYS(2, double, true); }
         }
 #undef KEYS
-        const int64_t seams = (np - 1) / 64;
-        if (seams > 0)
-            hipLaunchKernelGGL(k_wave_seams, dim3(nblk(seams, 256)), dim3(256), 0, s, key, np, err,
-                               predict_sorted ? count : nullptr);
         PPMG_HIP(hipGetLastError());
     }
-    int herr[2] = {0, 0};
-    PPMG_HIP(hipMemcpyAsync(herr, plan->err.p, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    int herr[3] = {0, 0, 0};
+    PPMG_HIP(hipMemcpyAsync(herr, plan->err.p, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
     PPMG_HIP(hipStreamSynchronize(s));
     *nbad = herr[0];
     if (herr[0]) return 0;
     const bool sorted = herr[1] == 0;
-    if (np > 0) {
-        if (sorted && !predict_sorted) {
-            hipLaunchKernelGGL(k_identity, dim3(nblk(np, 256)), dim3(256), 0, s, np, plan->perm.as<int>());
-        } else if (!sorted && predict_sorted) {
-            // the run lengths written under the wrong prediction are not counts
+    if (predict_sorted && sorted && herr[2] <= gapcap) {
+        // the heads are written but for the queued runs of empty cells
+        if (herr[2] > 0)
+            hipLaunchKernelGGL(k_fill_gaps, dim3((unsigned)(herr[2] < 4096 ? herr[2] : 4096)), dim3(256), 0, s, gaps,
+                               herr[2], start);
+    } else {
+        if (predict_sorted) {
+            // wrong prediction (or too many long gaps): count from the keys
             if (fill_zero(plan->count.p, sizeof(int) * (size_t)ncells, s)) return PPMG_ERROR_FATAL;
-            hipLaunchKernelGGL(k_count, dim3(nblk(np, 256)), dim3(256), 0, s, plan->key.as<int>(), np,
-                               plan->slot.as<int>(), plan->count.as<int>());
+            hipLaunchKernelGGL(k_count, dim3(nblk(np, 256)), dim3(256), 0, s, key, np, slot, count);
         }
-        PPMG_HIP(hipGetLastError());
+        if (exclusive_scan(count, ncells, start, plan->scan_tmp, np, s)) return PPMG_ERROR_FATAL;
+        if (!sorted) {
+            hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 256)), dim3(256), 0, s, key, slot, start, np,
+                               plan->perm.as<int>());
+            hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, start, ncells,
+                               plan->perm.as<int>());
+        }
     }
-    if (exclusive_scan(plan->count.as<int>(), ncells, plan->start.as<int>(), plan->scan_tmp, np, s))
-        return PPMG_ERROR_FATAL;
-    if (np > 0 && !sorted) {
-        hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 256)), dim3(256), 0, s, plan->key.as<int>(),
-                           plan->slot.as<int>(), plan->start.as<int>(), np, plan->perm.as<int>());
-        hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, plan->start.as<int>(),
-                           ncells, plan->perm.as<int>());
-        PPMG_HIP(hipGetLastError());
-    }
+    PPMG_HIP(hipGetLastError());
+    // input in cell order: the permutation is the identity, written only on demand
+    plan->perm_identity = sorted;
+    plan->perm_written = !sorted;
     plan->sorted_input = sorted;
-    plan->np = np;
-    plan->ncells = ncells;
+    return 0;
+}
+
+int materialize_perm(ppmg_plan* plan, hipStream_t s) {
+    if (plan->perm_identity && !plan->perm_written && plan->np > 0) {
+        hipLaunchKernelGGL(k_identity, dim3(nblk(plan->np, 256)), dim3(256), 0, s, plan->np, plan->perm.as<int>());
+        PPMG_HIP(hipGetLastError());
+    }
+    plan->perm_written = true;
     return 0;
 }
 

@@ -115,6 +115,60 @@ def test_binning_sorted_and_unsorted_input(core, oracle):
         assert _rel(field.cpu().numpy(), ref) < 1e-12, order
 
 
+@pytest.mark.parametrize("case", ["lattice", "long_gaps_overflow", "head_tail_chunks", "one_cell"])
+def test_binning_direct_heads(core, oracle, case):
+    """Input in cell order: the binning writes the cell heads straight from
+    the keys (runs of up to 32 empty cells by the particle's lane, longer runs
+    queued in pieces of 65536 cells, a full queue falls back to counting) and
+    leaves the permutation implicit.  Cases: a jittered lattice (one particle
+    per cell); 6000 particles 40 cells apart (more long gaps than the queue
+    holds); particles only in z planes 30..33 of 64 (head and tail runs of
+    over 65536 cells); all particles in one cell.  The plan is fed the same
+    input twice (first call predicts unsorted input, second sorted): heads,
+    identity permutation and keys must equal numpy's, the tiled P2M must
+    match the oracle and repeat bitwise, and the tiled M2P must match the
+    oracle in the caller's order."""
+    N = 16 if case == "lattice" else 64
+    rng = np.random.default_rng({"lattice": 1, "long_gaps_overflow": 2, "head_tail_chunks": 3, "one_cell": 4}[case])
+    if case == "lattice":
+        cells = np.arange(N ** 3)
+    elif case == "long_gaps_overflow":
+        cells = np.arange(0, N ** 3, 40)[:6000]
+    elif case == "head_tail_chunks":
+        cells = np.sort(rng.integers(30 * N * N, 34 * N * N, size=3000))
+    else:
+        cells = np.full(500, 100000)
+    ijk = np.stack([cells % N, (cells // N) % N, cells // (N * N)], 1)
+    xp = (ijk + rng.uniform(0.05, 0.95, size=ijk.shape)) / N
+    up = rng.normal(size=(len(xp), 3))
+    t = oracle.topo(3, [N + 1] * 3)
+    info, ref = oracle.p2m(t, MP4, xp, up, [1, 1, 1])
+    assert info == 0
+    ref = oracle.unique_nodes(ref, [1, 1, 1], 3)
+    mesh = core.Mesh(3, N + 1)
+    x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
+    heads = np.concatenate([[0], np.cumsum(np.bincount(cells, minlength=N ** 3))])
+    plan = core.Plan()
+    fields = []
+    for call in range(2):
+        f = torch.empty(mesh.compact_shape(3), dtype=torch.float64, device="cuda")
+        core.p2m(mesh, x, u, f, [0] * 3, MP4, F64, plan)
+        assert _rel(f.cpu().numpy(), ref) < 1e-12, (case, call)
+        fields.append(f)
+        k, perm, start = plan.sort_view()
+        assert np.array_equal(start.cpu().numpy(), heads), (case, call)
+        assert np.array_equal(k.cpu().numpy(), cells), (case, call)
+        assert np.array_equal(perm.cpu().numpy(), np.arange(len(xp))), (case, call)
+    assert torch.equal(fields[0], fields[1])
+    uniq = rng.normal(size=(N, N, N, 3))
+    oref = bf.m2p_periodic(xp, uniq, N)
+    mplan = core.Plan()
+    for call in range(2):
+        out = torch.empty((len(xp), 3), dtype=torch.float64, device="cuda")
+        core.m2p(mesh, x, _dev(uniq, torch.float64), [0] * 3, out, MP4, F64, mplan)
+        assert _rel(out.cpu().numpy(), oref) < 1e-12, (case, call)
+
+
 # ----------------------------------------------------------- variants --
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
