@@ -4,22 +4,23 @@
 // 2540-3055) plus its periodic ghost fold (:3670-3716) for fully periodic
 // meshes written in the compact layout (unique nodes only).
 //
-// Work decomposition (one wavefront per workgroup):
-//   * a workgroup owns a 13 x 9 node tile in x,y and a chunk of node planes in z;
+// Work decomposition (three wavefronts per workgroup):
+//   * a workgroup owns a 13 x 21 node tile in x,y and a chunk of node planes in z;
 //   * it marches the cell planes cz = z0-2 .. z1 (the cells whose M'4 stencil
 //     touches its planes) and keeps a ring of 4 node planes in LDS: cell plane
 //     cz deposits into node planes cz-1 .. cz+2, after which node plane cz-1
 //     is complete and is written out with coalesced row stores;
-//   * the 16 x 12 cells whose stencils touch the tile (13 + 3 and 9 + 3 halo)
+//   * the 16 x 24 cells whose stencils touch the tile (13 + 3 and 21 + 3 halo)
 //     are taken 64 at a time, one cell per lane; each lane walks its cell's
 //     particles (binned, in stable order) and adds the owned part of the
 //     4x4x4 stencil.
-// Lanes of one instruction hold distinct cells, hence write distinct nodes,
-// and one wavefront owns the ring: the order of every node's additions is a
-// fixed function of the binning, so results are bitwise reproducible with no
-// global float atomics.  The periodic fold happens through the index wrap of
-// the cells around the tile.
+// Lanes of one instruction hold distinct cells, hence write distinct nodes;
+// the wavefronts of a workgroup work on disjoint node rows between barriers,
+// so the order of every node's additions is a fixed function of the binning,
+// and results are bitwise reproducible with no global float atomics.  The
+// periodic fold happens through the index wrap of the cells around the tile.
 #include <atomic>
+#include <cstdlib>
 
 #include "ppmg_internal.h"
 #include "ppmg_interp.h"
@@ -28,7 +29,7 @@
 namespace ppmg {
 
 constexpr int P3_TT = 13;   // owned nodes per tile along x (16-lane DPP row = 13 + 3 halo cells)
-constexpr int P3_TY = 9;    // owned node rows per tile along y (TY + 3 cell rows = 3 chunks of 4)
+constexpr int P3_TY = 21;   // owned node rows per tile along y (24 cell rows: 6 chunks, 3 wavefronts)
 
 // LDS accumulator of one node component.  Every form accumulates in an FP64
 // slot with ds_add_f64 (fire-and-forget, no read-modify-write latency chain):
@@ -65,31 +66,41 @@ __device__ __forceinline__ void shift_in(const ValT (&wx)[4], const ValT (&wz)[4
 
 // Tiled deposit.  Lane = one cell of the 16 x (TY+3) window: ex = lane & 15
 // along x, so each 16-lane DPP row is a run of 16 consecutive cells at fixed
-// y; the window's cell rows are taken 4 at a time (one per DPP row).  The tile
-// is 13 x 9 nodes: a smaller ring (4 planes x 117 nodes) leaves room for more
-// resident workgroups, which this latency-bound kernel needs more than it
-// needs the lower halo share of a taller tile.  In
-// round r every lane takes the r-th particle of its cell (zero strength when
-// it has none).  Lane ex finishes node ex-3 of its row, which receives x-offset
-// k from the particle of lane ex-k:
+// y; the window's cell rows are taken 4 at a time (one per DPP row), a
+// "chunk".  In round r every lane takes the r-th particle of its cell (zero
+// strength when it has none).  Lane ex finishes node ex-3 of its row, which
+// receives x-offset k from the particle of lane ex-k:
 //   node(ex - 3) += sum_k wx_k(ex-k) u(ex-k) wy_oy(ex-k) wz_oz(ex-k)
 // The 11 inputs of each neighbour (wx_k u, wy, wz) come over by DPP row
 // shifts, the 16 (y,z) sums are formed locally, and each lane adds ONE value
 // per (y, z offset, component) to the ring -- 48 LDS atomics per particle for
 // lda = 3 instead of 192.  The ring is stored component-major (SoA) so a row's
 // 16 lanes hit consecutive 8-byte words.
-template <int LDA, typename PosT, typename ValT, int TY>
-__global__ void __launch_bounds__(64)
+//
+// NW wavefronts share one ring.  With NW = 1 the wavefront takes the chunks
+// of a plane in order.  With NW > 1 the tile has 2*NW chunks per plane and
+// every plane runs in two phases: wavefront w takes chunk 2w in the first and
+// chunk 2w+1 in the second.  Chunk q adds to node rows 4q-3 .. 4q+3, so the
+// chunks of one phase touch disjoint nodes, and a barrier orders the phases:
+// every node receives its additions in an order fixed by the program and the
+// binning, as with one wavefront.  A taller tile (more wavefronts) re-reads
+// fewer halo cells: the window is 16 x (TY+3) cells for 13 x TY nodes.
+template <int LDA, typename PosT, typename ValT, int TY, int NW>
+__global__ void __launch_bounds__(64 * NW)
 k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
           const ValT* __restrict__ up, int lda, int c0, int tz, ValT* __restrict__ field) {
     using A = LdsAcc<ValT>;
     using slot_t = typename A::slot_t;
+    constexpr int NT = 64 * NW;
     constexpr int P3_PL = P3_TT * TY;              // nodes per ring plane
     constexpr int NQ = (TY + 3) / 4;               // chunks of 4 cell rows per plane
     static_assert((TY + 3) % 4 == 0, "tile height + 3 must be a multiple of 4");
+    static_assert(NW == 1 || NQ == 2 * NW, "with several wavefronts each takes one chunk per phase");
+    constexpr int NC = NW == 1 ? NQ : 2;           // chunks per wavefront and plane
     constexpr int PLANE = P3_PL * LDA;            // slots per ring plane: [c][node]
     __shared__ slot_t ring[4 * PLANE];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const int Nx = m.N[0], Ny = m.N[1], Nz = m.N[2];
     const int ntx = (Nx + P3_TT - 1) / P3_TT, nty = (Ny + TY - 1) / TY;
     const int64_t tile = xcd_tile(blockIdx.x, (int64_t)gridDim.x);
@@ -97,34 +108,35 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const int z0 = (int)(tile / ((int64_t)ntx * nty)) * tz;
     const int z1 = min(z0 + tz, Nz);
     const int nxo = min(P3_TT, Nx - tx0), nyo = min(TY, Ny - ty0);
-    for (int i = lane; i < 4 * PLANE; i += 64) A::zero(ring[i]);
+    for (int i = tid; i < 4 * PLANE; i += NT) A::zero(ring[i]);
 
     const int ex = lane & 15, erow = lane >> 4;
     const int lx = ex - 3;                          // node this lane owns after the row sum
     const bool xown = lx >= 0 && lx < nxo;
     int cx = tx0 - 2 + ex;
     cx %= Nx; if (cx < 0) cx += Nx;
-    int keyxy[NQ], ey4[NQ];
+    int keyxy[NC], ey4[NC];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
+    for (int j = 0; j < NC; ++j) {
+        const int q = NW == 1 ? j : 2 * wave + j;
         int ey = q * 4 + erow;
-        ey4[q] = ey;
+        ey4[j] = ey;
         int cy = ty0 - 2 + ey;
         cy %= Ny; if (cy < 0) cy += Ny;
-        keyxy[q] = cx + Nx * cy;
+        keyxy[j] = cx + Nx * cy;
     }
     __syncthreads();
     const int64_t plane_cells = (int64_t)Nx * Ny;
 
     // The cell heads of the step after next are loaded while the current
     // (plane, chunk) step computes.
-    auto heads = [&](int cz, int q, int& f, int& n) {
+    auto heads = [&](int cz, int j, int& f, int& n) {
         f = 0;
         n = 0;
         if (cz <= z1) {
             int czw = cz % Nz;
             if (czw < 0) czw += Nz;
-            const int key = (int)(plane_cells * czw + keyxy[q]);
+            const int key = (int)(plane_cells * czw + keyxy[j]);
             f = start[key];
             n = start[key + 1] - f;
         }
@@ -148,17 +160,19 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     };
     int f1, n1, f2, n2;
     heads(z0 - 2, 0, f1, n1);
-    heads(z0 - 2, 1, f2, n2);
+    if (NC > 1) heads(z0 - 2, 1, f2, n2);
+    else heads(z0 - 1, 0, f2, n2);
 
     for (int cz = z0 - 2; cz <= z1; ++cz) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
+        for (int j = 0; j < NC; ++j) {
             const int first = f1, cnt = n1;
             f1 = f2;
             n1 = n2;
-            if (q + 2 < NQ) heads(cz, q + 2, f2, n2);
-            else heads(cz + 1, q + 2 - NQ, f2, n2);
-            const int ly0 = ey4[q] - 3;              // node row of stencil offset oy = 0
+            if (j + 2 < NC) heads(cz, j + 2, f2, n2);
+            else if (j + 2 - NC < NC) heads(cz + 1, j + 2 - NC, f2, n2);
+            else heads(cz + 2, 0, f2, n2);
+            const int ly0 = ey4[j] - 3;              // node row of stencil offset oy = 0
             for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
                 ValT wx[4], wy[4], wz[4], u[LDA], fr[3];
                 particle(first + r, cnt - r, fr, u);
@@ -197,22 +211,31 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                     }
                 }
             }
+            // end of a phase: the other phase's chunks share node rows with this one's
+            if (NW > 1 || j == NC - 1) __syncthreads();
         }
-        __syncthreads();
-        // node plane cz-1 has all its cells: write it if owned, then recycle the slot
+        // node plane cz-1 has all its cells: write it if owned, then recycle the
+        // slot.  Only owned nodes ever receive additions, so an owned plane is
+        // cleared by the thread that writes each node out (one thread per slot:
+        // no wavefront clears a slot another has yet to read); a plane outside
+        // the chunk is cleared whole.
         const int n = cz - 1;
         slot_t* pl = ring + (n & 3) * PLANE;
         if (n >= z0 && n < z1) {
             const int cntw = nyo * nxo * LDA;
             ValT* out = field + (int64_t)n * plane_cells * lda + c0;
-            for (int i = lane; i < cntw; i += 64) {
+            for (int i = tid; i < cntw; i += NT) {
                 int c = i % LDA;
                 int nd = i / LDA;
                 int x = nd % nxo, y = nd / nxo;
-                out[((int64_t)(ty0 + y) * Nx + (tx0 + x)) * lda + c] = (ValT)A::value(pl[c * P3_PL + x + P3_TT * y]);
+                slot_t& a = pl[c * P3_PL + x + P3_TT * y];
+                const ValT v = (ValT)A::value(a);
+                A::zero(a);
+                out[((int64_t)(ty0 + y) * Nx + (tx0 + x)) * lda + c] = v;
             }
+        } else {
+            for (int i = tid; i < PLANE; i += NT) A::zero(pl[i]);
         }
-        for (int i = lane; i < PLANE; i += 64) A::zero(pl[i]);
         __syncthreads();
     }
 }
@@ -227,19 +250,42 @@ bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGe
     return true;
 }
 
-// One launch of the deposit for components c0 .. c0+LL-1.  The z chunk is
-// chosen from the device's resident workgroup count (queried once per
-// instantiation): each chunk marches 3 extra cell planes.
+// One launch of the deposit for components c0 .. c0+LL-1 with tiles of
+// 13 x TY nodes and NW wavefronts per tile.  The z chunk is chosen from the
+// device's resident workgroup count (queried once per instantiation): each
+// chunk marches 3 extra cell planes.
+template <int LL, typename PosT, typename ValT, int TY, int NW>
+static void launch_p3_shape(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, int lda, int c0,
+                            ValT* field, hipStream_t s) {
+    static std::atomic<int> slots{0};
+    if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT, TY, NW>, 64 * NW));
+    const int tx = (m.N[0] + P3_TT - 1) / P3_TT, ty = (m.N[1] + TY - 1) / TY;
+    const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 3);
+    dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
+    hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT, TY, NW>), grid, dim3(64 * NW), 0, s, m, plan->start.as<int>(),
+                       tile_perm(plan), xp, up, lda, c0, tz, field);
+}
+
+// Tile height: PPMG_P2M_ROWS (9, 13, 21 or 29 node rows: 1, 2, 3 or 4
+// wavefronts per tile) overrides the default of 21, for tuning.  Measured on
+// the 512^3 advected lattice (lda 3, FP64): 5.90, 5.49, 5.39 and 5.59 ms --
+// taller tiles re-read fewer halo cells until the phase barriers and the
+// larger ring (fewer resident workgroups) cost more than they save.
+static int p3_rows() {
+    const char* e = std::getenv("PPMG_P2M_ROWS");
+    const int r = e ? std::atoi(e) : 0;
+    return r == 9 || r == 13 || r == 21 || r == 29 ? r : P3_TY;
+}
+
 template <int LL, typename PosT, typename ValT>
 static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, int lda, int c0, ValT* field,
                       hipStream_t s) {
-    static std::atomic<int> slots{0};
-    if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT, P3_TY>, 64));
-    const int tx = (m.N[0] + P3_TT - 1) / P3_TT, ty = (m.N[1] + P3_TY - 1) / P3_TY;
-    const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 3);
-    dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
-    hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT, P3_TY>), grid, dim3(64), 0, s, m, plan->start.as<int>(),
-                       tile_perm(plan), xp, up, lda, c0, tz, field);
+    switch (p3_rows()) {
+        case 9: launch_p3_shape<LL, PosT, ValT, 9, 1>(plan, m, xp, up, lda, c0, field, s); break;
+        case 13: launch_p3_shape<LL, PosT, ValT, 13, 2>(plan, m, xp, up, lda, c0, field, s); break;
+        case 29: launch_p3_shape<LL, PosT, ValT, 29, 4>(plan, m, xp, up, lda, c0, field, s); break;
+        default: launch_p3_shape<LL, PosT, ValT, 21, 3>(plan, m, xp, up, lda, c0, field, s); break;
+    }
 }
 
 template <typename PosT, typename ValT>

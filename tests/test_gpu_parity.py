@@ -232,6 +232,33 @@ def test_p2m_tiled_fast_path(core, oracle, prec, lda, monkeypatch):
     assert torch.equal(fast, again)
 
 
+@pytest.mark.parametrize("rows", [9, 13, 21, 29])
+@pytest.mark.parametrize("N", [50, 64])
+def test_p2m_tile_heights(core, rows, N, monkeypatch):
+    """Every tile height of the tiled deposit (1 to 4 wavefronts sharing the
+    LDS ring, the planes written out and cleared by all of them): the field
+    equals the general path's to rounding, its unique nodes hold the total
+    strength, and three launches agree bitwise.  N = 50 leaves ragged tiles
+    in x and y for every height."""
+    monkeypatch.setenv("PPMG_P2M_ROWS", str(rows))
+    xp = wl.advect(wl.jittered_lattice(N, seed=rows), wl.fourier_modes(rows), N)[0]
+    up = wl.strengths(rows, len(xp), 3)
+    mesh = core.Mesh(3, N + 1)
+    x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
+    runs = []
+    for _ in range(3):
+        f = torch.empty(mesh.compact_shape(3), dtype=torch.float64, device="cuda")
+        core.p2m(mesh, x, u, f, [0] * 3, MP4, F64)
+        runs.append(f)
+    assert torch.equal(runs[0], runs[1]) and torch.equal(runs[0], runs[2])
+    got = runs[0].cpu().numpy()
+    assert np.allclose(got.reshape(-1, 3).sum(0), up.sum(0), rtol=0, atol=1e-10 * np.abs(up).sum())
+    monkeypatch.setenv("PPMG_GENERAL", "1")
+    gen = torch.empty_like(runs[0])
+    core.p2m(mesh, x, u, gen, [0] * 3, MP4, F64)
+    assert _rel(got, gen.cpu().numpy()) < 1e-13
+
+
 def test_p2m_nonperiodic_axes(core, oracle):
     N = 10
     xp, up = _case(3, N, 800, 11, 2)
