@@ -77,17 +77,40 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const int* __restric
     if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
-// Exclusive scan of the block sums by one workgroup, chunk by chunk.
-__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_sums(int64_t* __restrict__ sums, int64_t nb) {
-    __shared__ int64_t sh[SCAN_BLOCK / 64 + 1];
-    int64_t carry = 0;
-    for (int64_t base = 0; base < nb; base += SCAN_BLOCK) {
-        int64_t idx = base + threadIdx.x;
-        int64_t v = idx < nb ? sums[idx] : 0;
-        int64_t excl;
-        int64_t tot = block_exclusive_scan(v, sh, excl);
-        if (idx < nb) sums[idx] = carry + excl;
-        carry += tot;
+// Exclusive scan of the block sums by one workgroup of SUMS_THREADS threads:
+// each thread totals a contiguous segment, the segment totals are scanned
+// across the workgroup, and each thread rewrites its segment as prefixes.
+constexpr int SUMS_THREADS = 1024;
+__global__ void __launch_bounds__(SUMS_THREADS) k_scan_sums(int64_t* __restrict__ sums, int64_t nb) {
+    __shared__ int64_t sh[SUMS_THREADS / 64 + 1];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int64_t seg = (nb + SUMS_THREADS - 1) / SUMS_THREADS;
+    const int64_t b = t * seg, e = b + seg < nb ? b + seg : nb;
+    int64_t tot = 0;
+    for (int64_t i = b; i < e; ++i) tot += sums[i];
+    // workgroup exclusive scan of the segment totals
+    int64_t x = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wid] = x;
+    __syncthreads();
+    if (t == 0) {
+        int64_t run = 0;
+        for (int w = 0; w < SUMS_THREADS / 64; ++w) {
+            int64_t v = sh[w];
+            sh[w] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    int64_t run = x - tot + sh[wid];
+    for (int64_t i = b; i < e; ++i) {
+        const int64_t v = sums[i];
+        sums[i] = run;
+        run += v;
     }
 }
 
@@ -132,7 +155,7 @@ int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, int64_t tota
     const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
     if (vec) hipLaunchKernelGGL(k_scan_reduce<true>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums);
     else hipLaunchKernelGGL(k_scan_reduce<false>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, sums, nb);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SUMS_THREADS), 0, s, sums, nb);
     if (vec) hipLaunchKernelGGL(k_scan_down<true>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums, out, total);
     else hipLaunchKernelGGL(k_scan_down<false>, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums, out, total);
     PPMG_HIP(hipGetLastError());
@@ -144,7 +167,7 @@ int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, int64_t tota
 // 800-822 (x0 = (x - min)*dxi, INT truncation) and the sub test of :535-561;
 // -1 when the particle lies in no cell of the sub.
 template <int DIM, typename PosT>
-__device__ __forceinline__ int cell_key(const PosT* __restrict__ x, const DevMesh& m) {
+__device__ __forceinline__ int cell_key(const PosT (&x)[DIM], const DevMesh& m) {
     bool ok = true;
     int cell[3] = {0, 0, 0};
 #pragma unroll
@@ -206,9 +229,22 @@ __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool live = p < np;
-    const int k = live ? cell_key<DIM, PosT>(xp + p * DIM, m) : -1;
+    // the particle before this wavefront's first, at a wavefront-uniform
+    // address: read alongside the lanes' own positions, not after them
+    const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
+    const bool has_seam = p0 > 0 && p0 <= np;
+    // all loads first (the key arithmetic branches on mesh flags, which would
+    // otherwise keep each load behind the previous component's arithmetic)
+    PosT xv[DIM], sv[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        xv[d] = live ? xp[p * DIM + d] : PosT(0);
+        sv[d] = has_seam ? xp[(p0 - 1) * DIM + d] : PosT(0);
+    }
+    const int seam = has_seam ? cell_key<DIM, PosT>(sv, m) : -1;
+    const int k = live ? cell_key<DIM, PosT>(xv, m) : -1;
     int left = __shfl_up(k, 1);
-    if (lane == 0) left = live && p > 0 ? cell_key<DIM, PosT>(xp + (p - 1) * DIM, m) : -1;
+    if (lane == 0) left = seam;
     const bool bad = live && k < 0;
     const bool desc = live && p > 0 && k < left;
     const uint64_t bb = __ballot(bad), bd = __ballot(desc);
