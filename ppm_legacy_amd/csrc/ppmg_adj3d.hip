@@ -92,7 +92,7 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         if (plane_lane && z < z1) {
             const int key = (int)((int64_t)z * plane) + key0;
             f = start[key];
-            n = start[key + 1] - f;
+            n = start[key + 1];   // end, subtracted where used: the load stays in flight
         }
     };
     int first_n, cnt_n;
@@ -100,7 +100,10 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     __syncthreads();
 
     for (int cz = z0; cz < z1; ++cz) {
-        const int first = first_n, cnt = cnt_n;
+        const int first = first_n, cnt = cnt_n - first_n;   // cnt_n: end of the range
+        // take this plane's heads before the next loads are issued, so that
+        // waiting for them does not wait for those loads too
+        asm volatile("" ::"v"(first), "v"(cnt) : "memory");
         heads(cz + 1, first_n, cnt_n);
         const bool more = cz + 1 < z1;
         if (more) fetch_plane(cz + 3);

@@ -138,7 +138,7 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             if (czw < 0) czw += Nz;
             const int key = (int)(plane_cells * czw + keyxy[j]);
             f = start[key];
-            n = start[key + 1] - f;
+            n = start[key + 1];   // end, subtracted where used: the load stays in flight
         }
     };
     auto particle = [&](int f, int n, ValT (&fr)[3], ValT (&u)[LDA]) {
@@ -166,7 +166,10 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     for (int cz = z0 - 2; cz <= z1; ++cz) {
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
-            const int first = f1, cnt = n1;
+            const int first = f1, cnt = n1 - f1;   // n: end of the cell's range
+            // take this step's heads (loaded two steps ago) before the next load is
+            // issued, so that waiting for them does not wait for that load too
+            asm volatile("" ::"v"(first), "v"(cnt) : "memory");
             f1 = f2;
             n1 = n2;
             if (j + 2 < NC) heads(cz, j + 2, f2, n2);
