@@ -99,6 +99,18 @@ def time_form(core, torch, mesh, x, up, xfresh, prec, steps, warmup):
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1) / steps
     avg = {k: sum(v) / len(v) for k, v in phases.items()}
+    # backward of the last P2M (strength and position gradients of <g, field>),
+    # reusing its binning: reported beside the cycle, not part of it
+    g = torch.randn(field.shape, dtype=torch.float64, device="cuda").to(vdt)
+    core.p2m_adjoint(mesh, x, up, g, [0, 0, 0], p_plan, 2, prec)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(3):
+        core.p2m_adjoint(mesh, x, up, g, [0, 0, 0], p_plan, 2, prec)
+    e1.record()
+    torch.cuda.synchronize()
+    avg["adjoint"] = e0.elapsed_time(e1) / 3
+    del g
     return ms, wall, avg, field, out
 
 

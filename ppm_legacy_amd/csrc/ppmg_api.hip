@@ -183,7 +183,7 @@ int ppmg_plan_create(ppmg_plan** plan) {
 int ppmg_plan_destroy(ppmg_plan* plan) {
     if (!plan) return 0;
     DevBuf* bufs[] = {&plan->key, &plan->slot, &plan->count, &plan->start, &plan->perm, &plan->wsorted,
-                      &plan->usorted, &plan->scan_tmp, &plan->err, &plan->misc, &plan->misc2, &plan->gaps};
+                      &plan->usorted, &plan->scan_tmp, &plan->err, &plan->misc, &plan->misc2, &plan->gaps, &plan->multi, &plan->multi_n};
     for (DevBuf* b : bufs) b->release();
     if (plan->ev_init)
         for (auto& e : plan->ev) (void)hipEventDestroy(e);

@@ -87,7 +87,13 @@ __global__ void __launch_bounds__(SUMS_THREADS) k_scan_sums(int64_t* __restrict_
     const int64_t seg = (nb + SUMS_THREADS - 1) / SUMS_THREADS;
     const int64_t b = t * seg, e = b + seg < nb ? b + seg : nb;
     int64_t tot = 0;
-    for (int64_t i = b; i < e; ++i) tot += sums[i];
+    for (int64_t i0 = b; i0 < e; i0 += 8) {   // eight loads in flight at a time
+        int64_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = i0 + j < e ? sums[i0 + j] : 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) tot += v[j];
+    }
     // workgroup exclusive scan of the segment totals
     int64_t x = tot;
 #pragma unroll
@@ -107,10 +113,15 @@ __global__ void __launch_bounds__(SUMS_THREADS) k_scan_sums(int64_t* __restrict_
     }
     __syncthreads();
     int64_t run = x - tot + sh[wid];
-    for (int64_t i = b; i < e; ++i) {
-        const int64_t v = sums[i];
-        sums[i] = run;
-        run += v;
+    for (int64_t i0 = b; i0 < e; i0 += 8) {
+        int64_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = i0 + j < e ? sums[i0 + j] : 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (i0 + j < e) sums[i0 + j] = run;
+            run += v[j];
+        }
     }
 }
 
@@ -225,7 +236,8 @@ template <int DIM, typename PosT, bool SLOT>
 __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64_t np, DevMesh m, int64_t ncells,
                                               int* __restrict__ key, int* __restrict__ slot,
                                               int* __restrict__ count, int* __restrict__ start,
-                                              int* __restrict__ err, int4* __restrict__ gaps, int gapcap) {
+                                              int* __restrict__ err, int4* __restrict__ gaps, int gapcap,
+                                              int* __restrict__ multi, int* __restrict__ multi_n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool live = p < np;
@@ -252,14 +264,31 @@ __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64
         if (bb) atomicAdd(&err[0], (int)__popcll(bb));
         if (bd && *(volatile int*)&err[1] == 0) err[1] = 1;
     }
-    if (!live) return;
-    key[p] = k;
-    if (k < 0) return;
     if (SLOT) {
-        slot[p] = atomicAdd(&count[k], 1);
+        // arrival slot; the particle that arrives second in a cell lists the
+        // cell in its workgroup's block of `multi` for k_stabilize_listed
+        __shared__ int listed[256];
+        __shared__ int nlisted;
+        if (threadIdx.x == 0) nlisted = 0;
+        int sl = -1;
+        if (live) {
+            key[p] = k;
+            if (k >= 0) {
+                sl = atomicAdd(&count[k], 1);
+                slot[p] = sl;
+            }
+        }
+        __syncthreads();
+        if (sl == 1) listed[atomicAdd(&nlisted, 1)] = k;
+        __syncthreads();
+        const int n = nlisted;
+        if ((int)threadIdx.x < n) multi[(int64_t)blockIdx.x * 256 + threadIdx.x] = listed[threadIdx.x];
+        if (threadIdx.x == 0) multi_n[blockIdx.x] = n;
         return;
     }
-    if (desc) return;
+    if (!live) return;
+    key[p] = k;
+    if (k < 0 || desc) return;
     if (k != left) set_heads(start, left + 1, k, (int)p, err, gaps, gapcap);
     if (p == np - 1) set_heads(start, k + 1, (int)ncells, (int)np, err, gaps, gapcap);
 }
@@ -352,6 +381,25 @@ __global__ void __launch_bounds__(256) k_stabilize(const int* __restrict__ start
     }
 }
 
+// The same for the cells k_keys listed (those with two or more particles):
+// workgroup b sorts the cells listed by key workgroup b.
+__global__ void __launch_bounds__(256) k_stabilize_listed(const int* __restrict__ start,
+                                                          const int* __restrict__ multi,
+                                                          const int* __restrict__ multi_n, int* __restrict__ perm) {
+    if ((int)threadIdx.x >= multi_n[blockIdx.x]) return;
+    const int c = multi[(int64_t)blockIdx.x * 256 + threadIdx.x];
+    const int b = start[c], e = start[c + 1];
+    for (int i = b + 1; i < e; ++i) {
+        int v = perm[i];
+        int j = i - 1;
+        while (j >= b && perm[j] > v) {
+            perm[j + 1] = perm[j];
+            --j;
+        }
+        perm[j + 1] = v;
+    }
+}
+
 __global__ void k_to_one_based(int* __restrict__ a, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] += 1;
@@ -386,6 +434,8 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
         plan->count.ensure(sizeof(int) * (size_t)ncells) ||
         plan->start.ensure(sizeof(int) * (size_t)(ncells + 1)) ||
         plan->gaps.ensure(sizeof(int4) * (size_t)gapcap) ||
+        plan->multi.ensure(sizeof(int) * (size_t)nblk(np, 256) * 256) ||
+        plan->multi_n.ensure(sizeof(int) * (size_t)nblk(np, 256)) ||
         plan->err.ensure(sizeof(int) * 4))
         return PPMG_ERROR_FATAL;
     plan->bin_stream = s;
@@ -409,10 +459,12 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     int* start = plan->start.as<int>();
     int* err = plan->err.as<int>();
     int4* gaps = plan->gaps.as<int4>();
+    int* multi = plan->multi.as<int>();
+    int* multi_n = plan->multi_n.as<int>();
     {
         const unsigned g = nblk(np, 256);
 #define KEYS(D, T, S) hipLaunchKernelGGL((k_keys<D, T, S>), dim3(g), dim3(256), 0, s, (const T*)xp, np, m, ncells, key, \
-                                         slot, count, start, err, gaps, (int)gapcap)
+                                         slot, count, start, err, gaps, (int)gapcap, multi, multi_n)
         const bool single = prec == PPMG_PREC_F32;
         if (m.dim == 3) {
             if (single) { if (predict_sorted) KEYS(3, float, false); else KEYS(3, float, true); }
@@ -445,8 +497,14 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
         if (!sorted) {
             hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 256)), dim3(256), 0, s, key, slot, start, np,
                                plan->perm.as<int>());
-            hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, start, ncells,
-                               plan->perm.as<int>());
+            // the key pass listed the cells with two or more particles, unless a
+            // wrong prediction sent it down the cell-order path
+            if (predict_sorted)
+                hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, start, ncells,
+                                   plan->perm.as<int>());
+            else
+                hipLaunchKernelGGL(k_stabilize_listed, dim3(nblk(np, 256)), dim3(256), 0, s, start, multi, multi_n,
+                                   plan->perm.as<int>());
         }
     }
     PPMG_HIP(hipGetLastError());
