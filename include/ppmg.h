@@ -59,6 +59,10 @@ extern "C" {
 #define PPMG_PREC_F32  2   /* float xp and values: the reference's *_s* specifics
                               (sums compensated as in F32C) */
 
+/* field ghost mappings (ppm_param.h:187-188) */
+#define PPMG_MAP_GHOST_GET 5
+#define PPMG_MAP_GHOST_PUT 6
+
 /* memory space of caller arrays for the registry layer */
 #define PPMG_MEM_DEVICE 0
 #define PPMG_MEM_HOST   1
@@ -158,6 +162,19 @@ int ppmg_rmsh_create_part(const ppmg_mesh* mesh, int prec, const ppmg_field* fie
                           int64_t maxnp, int64_t* np_out, void* xp, void* up,
                           const ppmg_field* slave, void* wp, ppmg_plan* plan, void* stream);
 
+/* Ghost mapping of a padded field: replaces ppm_map_field_ghost with maptype
+ * ppm_param_map_ghost_get or ppm_param_map_ghost_put followed by push, send
+ * and pop (ppm_map_field_ghost.f:627-714; block lists of
+ * ppm_map_field_ghost_init.f:234-447 for one periodic sub).  field must hold
+ * nodes 1-ghostsize .. Nm+ghostsize of the reference's numbering (lo <= -g,
+ * lo+ext-1 >= nm-1+g, 0-based).  PPMG_MAP_GHOST_GET copies each real node onto
+ * its stored periodic images (ghost layers and the duplicate face node Nm);
+ * PPMG_MAP_GHOST_PUT adds each stored image onto its real node.  Blocks are
+ * all read before any is written, then applied in the reference's list order.
+ * Element type as for the field of ppmg_p2m (double for PPMG_PREC_F64). */
+int ppmg_map_field_ghost(const ppmg_mesh* mesh, int prec, const ppmg_field* field, const int* ghostsize,
+                         int maptype, ppmg_plan* plan, void* stream);
+
 /* Phase timing with HIP events on the call's stream (off by default; the
  * PPMG_TIMING environment variable switches it on for new plans).
  * ppmg_plan_timings waits for the last call on the plan and returns up to n
@@ -227,6 +244,11 @@ int ppmg_rmsh_create_part_reg(const void* field_up, int lda, int topo_id, int me
                               const double* cutoff, const double* cutoff_weights, int resetpos, int maxnp,
                               int* np_out, void* xp, void* up, const void* field_wp, int lda2, void* wp,
                               int prec, int mem);
+/* ppm_map_field_ghost(fv,[lda,]topo_id,mesh_id,ghostsize,maptype,info)
+ * (ppm_map_field_ghost.f:627-714) for maptype ghost_get / ghost_put, on the
+ * padded field_up (lower bound 1-ghostsize), host or device. */
+int ppmg_map_field_ghost_reg(void* field_up, int lda, int topo_id, int mesh_id, const int* ghostsize, int maptype,
+                             int prec, int mem);
 /* ppm_util_rank3d / rank2d with host or device arrays. */
 int ppmg_util_rank_reg(int dim, const void* xp, int np, const void* xmin, const void* xmax,
                        const int* ncells, const int* ngl, int* lpdx, int* lhbx, int* icount,

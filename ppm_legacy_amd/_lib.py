@@ -10,11 +10,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libppmg.so")
 
 PPMG_OK = 0
+PPMG_ERROR_FATAL = -1         # ppm_param.h:372-375
+PPMG_ERROR_ERROR = -2
 KERNEL_BSP2 = 1
 KERNEL_MP4 = 2
 PREC_F64 = 0
 PREC_F32C = 1
 PREC_F32 = 2
+MAP_GHOST_GET = 5          # ppm_param.h:187-188
+MAP_GHOST_PUT = 6
 MEM_DEVICE = 0
 MEM_HOST = 1
 
@@ -54,6 +58,7 @@ EXPORTS = [
     "ppmg_finalize", "ppmg_set_precision_mode", "ppmg_mktopo_field", "ppmg_topo_destroy",
     "ppmg_interp_p2m", "ppmg_interp_m2p", "ppmg_rmsh_comp_weights_reg", "ppmg_rmsh_remesh_reg",
     "ppmg_util_rank_reg", "ppmg_topo_info", "ppmg_impose_part_bc_reg", "ppmg_rmsh_create_part_reg", "ppmg_gen_jittered_lattice", "ppmg_gen_normals", "ppmg_gen_divfree_velocity",
+    "ppmg_map_field_ghost", "ppmg_map_field_ghost_reg",
 ]
 
 _lib = None

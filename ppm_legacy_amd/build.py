@@ -77,7 +77,8 @@ def build_fortran(verbose=True):
     out = os.path.join(BUILD, "fortran")
     os.makedirs(out, exist_ok=True)
     order = ["ppm_module_ppmg.f90", "ppm_module_core.f90", "ppm_module_interp_p2m.f90",
-             "ppm_module_interp_m2p.f90", "ppm_module_rmsh.f90", "ppm_module_util_rank.f90"]
+             "ppm_module_interp_m2p.f90", "ppm_module_rmsh.f90", "ppm_module_util_rank.f90",
+             "ppm_module_map_field_ghost.f90"]
     srcs = [os.path.join(fdir, f) for f in order]
     objs = []
     for s in srcs:

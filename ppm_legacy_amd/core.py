@@ -189,6 +189,18 @@ def p2m_adjoint(mesh, xp, up, g, lo, plan, kernel=KERNEL_MP4, prec=PREC_F64, wan
     return gu, gx
 
 
+def map_field_ghost(mesh, field, lo, ghostsize, maptype, prec=PREC_F64, plan=None):
+    """ppm_map_field_ghost ghost_get (L.MAP_GHOST_GET) or ghost_put
+    (L.MAP_GHOST_PUT) + push/send/pop on a stored box holding nodes
+    1-ghostsize .. Nm+ghostsize (lo <= -ghostsize), in place."""
+    _check_tensor(field, VAL_DTYPE[prec], "field")
+    plan = plan or Plan()
+    f = field_desc(mesh, field, lo)
+    g = (ctypes.c_int * 3)(*([int(v) for v in ghostsize] + [0] * 3)[:3])
+    check(lib().ppmg_map_field_ghost(mesh.ref(), prec, ctypes.byref(f), g, int(maptype), plan.handle, _stream()))
+    return plan
+
+
 def util_rank(xp, xmin, xmax, ncells, ngl, plan=None):
     """ppm_util_rank3d / rank2d on device: returns (lpdx[:icount] 1-based, lhbx 1-based)."""
     dim = xp.shape[1]

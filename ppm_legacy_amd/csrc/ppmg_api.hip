@@ -183,7 +183,7 @@ int ppmg_plan_create(ppmg_plan** plan) {
 int ppmg_plan_destroy(ppmg_plan* plan) {
     if (!plan) return 0;
     DevBuf* bufs[] = {&plan->key, &plan->slot, &plan->count, &plan->start, &plan->perm, &plan->wsorted,
-                      &plan->usorted, &plan->scan_tmp, &plan->err, &plan->misc, &plan->misc2, &plan->gaps, &plan->multi, &plan->multi_n};
+                      &plan->usorted, &plan->scan_tmp, &plan->err, &plan->misc, &plan->misc2, &plan->gaps, &plan->multi, &plan->multi_n, &plan->ghost};
     for (DevBuf* b : bufs) b->release();
     if (plan->ev_init)
         for (auto& e : plan->ev) (void)hipEventDestroy(e);
@@ -209,6 +209,30 @@ int ppmg_plan_timings(ppmg_plan* plan, double* ms, int n) {
         plan->last_ms[k] = t;
     }
     return k;
+}
+
+int ppmg_map_field_ghost(const ppmg_mesh* mesh, int prec, const ppmg_field* field, const int* ghostsize, int maptype,
+                         ppmg_plan* plan, void* stream) {
+    clear_error();
+    if (check_plan(plan)) return PPMG_ERROR_ERROR;
+    if (maptype != PPMG_MAP_GHOST_GET && maptype != PPMG_MAP_GHOST_PUT)
+        return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "maptype must be ghost_get or ghost_put");
+    if (!ghostsize) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "ghostsize is NULL");
+    DevMesh m;
+    if (make_mesh(mesh, prec, Spacing::MK, &m)) return g_status.info;
+    FieldGeom fg;
+    if (make_field(field, m, &fg)) return g_status.info;
+    int g[3] = {0, 0, 0};
+    for (int d = 0; d < m.dim; ++d) {
+        g[d] = ghostsize[d];
+        if (g[d] < 0) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "ghostsize must be >=0 in all dimensions");
+        // the stored box must hold nodes 1-g .. Nm+g (0-based -g .. N+g)
+        if (fg.lo[d] > -g[d] || fg.lo[d] + fg.ext[d] - 1 < m.N[d] + g[d])
+            return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "field does not hold the mesh and its ghost layers");
+    }
+    if (map_field_ghost(m, prec, fg, field->data, g, maptype == PPMG_MAP_GHOST_GET, plan->ghost, as_stream(stream)))
+        return g_status.info;
+    return 0;
 }
 
 int ppmg_plan_sort_view(const ppmg_plan* plan, const int** key, const int** perm, const int** start, int64_t* np,

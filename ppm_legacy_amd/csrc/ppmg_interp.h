@@ -33,4 +33,16 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
 int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* field,
                void* up, hipStream_t s);
 
+// Ghost mapping of a padded field (ppmg_ghost.hip).  A block is the part of
+// one periodic image of the real nodes that the stored box holds: src is its
+// first real node (1-based), size its extent, off the image shift.
+struct GhostBlock {
+    int src[3];
+    int size[3];
+    int off[3];
+};
+int ghost_blocks(const DevMesh& m, const int* g, GhostBlock* out);
+int map_field_ghost(const DevMesh& m, int prec, const FieldGeom& fg, void* field, const int* g, bool get,
+                    DevBuf& scratch, hipStream_t s);
+
 }  // namespace ppmg

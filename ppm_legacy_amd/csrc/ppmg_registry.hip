@@ -412,6 +412,34 @@ int ppmg_rmsh_remesh_reg(const void* xp, int np, const void* up, int lda, int to
     return 0;
 }
 
+int ppmg_map_field_ghost_reg(void* field_up, int lda, int topo_id, int mesh_id, const int* ghostsize, int maptype,
+                             int prec, int mem) {
+    clear_error();
+    Topo t;
+    if (lookup(topo_id, mesh_id, &t)) return current_info();
+    if (!ghostsize || !field_up) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "ghostsize or field_up is NULL");
+    if (lda < 1) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "lda must be >0");
+    int g[3] = {0, 0, 0};
+    for (int d = 0; d < t.mesh.dim; ++d) g[d] = ghostsize[d];
+    // a copy / add of stored values: the double arrays take the FP64 form whatever the precision mode
+    const bool single = prec == PPMG_PREC_F32;
+    const int form = single ? PPMG_PREC_F32 : PPMG_PREC_F64;
+    const size_t esz = single ? 4 : 8;
+    ppmg_field f = padded_field(t, lda, g, nullptr);
+    const size_t bytes = esz * (size_t)field_count(f);
+    ppmg_plan* plan = stage_plan();
+    if (!plan) return current_info();
+    void* dfield = nullptr;
+    if (stage_out_ptr(g_stage.field, field_up, bytes, mem, &dfield)) return current_info();
+    if (mem == PPMG_MEM_HOST) PPMG_HIP(hipMemcpy(dfield, field_up, bytes, hipMemcpyHostToDevice));
+    f.data = dfield;
+    int rc = ppmg_map_field_ghost(&t.mesh, form, &f, g, maptype, plan, nullptr);
+    if (rc) return rc;
+    if (mem == PPMG_MEM_HOST) PPMG_HIP(hipMemcpy(field_up, dfield, bytes, hipMemcpyDeviceToHost));
+    PPMG_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
 int ppmg_util_rank_reg(int dim, const void* xp, int np, const void* xmin, const void* xmax, const int* ncells,
                        const int* ngl, int* lpdx, int* lhbx, int* icount, int prec, int mem) {
     clear_error();

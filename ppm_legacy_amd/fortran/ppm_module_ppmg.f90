@@ -33,6 +33,14 @@ MODULE ppm_module_ppmg
   INTEGER, PARAMETER :: ppm_err_part_unass = 405
   INTEGER, PARAMETER :: ppm_err_argument   = 503
   INTEGER, PARAMETER :: ppm_err_ppm_noinit = 903
+  ! field mapping types (ppm_param.h:183-192)
+  INTEGER, PARAMETER :: ppm_param_map_pop       = 1
+  INTEGER, PARAMETER :: ppm_param_map_push      = 2
+  INTEGER, PARAMETER :: ppm_param_map_send      = 4
+  INTEGER, PARAMETER :: ppm_param_map_ghost_get = 5
+  INTEGER, PARAMETER :: ppm_param_map_ghost_put = 6
+  INTEGER, PARAMETER :: ppm_param_map_cancel    = 9
+  INTEGER, PARAMETER :: ppm_param_map_init      = 10
   ! decomposition / assignment selectors accepted (and ignored: one sub) by ppm_mktopo
   INTEGER, PARAMETER :: ppm_param_decomp_cuboid = 5
   INTEGER, PARAMETER :: ppm_param_assign_internal = 1
@@ -104,6 +112,13 @@ MODULE ppm_module_ppmg
       INTEGER(C_INT), VALUE :: lda, topo_id, mesh_id, resetpos, maxnp, lda2, prec, mem
       INTEGER(C_INT) :: ghostsize(*), np_out
       REAL(C_DOUBLE) :: cutoff(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ppmg_map_field_ghost_reg(field_up, lda, topo_id, mesh_id, ghostsize, maptype, prec, mem) &
+        BIND(C, NAME='ppmg_map_field_ghost_reg')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: field_up
+      INTEGER(C_INT), VALUE :: lda, topo_id, mesh_id, maptype, prec, mem
+      INTEGER(C_INT) :: ghostsize(*)
     END FUNCTION
     INTEGER(C_INT) FUNCTION ppmg_util_rank_reg(dim, xp, np, xmin, xmax, ncells, ngl, lpdx, lhbx, icount, prec, mem) &
         BIND(C, NAME='ppmg_util_rank_reg')
@@ -359,5 +374,25 @@ CONTAINS
                                 ploc_s(w2, np), ploc_s(w3, np), ppmg_prec_f32, ppmg_mem_host)
     CALL ppmg_report('ppm_rmsh_remesh', info)
   END SUBROUTINE remesh_core_s
+
+  SUBROUTINE ghost_core_d(field, lda, topo_id, mesh_id, cg, maptype, info)
+    REAL(C_DOUBLE), TARGET :: field(*)
+    INTEGER, INTENT(IN) :: lda, topo_id, mesh_id, maptype
+    INTEGER(C_INT), INTENT(IN) :: cg(3)
+    INTEGER, INTENT(OUT) :: info
+    info = ppmg_map_field_ghost_reg(C_LOC(field(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
+                                    cg, INT(maptype, C_INT), ppmg_prec_f64, ppmg_mem_host)
+    CALL ppmg_report('ppm_map_field_ghost', info)
+  END SUBROUTINE ghost_core_d
+
+  SUBROUTINE ghost_core_s(field, lda, topo_id, mesh_id, cg, maptype, info)
+    REAL(C_FLOAT), TARGET :: field(*)
+    INTEGER, INTENT(IN) :: lda, topo_id, mesh_id, maptype
+    INTEGER(C_INT), INTENT(IN) :: cg(3)
+    INTEGER, INTENT(OUT) :: info
+    info = ppmg_map_field_ghost_reg(C_LOC(field(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
+                                    cg, INT(maptype, C_INT), ppmg_prec_f32, ppmg_mem_host)
+    CALL ppmg_report('ppm_map_field_ghost', info)
+  END SUBROUTINE ghost_core_s
 
 END MODULE ppmg_fcore

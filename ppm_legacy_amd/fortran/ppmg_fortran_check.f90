@@ -20,6 +20,7 @@ PROGRAM ppmg_fortran_check
   USE ppm_module_rmsh
   USE ppm_module_util_rank
   USE ppm_module_impose_part_bc
+  USE ppm_module_map_field_ghost
   IMPLICIT NONE
   INTEGER, PARAMETER :: MK = ppm_kind_double
   CHARACTER(LEN=1024) :: fin, fout
@@ -29,7 +30,7 @@ PROGRAM ppmg_fortran_check
   REAL(MK), DIMENSION(3) :: minp, maxp
   REAL(MK), DIMENSION(:,:), POINTER :: xp => NULL(), up => NULL(), xq => NULL(), uq => NULL()
   REAL(MK), DIMENSION(:,:), POINTER :: xn => NULL(), un => NULL(), wn => NULL(), x2 => NULL()
-  REAL(MK), DIMENSION(:,:,:,:,:), POINTER :: field => NULL(), field2 => NULL(), fbig => NULL()
+  REAL(MK), DIMENSION(:,:,:,:,:), POINTER :: field => NULL(), field2 => NULL(), fbig => NULL(), fg => NULL()
   REAL(MK), DIMENSION(:,:,:), POINTER :: f2d => NULL()
   REAL(MK), DIMENSION(:), POINTER :: u1 => NULL()
   REAL(ppm_kind_single), DIMENSION(:,:), POINTER :: xs => NULL()
@@ -133,6 +134,33 @@ PROGRAM ppmg_fortran_check
   ! 10. error path: a particle outside the domain is the reference's fatal 405
   xp(2, 3) = 1.5_MK
   CALL ppm_interp_p2m(xp, np, up, lda, topo, mesh, kernel, gs, field, info)
+  WRITE(20) info
+
+  ! 11. ghost mapping on a copy of the remeshed field: ghost_put, then ghost_get,
+  !     each as init / map / push / send / pop; then a pop without push
+  ALLOCATE(fg(lda, 1-g:nm+g, 1-g:nm+g, 1-g:nm+g, 1))
+  fg = field2
+  CALL ppm_map_field_ghost(fg, lda, topo, mesh, gs, ppm_param_map_init, info)
+  CALL check(info, 'ppm_map_field_ghost init')
+  CALL ppm_map_field_ghost(fg, lda, topo, mesh, gs, ppm_param_map_ghost_put, info)
+  CALL check(info, 'ppm_map_field_ghost ghost_put')
+  CALL ppm_map_field_ghost(fg, lda, topo, mesh, gs, ppm_param_map_push, info)
+  CALL check(info, 'ppm_map_field_ghost push')
+  CALL ppm_map_field_ghost(fg, lda, topo, mesh, gs, ppm_param_map_send, info)
+  CALL check(info, 'ppm_map_field_ghost send')
+  CALL ppm_map_field_ghost(fg, lda, topo, mesh, gs, ppm_param_map_pop, info)
+  CALL check(info, 'ppm_map_field_ghost pop (put)')
+  WRITE(20) fg
+  CALL ppm_map_field_ghost(fg, lda, topo, mesh, gs, ppm_param_map_ghost_get, info)
+  CALL check(info, 'ppm_map_field_ghost ghost_get')
+  CALL ppm_map_field_ghost(fg, lda, topo, mesh, gs, ppm_param_map_push, info)
+  CALL check(info, 'ppm_map_field_ghost push')
+  CALL ppm_map_field_ghost(fg, lda, topo, mesh, gs, ppm_param_map_send, info)
+  CALL check(info, 'ppm_map_field_ghost send')
+  CALL ppm_map_field_ghost(fg, lda, topo, mesh, gs, ppm_param_map_pop, info)
+  CALL check(info, 'ppm_map_field_ghost pop (get)')
+  WRITE(20) fg
+  CALL ppm_map_field_ghost(fg, lda, topo, mesh, gs, ppm_param_map_pop, info)
   WRITE(20) info
 
   CLOSE(20)

@@ -10,6 +10,7 @@ convention) over libppmg's topo_id / mesh_id registry.
     info, xp, up = ppm_rmsh_create_part(field_up, topo_id, mesh_id, cutoff, ghostsize, resetpos, cutoff_weights)
     info = ppm_util_rank3d(xp, Np, xmin, xmax, Nm, Ngl, lpdx, lhbx)
     info = ppm_impose_part_bc(xp, Npart, topo_id)
+    info = ppm_map_field_ghost(field_up, topo_id, mesh_id, ghostsize, maptype, lda=None)
 
 Arrays are numpy arrays (host: staged to the GPU by the library) or torch
 tensors (CUDA tensors stay on the device), with the Fortran layouts in C
@@ -22,7 +23,8 @@ The double specifics run in the form chosen by ppm_set_precision_mode
 specifics.  Reference: ppm_interp_p2m.f:130-160, ppm_interp_m2p.f:129-159,
 ppm_rmsh_remesh.f:91-121, ppm_rmsh_comp_weights.f:97-103,
 ppm_rmsh_create_part.f:90-171, ppm_util_rank3d.f:147-151,
-ppm_impose_part_bc.f:67-214, ppm_init.f:161, ppm_topo_mkfield.f:316-319.
+ppm_impose_part_bc.f:67-214, ppm_init.f:161, ppm_topo_mkfield.f:316-319,
+ppm_map_field_ghost.f:627-714.
 """
 import ctypes
 
@@ -39,6 +41,14 @@ ppm_param_bcdef_periodic = 1
 ppm_param_bcdef_freespace = 2
 ppm_kind_double = PREC_F64
 ppm_kind_single = PREC_F32
+# field mapping types (ppm_param.h:183-192)
+ppm_param_map_pop = 1
+ppm_param_map_push = 2
+ppm_param_map_send = 4
+ppm_param_map_ghost_get = L.MAP_GHOST_GET
+ppm_param_map_ghost_put = L.MAP_GHOST_PUT
+ppm_param_map_cancel = 9
+ppm_param_map_init = 10
 
 _topos = {}
 
@@ -193,3 +203,59 @@ def ppm_rmsh_create_part(field_up, topo_id, mesh_id, cutoff, ghostsize, resetpos
     except L.PpmgError as e:
         return (e.info, None, None) if w is None else (e.info, None, None, None)
     return (0,) + tuple(res)
+
+
+# pending ghost mapping of ppm_map_field_ghost: set by ghost_get / ghost_put,
+# carried out by pop (push and send move no data on one process)
+_ghost_map = {"type": None, "pushed": False}
+
+
+def ppm_map_field_ghost(field_up, topo_id, mesh_id, ghostsize, maptype, lda=None):
+    """ppm_map_field_ghost(fv,[lda,]topo_id,mesh_id,ghostsize,maptype,info)
+    (ppm_map_field_ghost.f:627-714) for one sub: init, ghost_get or ghost_put,
+    push, send, pop in the reference's order.  The mapping is applied to
+    field_up at pop (ghost_get: periodic images := real nodes; ghost_put:
+    images added onto their real nodes)."""
+    if maptype == ppm_param_map_init or maptype == ppm_param_map_cancel:
+        _ghost_map.update(type=None, pushed=False)
+        return 0
+    if maptype in (ppm_param_map_ghost_get, ppm_param_map_ghost_put):
+        _ghost_map.update(type=maptype, pushed=False)
+        return 0
+    if maptype == ppm_param_map_push:
+        if _ghost_map["type"] is None:
+            return _fail_map("no ghost mapping (ghost_get or ghost_put) was requested before push")
+        _ghost_map["pushed"] = True
+        return 0
+    if maptype == ppm_param_map_send:
+        return 0 if _ghost_map["pushed"] else _fail_map("send without push")
+    if maptype == ppm_param_map_pop:
+        if not _ghost_map["pushed"]:
+            return _fail_map("pop without push")
+        mem, pf, dt = _mem_ptr(field_up)
+        info, dim = _topo_dim(topo_id)
+        if info:
+            return info
+        if lda is None:
+            lda = field_up.shape[-1] if field_up.ndim == dim + 1 else 1
+        info = lib().ppmg_map_field_ghost_reg(pf, int(lda), int(topo_id), int(mesh_id), _gs(ghostsize, dim),
+                                              int(_ghost_map["type"]), _prec_of(dt), mem)
+        _ghost_map.update(pushed=False)
+        return info
+    return _fail_map("unknown maptype")
+
+
+def _fail_map(msg):
+    """A call out of the init / ghost_get|put / push / send / pop sequence:
+    the reference's ppm_err_argument with severity ppm_error_error."""
+    return L.PPMG_ERROR_ERROR
+
+
+def _topo_dim(topo_id):
+    d = ctypes.c_int()
+    nm = (ctypes.c_int * 3)()
+    mn = (ctypes.c_double * 3)()
+    mx = (ctypes.c_double * 3)()
+    bc = (ctypes.c_int * 6)()
+    info = lib().ppmg_topo_info(int(topo_id), ctypes.byref(d), nm, mn, mx, bc)
+    return info, d.value

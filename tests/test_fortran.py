@@ -139,4 +139,13 @@ def test_fortran_dropin_against_oracle(tmp_path, oracle, kernel):
     assert _rel(f2d, r2[..., 0]) < 1e-12
     # 10. particle outside the domain: the reference's fatal error
     assert int(rd.take(np.int32, 1)[0]) == -1
+    # 11. ppm_map_field_ghost: ghost_put then ghost_get on the remeshed field,
+    #     against the oracle's list-order restatements; pop without push is an error
+    fput = rd.take(np.float64, lda * n ** 3).reshape(n, n, n, lda)
+    info, ref = oracle.ghost_put(t, field2, [g] * 3)
+    assert info == 0 and np.array_equal(fput, ref)
+    fget = rd.take(np.float64, lda * n ** 3).reshape(n, n, n, lda)
+    info, ref = oracle.ghost_get(t, ref, [g] * 3)
+    assert info == 0 and np.array_equal(fget, ref)
+    assert int(rd.take(np.int32, 1)[0]) == -2
     assert rd.done()
