@@ -41,6 +41,7 @@ extern "C" {
 /* error codes (ppm_error.h:161-299), plus device-side codes of this library */
 #define PPMG_ERR_ALLOC        101
 #define PPMG_ERR_PART_RANGE   402
+#define PPMG_ERR_PART_LOST    403
 #define PPMG_ERR_PART_UNASS   405
 #define PPMG_ERR_INDEX_CORR   410
 #define PPMG_ERR_ARGUMENT     503
@@ -136,6 +137,15 @@ int ppmg_p2m_adjoint(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, co
 int ppmg_util_rank(int dim, int prec, int64_t np, const void* xp, const void* xmin,
                    const void* xmax, const int* ncells, const int* ngl, int* lpdx, int* lhbx,
                    int64_t* icount, ppmg_plan* plan, void* stream);
+
+/* Cell sort: replaces ppm_util_sort3d_{s,d} / ppm_util_sort2d_{s,d}
+ * (ppm_util_sort3d.f:88-260): ranks xp into ncells boxes with no ghost layers,
+ * then reorders xp in place so that the particles of a box are consecutive,
+ * in the stable order of ppm_util_rank3d; npbx (prod(ncells) entries, may be
+ * NULL) receives the particles per box.  A particle outside the boxes is the
+ * reference's error PPMG_ERR_PART_LOST. */
+int ppmg_util_sort(int dim, int prec, int64_t np, void* xp, const void* xmin, const void* xmax,
+                   const int* ncells, int* npbx, ppmg_plan* plan, void* stream);
 
 /* ppm_impose_part_bc (ppm_impose_part_bc.f:67-214): periodic wrap in place. */
 int ppmg_impose_part_bc(const ppmg_mesh* mesh, int prec, int64_t np, void* xp, void* stream);
@@ -249,6 +259,9 @@ int ppmg_rmsh_create_part_reg(const void* field_up, int lda, int topo_id, int me
  * padded field_up (lower bound 1-ghostsize), host or device. */
 int ppmg_map_field_ghost_reg(void* field_up, int lda, int topo_id, int mesh_id, const int* ghostsize, int maptype,
                              int prec, int mem);
+/* ppm_util_sort3d / sort2d(xp,Np,xmin,xmax,Nm,npbx,info) with host or device arrays. */
+int ppmg_util_sort_reg(int dim, void* xp, int np, const void* xmin, const void* xmax, const int* ncells, int* npbx,
+                       int prec, int mem);
 /* ppm_util_rank3d / rank2d with host or device arrays. */
 int ppmg_util_rank_reg(int dim, const void* xp, int np, const void* xmin, const void* xmax,
                        const int* ncells, const int* ngl, int* lpdx, int* lhbx, int* icount,

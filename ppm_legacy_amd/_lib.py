@@ -24,6 +24,7 @@ MEM_HOST = 1
 
 ERR_ALLOC = 101
 ERR_PART_RANGE = 402
+ERR_PART_LOST = 403
 ERR_PART_UNASS = 405
 ERR_INDEX_CORR = 410
 ERR_ARGUMENT = 503
@@ -58,7 +59,7 @@ EXPORTS = [
     "ppmg_finalize", "ppmg_set_precision_mode", "ppmg_mktopo_field", "ppmg_topo_destroy",
     "ppmg_interp_p2m", "ppmg_interp_m2p", "ppmg_rmsh_comp_weights_reg", "ppmg_rmsh_remesh_reg",
     "ppmg_util_rank_reg", "ppmg_topo_info", "ppmg_impose_part_bc_reg", "ppmg_rmsh_create_part_reg", "ppmg_gen_jittered_lattice", "ppmg_gen_normals", "ppmg_gen_divfree_velocity",
-    "ppmg_map_field_ghost", "ppmg_map_field_ghost_reg",
+    "ppmg_map_field_ghost", "ppmg_map_field_ghost_reg", "ppmg_util_sort", "ppmg_util_sort_reg",
 ]
 
 _lib = None

@@ -78,7 +78,7 @@ def build_fortran(verbose=True):
     os.makedirs(out, exist_ok=True)
     order = ["ppm_module_ppmg.f90", "ppm_module_core.f90", "ppm_module_interp_p2m.f90",
              "ppm_module_interp_m2p.f90", "ppm_module_rmsh.f90", "ppm_module_util_rank.f90",
-             "ppm_module_map_field_ghost.f90"]
+             "ppm_module_map_field_ghost.f90", "ppm_module_util_sort.f90", "ppm_module_neighlist.f90"]
     srcs = [os.path.join(fdir, f) for f in order]
     objs = []
     for s in srcs:

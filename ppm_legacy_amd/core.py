@@ -224,6 +224,26 @@ def util_rank(xp, xmin, xmax, ncells, ngl, plan=None):
     return lpdx[: ic.value], lhbx
 
 
+def util_sort(xp, xmin, xmax, ncells, plan=None):
+    """ppm_util_sort3d / sort2d on device: xp reordered in place into cell
+    order (no ghost layers); returns npbx, the particles per box."""
+    dim = xp.shape[1]
+    prec = PREC_F32 if xp.dtype == torch.float32 else PREC_F64
+    _check_tensor(xp, POS_DTYPE[prec], "xp", dim)
+    nbox = 1
+    for d in range(dim):
+        nbox *= ncells[d]
+    npbx = torch.empty(max(nbox, 1), dtype=torch.int32, device=xp.device)
+    ctype = ctypes.c_float if prec == PREC_F32 else ctypes.c_double
+    mn = (ctype * 3)(*list(xmin))
+    mx = (ctype * 3)(*list(xmax))
+    nc = (ctypes.c_int * 3)(*list(ncells))
+    plan = plan or Plan()
+    check(lib().ppmg_util_sort(dim, prec, ctypes.c_int64(xp.shape[0]), _ptr(xp), mn, mx, nc, _ptr(npbx), plan.handle,
+                               _stream()))
+    return npbx[:nbox]
+
+
 def impose_part_bc(mesh, xp):
     prec = PREC_F32 if xp.dtype == torch.float32 else PREC_F64
     _check_tensor(xp, POS_DTYPE[prec], "xp", mesh.dim)

@@ -411,6 +411,29 @@ int ppmg_util_rank(int dim, int prec, int64_t np, const void* xp, const void* xm
     return 0;
 }
 
+int ppmg_util_sort(int dim, int prec, int64_t np, void* xp, const void* xmin, const void* xmax, const int* ncells,
+                   int* npbx, ppmg_plan* plan, void* stream) {
+    clear_error();
+    if (check_plan(plan)) return PPMG_ERROR_ERROR;
+    if (dim != 2 && dim != 3) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "dim must be 2 or 3");
+    if (!ncells) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "Nm is NULL");
+    int64_t nbox = 1;
+    for (int d = 0; d < dim; ++d) nbox *= ncells[d] > 0 ? ncells[d] : 1;
+    const int ngl[6] = {0, 0, 0, 0, 0, 0};
+    if (plan->misc.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
+        plan->misc2.ensure(sizeof(int) * (size_t)(nbox + 1)))
+        return g_status.info;
+    int* lp = plan->misc.as<int>();
+    int* lh = plan->misc2.as<int>();
+    int64_t icount = 0;
+    int rc = ppmg_util_rank(dim, prec, np, xp, xmin, xmax, ncells, ngl, lp, lh, &icount, plan, stream);
+    if (rc) return rc;
+    // ppm_util_sort3d.f:226-232: every particle must lie in a box
+    if (icount < np) return fail(PPMG_ERROR_ERROR, PPMG_ERR_PART_LOST, "Not all particles have been ranked");
+    if (sort_rows(dim, prec, np, xp, lp, lh, nbox, npbx, plan->usorted, as_stream(stream))) return g_status.info;
+    return 0;
+}
+
 int ppmg_impose_part_bc(const ppmg_mesh* mesh, int prec, int64_t np, void* xp, void* stream) {
     clear_error();
     DevMesh m;

@@ -440,6 +440,37 @@ int ppmg_map_field_ghost_reg(void* field_up, int lda, int topo_id, int mesh_id, 
     return 0;
 }
 
+int ppmg_util_sort_reg(int dim, void* xp, int np, const void* xmin, const void* xmax, const int* ncells, int* npbx,
+                       int prec, int mem) {
+    clear_error();
+    if (!ncells || (np > 0 && !xp)) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "xp or Nm is NULL");
+    if (dim != 2 && dim != 3) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "dim must be 2 or 3");
+    const bool single = prec == PPMG_PREC_F32;
+    const int form = single ? PPMG_PREC_F32 : PPMG_PREC_F64;
+    const size_t esz = single ? 4 : 8;
+    int64_t nbox = 1;
+    for (int d = 0; d < dim; ++d) nbox *= ncells[d] > 0 ? ncells[d] : 1;
+    ppmg_plan* plan = stage_plan();
+    if (!plan) return current_info();
+    void* dxp = nullptr;
+    int* dnp = nullptr;
+    if (stage_out_ptr(g_stage.xp, xp, esz * (size_t)(np > 0 ? np : 0) * dim, mem, &dxp)) return current_info();
+    if (mem == PPMG_MEM_HOST && np > 0) PPMG_HIP(hipMemcpy(dxp, xp, esz * (size_t)np * dim, hipMemcpyHostToDevice));
+    if (npbx) {
+        void* p = nullptr;
+        if (stage_out_ptr(g_stage.lh, npbx, sizeof(int) * (size_t)nbox, mem, &p)) return current_info();
+        dnp = (int*)p;
+    }
+    int rc = ppmg_util_sort(dim, form, np, dxp, xmin, xmax, ncells, dnp, plan, nullptr);
+    if (rc) return rc;
+    if (mem == PPMG_MEM_HOST) {
+        if (np > 0) PPMG_HIP(hipMemcpy(xp, dxp, esz * (size_t)np * dim, hipMemcpyDeviceToHost));
+        if (npbx) PPMG_HIP(hipMemcpy(npbx, dnp, sizeof(int) * (size_t)nbox, hipMemcpyDeviceToHost));
+    }
+    PPMG_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
 int ppmg_util_rank_reg(int dim, const void* xp, int np, const void* xmin, const void* xmax, const int* ncells,
                        const int* ngl, int* lpdx, int* lhbx, int* icount, int prec, int mem) {
     clear_error();

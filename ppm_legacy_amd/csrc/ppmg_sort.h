@@ -27,5 +27,9 @@ inline const int* tile_perm(const ppmg_plan* plan) { return plan->perm_identity 
 int materialize_perm(ppmg_plan* plan, hipStream_t s);
 int rank_particles(ppmg_plan* plan, const RankGeom& g, int prec, int64_t np, const void* xp,
                    int* lpdx, int* lhbx, int64_t* icount, hipStream_t s);
+// xp (np rows of dim) reordered by the 1-based ranking lpdx1; npbx (may be
+// NULL) = particles per box from the 1-based heads lhbx1.
+int sort_rows(int dim, int prec, int64_t np, void* xp, const int* lpdx1, const int* lhbx1, int64_t nbox, int* npbx,
+              DevBuf& tmp, hipStream_t s);
 
 }  // namespace ppmg

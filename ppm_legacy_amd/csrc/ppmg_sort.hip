@@ -524,6 +524,41 @@ int materialize_perm(ppmg_plan* plan, hipStream_t s) {
     return 0;
 }
 
+// ppm_util_sort3d / sort2d (ppm_util_sort3d.f:237-260): particle rows in the
+// ranking's order (1-based lpdx), and the particle count of every box.
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather_rows(const T* __restrict__ src, const int* __restrict__ lpdx1,
+                                                     int64_t n, int dim, T* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p = lpdx1[i] - 1;
+    for (int d = 0; d < dim; ++d) dst[i * dim + d] = src[p * dim + d];
+}
+
+__global__ void __launch_bounds__(256) k_box_counts(const int* __restrict__ lhbx1, int64_t nbox,
+                                                    int* __restrict__ npbx) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nbox) npbx[c] = lhbx1[c + 1] - lhbx1[c];
+}
+
+int sort_rows(int dim, int prec, int64_t np, void* xp, const int* lpdx1, const int* lhbx1, int64_t nbox, int* npbx,
+              DevBuf& tmp, hipStream_t s) {
+    const size_t esz = prec == PPMG_PREC_F32 ? 4 : 8;
+    if (np > 0) {
+        if (tmp.ensure(esz * (size_t)np * dim)) return PPMG_ERROR_FATAL;
+        if (esz == 4)
+            hipLaunchKernelGGL(k_gather_rows<float>, dim3(nblk(np, 256)), dim3(256), 0, s, (const float*)xp, lpdx1, np,
+                               dim, tmp.as<float>());
+        else
+            hipLaunchKernelGGL(k_gather_rows<double>, dim3(nblk(np, 256)), dim3(256), 0, s, (const double*)xp, lpdx1,
+                               np, dim, tmp.as<double>());
+        PPMG_HIP(hipMemcpyAsync(xp, tmp.p, esz * (size_t)np * dim, hipMemcpyDeviceToDevice, s));
+    }
+    if (npbx) hipLaunchKernelGGL(k_box_counts, dim3(nblk(nbox, 256)), dim3(256), 0, s, lhbx1, nbox, npbx);
+    PPMG_HIP(hipGetLastError());
+    return 0;
+}
+
 int rank_particles(ppmg_plan* plan, const RankGeom& g, int prec, int64_t np, const void* xp,
                    int* lpdx, int* lhbx, int64_t* icount, hipStream_t s) {
     int64_t ncells = (int64_t)g.ntot[0] * g.ntot[1] * g.ntot[2];

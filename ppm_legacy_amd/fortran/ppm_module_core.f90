@@ -12,7 +12,7 @@ MODULE ppm_module_data
        ppm_param_bcdef_freespace, ppm_param_bcdef_symmetry, ppm_param_bcdef_neumann, &
        ppm_param_bcdef_dirichlet, ppm_param_bcdef_robin, ppm_error_notice, ppm_error_warning, &
        ppm_error_error, ppm_error_fatal, ppm_err_alloc, ppm_err_part_unass, ppm_err_argument, &
-       ppm_err_ppm_noinit, ppm_param_decomp_cuboid, ppm_param_assign_internal, ppm_debug, ppm_dim, &
+       ppm_err_ppm_noinit, ppm_err_part_lost, ppm_param_decomp_cuboid, ppm_param_assign_internal, ppm_debug, ppm_dim, &
        ppm_param_map_pop, ppm_param_map_push, ppm_param_map_send, ppm_param_map_ghost_get, &
        ppm_param_map_ghost_put, ppm_param_map_cancel, ppm_param_map_init
   IMPLICIT NONE
@@ -91,6 +91,7 @@ CONTAINS
     IF (info /= 0) RETURN
     topo_id = tid
     mesh_id = mid
+    ppmg_current_topo = tid
     nsubs = 1
     nsublist = 1
     ALLOCATE(isublist(1), sub2proc(1), istart(dim, 1), ndata(dim, 1))

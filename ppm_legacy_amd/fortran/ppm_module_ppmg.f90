@@ -30,6 +30,7 @@ MODULE ppm_module_ppmg
   INTEGER, PARAMETER :: ppm_error_fatal   = -1
   ! error codes (ppm_error.h:161-299)
   INTEGER, PARAMETER :: ppm_err_alloc      = 101
+  INTEGER, PARAMETER :: ppm_err_part_lost  = 403
   INTEGER, PARAMETER :: ppm_err_part_unass = 405
   INTEGER, PARAMETER :: ppm_err_argument   = 503
   INTEGER, PARAMETER :: ppm_err_ppm_noinit = 903
@@ -51,6 +52,9 @@ MODULE ppm_module_ppmg
 
   INTEGER :: ppm_debug = 0
   INTEGER :: ppm_dim = 0
+  ! the topology the last ppm_mktopo made: the reference's current topology
+  ! (ppm_topoid), which ppm_neighlist_clist works on
+  INTEGER :: ppmg_current_topo = 0
 
   INTERFACE
     INTEGER(C_INT) FUNCTION ppmg_init(dim, prec, tolexp, debug) BIND(C, NAME='ppmg_init')
@@ -119,6 +123,13 @@ MODULE ppm_module_ppmg
       TYPE(C_PTR), VALUE :: field_up
       INTEGER(C_INT), VALUE :: lda, topo_id, mesh_id, maptype, prec, mem
       INTEGER(C_INT) :: ghostsize(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ppmg_util_sort_reg(dim, xp, np, xmin, xmax, ncells, npbx, prec, mem) &
+        BIND(C, NAME='ppmg_util_sort_reg')
+      IMPORT :: C_INT, C_PTR
+      INTEGER(C_INT), VALUE :: dim, np, prec, mem
+      TYPE(C_PTR), VALUE :: xp, xmin, xmax, npbx
+      INTEGER(C_INT) :: ncells(*)
     END FUNCTION
     INTEGER(C_INT) FUNCTION ppmg_util_rank_reg(dim, xp, np, xmin, xmax, ncells, ngl, lpdx, lhbx, icount, prec, mem) &
         BIND(C, NAME='ppmg_util_rank_reg')

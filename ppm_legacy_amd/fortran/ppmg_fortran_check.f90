@@ -21,6 +21,8 @@ PROGRAM ppmg_fortran_check
   USE ppm_module_util_rank
   USE ppm_module_impose_part_bc
   USE ppm_module_map_field_ghost
+  USE ppm_module_util_sort
+  USE ppm_module_neighlist
   IMPLICIT NONE
   INTEGER, PARAMETER :: MK = ppm_kind_double
   CHARACTER(LEN=1024) :: fin, fout
@@ -41,6 +43,10 @@ PROGRAM ppmg_fortran_check
   INTEGER, DIMENSION(:,:), POINTER :: istart => NULL(), ndata => NULL()
   INTEGER, DIMENSION(:), POINTER :: sub2proc => NULL(), isublist => NULL(), lpdx => NULL(), lhbx => NULL()
   REAL(MK), DIMENSION(2) :: cutoff
+  INTEGER, DIMENSION(:), POINTER :: npbx => NULL()
+  INTEGER, DIMENSION(:,:), POINTER :: nmc => NULL()
+  TYPE(ppm_type_ptr_to_clist), DIMENSION(:), POINTER :: clist => NULL()
+  REAL(MK), DIMENSION(3) :: rc
 
   CALL GET_COMMAND_ARGUMENT(1, fin)
   CALL GET_COMMAND_ARGUMENT(2, fout)
@@ -162,6 +168,28 @@ PROGRAM ppmg_fortran_check
   WRITE(20) fg
   CALL ppm_map_field_ghost(fg, lda, topo, mesh, gs, ppm_param_map_pop, info)
   WRITE(20) info
+
+  ! 12. util_sort3d on the (wrapped) shifted positions x2: 5 x 4 x 3 boxes
+  nmv = (/ 5, 4, 3 /)
+  CALL ppm_util_sort3d(x2, np, minp, maxp, nmv, npbx, info)
+  CALL check(info, 'ppm_util_sort3d')
+  WRITE(20) x2
+  WRITE(20) npbx
+
+  ! 13. cell list of the P2M particles (3D topology made last is the 2D one:
+  !     make the 3D one current again), cutoff 0.3 / 0.25 / 0.2, no symmetry
+  CALL ppm_mktopo(xq, np, (/ nm, nm, nm /), decomp, ppm_param_assign_internal, minp, maxp, bc, gs, topo2, mesh2, &
+                  min_sub, max_sub, cost, sub2proc, nsubs, isublist, nsublist, istart, ndata, info)
+  CALL check(info, 'ppm_mktopo (current topology for the cell list)')
+  rc = (/ 0.3_MK, 0.25_MK, 0.2_MK /)
+  CALL ppm_neighlist_clist(xq, np, rc, .FALSE., clist, nmc, info)
+  CALL check(info, 'ppm_neighlist_clist')
+  WRITE(20) SHAPE(nmc), nmc
+  WRITE(20) SIZE(clist(1)%lpdx), SIZE(clist(1)%lhbx)
+  WRITE(20) clist(1)%lpdx, clist(1)%lhbx
+  CALL ppm_clist_destroy(clist, info)
+  CALL check(info, 'ppm_clist_destroy')
+  WRITE(20) ASSOCIATED(clist)
 
   CLOSE(20)
   CALL ppm_finalize(info)

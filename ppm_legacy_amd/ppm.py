@@ -9,6 +9,7 @@ convention) over libppmg's topo_id / mesh_id registry.
     info = ppm_rmsh_remesh(xp, Np, up, topo_id, mesh_id, kernel, ghostsize, field_up, wx1, wx2, wx3, lda=None)
     info, xp, up = ppm_rmsh_create_part(field_up, topo_id, mesh_id, cutoff, ghostsize, resetpos, cutoff_weights)
     info = ppm_util_rank3d(xp, Np, xmin, xmax, Nm, Ngl, lpdx, lhbx)
+    info = ppm_util_sort3d(xp, Np, xmin, xmax, Nm, npbx)
     info = ppm_impose_part_bc(xp, Npart, topo_id)
     info = ppm_map_field_ghost(field_up, topo_id, mesh_id, ghostsize, maptype, lda=None)
 
@@ -159,6 +160,22 @@ def ppm_util_rank3d(xp, Np, xmin, xmax, Nm, Ngl, lpdx, lhbx):
 
 
 ppm_util_rank2d = ppm_util_rank3d
+
+
+def ppm_util_sort3d(xp, Np, xmin, xmax, Nm, npbx):
+    """ppm_util_sort3d / sort2d (ppm_util_sort3d.f:88-91): xp reordered in place
+    into cell order, npbx (prod(Nm) entries) the particles per box; info."""
+    mem, pxp, dt = _mem_ptr(xp)
+    _, pnp, _ = _mem_ptr(npbx)
+    dim = xp.shape[1]
+    ctype = ctypes.c_float if _prec_of(dt) == PREC_F32 else ctypes.c_double
+    mn = (ctype * 3)(*list(xmin))
+    mx = (ctype * 3)(*list(xmax))
+    nc = (ctypes.c_int * 3)(*list(Nm))
+    return lib().ppmg_util_sort_reg(dim, pxp, int(Np), mn, mx, nc, pnp, _prec_of(dt), mem)
+
+
+ppm_util_sort2d = ppm_util_sort3d
 
 
 def _device_tensor(a):

@@ -148,4 +148,23 @@ def test_fortran_dropin_against_oracle(tmp_path, oracle, kernel):
     info, ref = oracle.ghost_get(t, ref, [g] * 3)
     assert info == 0 and np.array_equal(fget, ref)
     assert int(rd.take(np.int32, 1)[0]) == -2
+    # 12. util_sort3d of x2 into 5 x 4 x 3 boxes: rows in the stable rank order
+    xs = rd.take(np.float64, 3 * np_).reshape(np_, 3)
+    npbx = rd.take(np.int32, 60)
+    lref, href, ic, _, _ = oracle.rank(x2, [0.0] * 3, [1.0] * 3, [5, 4, 3], [0] * 6)
+    assert ic == np_
+    assert np.array_equal(xs, x2[lref - 1]) and np.array_equal(npbx, np.diff(href))
+    # 13. cell list of xq with cutoff (0.3, 0.25, 0.2): INT(1/cutoff) cells plus a
+    #     ghost layer on both sides, ranked as ppm_util_rank3d
+    shp = rd.take(np.int32, 2)
+    nmc = rd.take(np.int32, 3)
+    ncell = [int(1.0 / 0.3), int(1.0 / 0.25), int(1.0 / 0.2)]
+    assert list(shp) == [3, 1] and list(nmc) == [n + 2 for n in ncell]
+    nl = int(rd.take(np.int32, 1)[0])
+    nh = int(rd.take(np.int32, 1)[0])
+    lpdx = rd.take(np.int32, nl)
+    lhbx = rd.take(np.int32, nh)
+    lref, href, ic, _, _ = oracle.rank(xq, [0.0] * 3, [1.0] * 3, ncell, [1] * 6)
+    assert nl == ic and np.array_equal(lpdx, lref) and np.array_equal(lhbx, href)
+    assert int(rd.take(np.int32, 1)[0]) == 0            # clist deallocated
     assert rd.done()

@@ -678,3 +678,38 @@ def test_map_field_ghost_rejects_bad_arguments(core):
     with pytest.raises(L.PpmgError) as e:
         core.map_field_ghost(mesh, f, [-1] * 3, [1] * 3, 7)
     assert e.value.code == L.ERR_ARGUMENT
+
+
+# --------------------------------------------------------------- util_sort --
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_util_sort_vs_rank(core, oracle, dim, dtype):
+    """ppm_util_sort3d / sort2d: the rows of xp in the stable order of
+    ppm_util_rank with no ghost layers (the oracle's ranking), npbx the box
+    counts; both through the core API and the reference-named host mirror."""
+    from ppm_legacy_amd import ppm
+    rng = np.random.default_rng(dim)
+    ncells = [7, 5, 4][:dim]
+    xmin, xmax = [-1.0] * dim, [2.0] * dim
+    xp = rng.uniform(-1.0, 2.0, size=(5000, dim)).astype(np.float32 if dtype == torch.float32 else np.float64)
+    odt = np.float32 if dtype == torch.float32 else np.float64
+    lref, href, ic, _, _ = oracle.rank(xp, xmin, xmax, ncells, [0] * (2 * dim), dtype=odt)
+    assert ic == len(xp)
+    x = _dev(xp, dtype)
+    npbx = core.util_sort(x, xmin, xmax, ncells)
+    assert np.array_equal(x.cpu().numpy(), xp[lref - 1])
+    assert np.array_equal(npbx.cpu().numpy(), np.diff(href))
+    # host arrays through the registry
+    xh = xp.copy()
+    nh = np.zeros(int(np.prod(ncells)), np.int32)
+    assert ppm.ppm_util_sort3d(xh, len(xh), xmin, xmax, ncells, nh) == 0
+    assert np.array_equal(xh, xp[lref - 1]) and np.array_equal(nh, np.diff(href))
+
+
+def test_util_sort_particle_outside_is_part_lost(core):
+    from ppm_legacy_amd import _lib as L
+    xp = np.random.default_rng(0).uniform(0, 1, size=(100, 3))
+    xp[17] = [0.5, 1.5, 0.5]
+    with pytest.raises(L.PpmgError) as e:
+        core.util_sort(_dev(xp, torch.float64), [0.0] * 3, [1.0] * 3, [4, 4, 4])
+    assert e.value.info == -2 and e.value.code == L.ERR_PART_LOST
