@@ -304,6 +304,7 @@ int ppmg_m2p(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     if (np < 0) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "particles must be specified");
     if (lda < 1) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "lda must be >= 1");
     if (np > 0 && (!xp || !up)) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "xp or up is NULL");
+    if (np >= ((int64_t)1 << 31)) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "more than 2^31-1 particles");
     if (np == 0) return 0;
     DevMesh m;
     if (make_mesh(mesh, prec, Spacing::Single, &m)) return g_status.info;
@@ -383,6 +384,7 @@ int ppmg_util_rank(int dim, int prec, int64_t np, const void* xp, const void* xm
     if (check_plan(plan)) return PPMG_ERROR_ERROR;
     if (dim != 2 && dim != 3) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "dim must be 2 or 3");
     if (np < 0) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "Np must be >= 0");
+    if (np >= ((int64_t)1 << 31)) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "more than 2^31-1 particles");
     RankGeom g;
     std::memset(&g, 0, sizeof(g));
     g.dim = dim;
@@ -463,6 +465,7 @@ int ppmg_rmsh_remesh(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, co
     if (kernel != PPMG_KERNEL_MP4 && kernel != PPMG_KERNEL_BSP2)
         return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "wrong kernel definition");
     if (np < 0 || lda < 1) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "Np must be >= 0 and lda >= 1");
+    if (np >= ((int64_t)1 << 31)) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "more than 2^31-1 particles");
     DevMesh m;
     if (make_mesh(mesh, prec, Spacing::MK, &m)) return g_status.info;
     if (np > 0 && (!wx1 || !wx2 || (m.dim == 3 && !wx3)))
