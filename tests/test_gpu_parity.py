@@ -85,10 +85,10 @@ def test_p2m_config1_64cubed_fp64(core, oracle):
 
 
 def test_binning_sorted_and_unsorted_input(core, oracle):
-    """The binning has two paths: input already in cell order (identity
-    permutation, counts from runs of equal keys) and any other order (atomic
-    slots + per-cell stabilisation); a plan predicts the path from its last
-    input.  Every (predicted, actual) pair -- one plan fed sorted, sorted,
+    """The binning has two paths: input already in cell order (cell heads
+    written directly, identity permutation) and any other order (atomic slots,
+    scan, scatter, stabilisation of the listed cells); a plan predicts the path
+    from its last input.  Every (predicted, actual) pair -- one plan fed sorted, sorted,
     shuffled, shuffled, sorted input -- must give numpy's stable argsort, the
     exact cell heads and the oracle's field, on clustered particles with empty
     and crowded cells."""
