@@ -172,6 +172,61 @@ def test_ghost_get_fills_periodic_images(oracle):
     assert np.array_equal(out, expect)
 
 
+def _ghost_source(j, nm):
+    """Stored node j (1-based, 1-g..Nm+g) on a periodic axis after ghost_get
+    reads real node: ghost layers their periodic image, node 1 and node Nm
+    each other (the pack-then-unpack exchange of ppm_map_field_ghost), the
+    rest themselves."""
+    op = nm - 1
+    if j <= 1:
+        return j + op
+    if j >= nm:
+        return j - op
+    return j
+
+
+@pytest.mark.parametrize("dim,bc,g", [(3, [1] * 6, 2), (3, [1, 1, 2, 2, 1, 1], 1), (2, [1] * 4, 3), (2, [2, 2, 1, 1], 2)])
+def test_ghost_get_put_closed_forms(oracle, dim, bc, g):
+    """ghost_get and ghost_put on arbitrary (integer-valued, so every sum is
+    exact) data against per-axis closed forms: after get each stored node
+    holds the node _ghost_source maps it to on the periodic axes (from the
+    data before the call), unless it lies outside the real range of a
+    non-periodic axis (no image block reaches it: unchanged); after put each
+    real node holds the sum of the stored nodes
+    congruent to it modulo Nm-1 on its periodic axes, and ghost nodes are
+    unchanged."""
+    nm = [7, 6, 5][:dim]
+    rng = np.random.default_rng(dim * 10 + g)
+    t = oracle.topo(dim, nm, bcdef=bc)
+    per = [bc[2 * d] == 1 for d in range(dim)]
+    shape = tuple(reversed([n + 2 * g for n in nm])) + (2,)
+    f = rng.integers(-50, 50, size=shape).astype(np.float64)
+    # node j (1-based) on axis d sits at box index j - 1 + g; arrays are (z, y, x, c)
+    box = [np.arange(1 - g, n + g + 1) for n in nm]
+    info, got = oracle.ghost_get(t, f, [g] * dim)
+    assert info == 0
+    src = [np.array([_ghost_source(j, nm[d]) if per[d] else j for j in box[d]]) - 1 + g for d in range(dim)]
+    expect = f[np.ix_(*[src[d] for d in reversed(range(dim))], np.arange(2))]
+    for d in range(dim):
+        if not per[d]:
+            sl = [slice(None)] * (dim + 1)
+            sl[dim - 1 - d] = (box[d] < 1) | (box[d] > nm[d])
+            expect[tuple(sl)] = f[tuple(sl)]
+    assert np.array_equal(got, expect)
+    info, got = oracle.ghost_put(t, f, [g] * dim)
+    assert info == 0
+    expect = f.copy()
+    real = [(b >= 1) & (b <= n) for b, n in zip(box, nm)]
+    for idx in np.ndindex(*shape[:-1]):
+        node = [box[d][idx[dim - 1 - d]] for d in range(dim)]            # (x, y[, z]) 1-based
+        if not all(real[d][idx[dim - 1 - d]] for d in range(dim)):
+            continue
+        cong = [np.nonzero((box[d] - node[d]) % (nm[d] - 1) == 0)[0] if per[d] else np.nonzero(box[d] == node[d])[0]
+                for d in range(dim)]
+        expect[idx] = f[np.ix_(*[cong[d] for d in reversed(range(dim))], np.arange(2))].reshape(-1, 2).sum(0)
+    assert np.array_equal(got, expect)
+
+
 # -------------------------------------------------------------------- M2P ----
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
 @pytest.mark.parametrize("dim", [2, 3])
