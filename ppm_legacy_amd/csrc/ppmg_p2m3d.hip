@@ -141,15 +141,15 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             n = start[key + 1];   // end, subtracted where used: the load stays in flight
         }
     };
-    auto particle = [&](int f, int n, ValT (&fr)[3], ValT (&u)[LDA]) {
+    // particle p (an index into the caller's arrays) when has
+    auto particle = [&](int p, bool has, ValT (&fr)[3], ValT (&u)[LDA]) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) fr[d] = ValT(0);
 #pragma unroll
         for (int c = 0; c < LDA; ++c) u[c] = ValT(0);
-        if (n > 0) {
+        if (has) {
             // offsets (INT truncation as ppm_interp_p2m.f:800-822, in the position
-            // precision) and strengths, read through the permutation
-            const int p = perm ? perm[f] : f;
+            // precision) and strengths
             const PosT* x = xp + (int64_t)p * 3;
             int ci;
 #pragma unroll
@@ -158,27 +158,38 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             for (int c = 0; c < LDA; ++c) u[c] = up[(int64_t)p * lda + c0 + c];
         }
     };
+    // The permutation entry of a step's first particle is looked up one step
+    // ahead, so a round costs one dependent load (positions and strengths),
+    // not two.
+    auto lookup = [&](int f, int n) { return n > f ? (perm ? perm[f] : f) : 0; };
     int f1, n1, f2, n2;
     heads(z0 - 2, 0, f1, n1);
     if (NC > 1) heads(z0 - 2, 1, f2, n2);
     else heads(z0 - 1, 0, f2, n2);
+    int pn = lookup(f1, n1);
 
     for (int cz = z0 - 2; cz <= z1; ++cz) {
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
             const int first = f1, cnt = n1 - f1;   // n: end of the cell's range
-            // take this step's heads (loaded two steps ago) before the next load is
-            // issued, so that waiting for them does not wait for that load too
-            asm volatile("" ::"v"(first), "v"(cnt) : "memory");
+            const int p0 = pn;
+            // take this step's heads (loaded two steps ago) and first particle
+            // (looked up a step ago) before the next loads are issued, so that
+            // waiting for them does not wait for those loads too
+            asm volatile("" ::"v"(first), "v"(cnt), "v"(p0) : "memory");
             f1 = f2;
             n1 = n2;
+            pn = lookup(f1, n1);
             if (j + 2 < NC) heads(cz, j + 2, f2, n2);
             else if (j + 2 - NC < NC) heads(cz + 1, j + 2 - NC, f2, n2);
             else heads(cz + 2, 0, f2, n2);
             const int ly0 = ey4[j] - 3;              // node row of stencil offset oy = 0
             for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
                 ValT wx[4], wy[4], wz[4], u[LDA], fr[3];
-                particle(first + r, cnt - r, fr, u);
+                // (round 0's entry came with the heads; later rounds, rare, read it
+                // now, and only lanes whose cell still has a particle read at all)
+                const int p = r == 0 ? p0 : (cnt > r ? (perm ? perm[first + r] : first + r) : 0);
+                particle(p, cnt > r, fr, u);
                 mp4_unrolled<ValT>(fr[0], wx);
                 mp4_unrolled<ValT>(fr[1], wy);
                 mp4_unrolled<ValT>(fr[2], wz);
