@@ -141,42 +141,47 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             n = start[key + 1];   // end, subtracted where used: the load stays in flight
         }
     };
-    // particle p (an index into the caller's arrays) when has
-    auto particle = [&](int p, bool has, ValT (&fr)[3], ValT (&u)[LDA]) {
+    // position and strengths of particle p (an index into the caller's
+    // arrays) when has, zeros otherwise
+    auto load = [&](int p, bool has, PosT (&x)[3], ValT (&u)[LDA]) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) fr[d] = ValT(0);
+        for (int d = 0; d < 3; ++d) x[d] = PosT(0);
 #pragma unroll
         for (int c = 0; c < LDA; ++c) u[c] = ValT(0);
         if (has) {
-            // offsets (INT truncation as ppm_interp_p2m.f:800-822, in the position
-            // precision) and strengths
-            const PosT* x = xp + (int64_t)p * 3;
-            int ci;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) fr[d] = (ValT)cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
+            for (int d = 0; d < 3; ++d) x[d] = xp[(int64_t)p * 3 + d];
 #pragma unroll
             for (int c = 0; c < LDA; ++c) u[c] = up[(int64_t)p * lda + c0 + c];
         }
     };
+    // cell offsets (INT truncation as ppm_interp_p2m.f:800-822, in the position precision)
+    auto offsets = [&](const PosT (&x)[3], ValT (&fr)[3]) {
+        int ci;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) fr[d] = (ValT)cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
+    };
     // The permutation entry of a step's first particle is looked up one step
-    // ahead, so a round costs one dependent load (positions and strengths),
-    // not two.
+    // ahead, and its position and strengths at the end of the step before.
     auto lookup = [&](int f, int n) { return n > f ? (perm ? perm[f] : f) : 0; };
     int f1, n1, f2, n2;
     heads(z0 - 2, 0, f1, n1);
     if (NC > 1) heads(z0 - 2, 1, f2, n2);
     else heads(z0 - 1, 0, f2, n2);
     int pn = lookup(f1, n1);
+    // Round 0's particle of the next step is loaded at the end of a step, while
+    // the wavefront waits at the phase barrier; it is not live during the rounds.
+    PosT xn[3];
+    ValT un[LDA];
+    load(pn, n1 > f1, xn, un);
 
     for (int cz = z0 - 2; cz <= z1; ++cz) {
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
             const int first = f1, cnt = n1 - f1;   // n: end of the cell's range
-            const int p0 = pn;
-            // take this step's heads (loaded two steps ago) and first particle
-            // (looked up a step ago) before the next loads are issued, so that
-            // waiting for them does not wait for those loads too
-            asm volatile("" ::"v"(first), "v"(cnt), "v"(p0) : "memory");
+            // take this step's heads (loaded two steps ago) before the next loads
+            // are issued, so that waiting for them does not wait for those loads too
+            asm volatile("" ::"v"(first), "v"(cnt) : "memory");
             f1 = f2;
             n1 = n2;
             pn = lookup(f1, n1);
@@ -184,12 +189,11 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             else if (j + 2 - NC < NC) heads(cz + 1, j + 2 - NC, f2, n2);
             else heads(cz + 2, 0, f2, n2);
             const int ly0 = ey4[j] - 3;              // node row of stencil offset oy = 0
-            for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
-                ValT wx[4], wy[4], wz[4], u[LDA], fr[3];
-                // (round 0's entry came with the heads; later rounds, rare, read it
-                // now, and only lanes whose cell still has a particle read at all)
-                const int p = r == 0 ? p0 : (cnt > r ? (perm ? perm[first + r] : first + r) : 0);
-                particle(p, cnt > r, fr, u);
+            // one round: every lane deposits one particle of its cell (zero
+            // strength where it has none)
+            auto round = [&](const PosT (&x)[3], const ValT (&u)[LDA]) {
+                ValT wx[4], wy[4], wz[4], fr[3];
+                offsets(x, fr);
                 mp4_unrolled<ValT>(fr[0], wx);
                 mp4_unrolled<ValT>(fr[1], wy);
                 mp4_unrolled<ValT>(fr[2], wz);
@@ -224,7 +228,18 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                         }
                     }
                 }
+            };
+            // round 0 with the particle loaded during the previous step; later
+            // rounds (rare) load theirs, only lanes whose cell still has one
+            if (__ballot(cnt > 0) != 0ull) round(xn, un);
+            for (int r = 1; __ballot(cnt > r) != 0ull; ++r) {
+                PosT x[3];
+                ValT u[LDA];
+                const int p = cnt > r ? (perm ? perm[first + r] : first + r) : 0;
+                load(p, cnt > r, x, u);
+                round(x, u);
             }
+            load(pn, n1 > f1, xn, un);
             // end of a phase: the other phase's chunks share node rows with this one's
             if (NW > 1 || j == NC - 1) __syncthreads();
         }
