@@ -121,6 +121,11 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     };
     int first_n, cnt_n;
     heads(z0, first_n, cnt_n);
+    // Round 0's particle of a plane is loaded at the end of the plane before,
+    // while the workgroup refills the ring.
+    int pn;
+    PosT xn[3];
+    particle(first_n, cnt_n - first_n, pn, xn);
     __syncthreads();
 
     for (int cz = z0; cz < z1; ++cz) {
@@ -131,13 +136,10 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         heads(cz + 1, first_n, cnt_n);
         const bool more = cz + 1 < z1;
         if (more) fetch_plane(cz + 3);
-        for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
-            const bool has = cnt > r;
-            int p = 0;
+        // one round: the lanes of cells with a particle left interpolate it
+        auto round = [&](int p, bool has, const PosT (&x)[3]) {
             AccT wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, wz[4] = {0, 0, 0, 0};
             if (has) {
-                PosT x[3];
-                particle(first + r, cnt - r, p, x);
                 int ci;
                 mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci), wx);
                 mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci), wy);
@@ -176,7 +178,15 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                 u += wx[3] * row_shl<2>(C[3]);
                 if (has) up[(int64_t)p * lda + c0 + c] = (ValT)u;
             }
+        };
+        if (__ballot(cnt > 0) != 0ull) round(pn, cnt > 0, xn);
+        for (int r = 1; __ballot(cnt > r) != 0ull; ++r) {
+            int p;
+            PosT x[3];
+            particle(first + r, cnt - r, p, x);
+            round(p, cnt > r, x);
         }
+        particle(first_n, cnt_n - first_n, pn, xn);   // next plane's round 0
         __syncthreads();   // every row is done with plane cz-1: its slot takes plane cz+3
         if (more) store_plane(cz + 3);
         __syncthreads();
