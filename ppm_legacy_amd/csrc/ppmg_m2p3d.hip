@@ -155,7 +155,9 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                 a[3][o] = row_shr<2>(wy[o]); b[3][o] = row_shr<2>(wz[o]);
             }
             // per component: column partial sums Ck = sum_oz b_k[oz] * sum_oy a_k[oy] f(oz, oy)
-            // (one 4-node y slice of the column live at a time), then the particle's sum
+            // (one 4-node y slice of the column live at a time), then the particle's sum;
+            // the components are stored together (one wide store per lane)
+            ValT res[LDA];
 #pragma unroll
             for (int c = 0; c < LDA; ++c) {
                 AccT C[4] = {AccT(0), AccT(0), AccT(0), AccT(0)};
@@ -176,7 +178,14 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                 u += wx[1] * C[1];
                 u += wx[2] * row_shl<1>(C[2]);
                 u += wx[3] * row_shl<2>(C[3]);
-                if (has) up[(int64_t)p * lda + c0 + c] = (ValT)u;
+                res[c] = (ValT)u;
+                // one component at a time: keeps the column reads of the next
+                // component after this one's sums (register pressure)
+                asm volatile("" : "+v"(res[c])::"memory");
+            }
+            if (has) {
+#pragma unroll
+                for (int c = 0; c < LDA; ++c) up[(int64_t)p * lda + c0 + c] = res[c];
             }
         };
         if (__ballot(cnt > 0) != 0ull) round(pn, cnt > 0, xn);
