@@ -41,9 +41,10 @@ constexpr int M3_RY = M3_TY + 3;        // node rows in the ring
 constexpr int M3_PL = M3_RY * 16;       // node slots per ring plane and component
 constexpr int M3_THREADS = 16 * M3_TY;  // 256
 
-// WPE: minimum wavefronts per SIMD the register allocation must allow.  The
-// FP64 ring runs 2 (no spills); the FP32 ring (FP32c and FP32 forms) runs 3,
-// at the cost of a few spills reloaded once per plane, outside the particle loop.
+// WPE: minimum wavefronts per SIMD the register allocation must allow (1: no
+// constraint).  FP64 needs 158 VGPRs and runs 3 wavefronts per SIMD as it is;
+// the FP32 ring (FP32c and FP32 forms) is held to 3 (134-140 VGPRs, no
+// spills).  Holding it to 4 spills inside the plane loop and is slower.
 template <int LDA, typename PosT, typename ValT, typename AccT, int WPE>
 __global__ void __launch_bounds__(M3_THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
