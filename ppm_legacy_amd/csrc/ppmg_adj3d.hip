@@ -34,7 +34,7 @@ constexpr int A3_PL = A3_RY * 16;
 constexpr int A3_THREADS = 16 * A3_TY;
 
 template <int LDA, typename PosT, typename ValT, typename AccT>
-__global__ void __launch_bounds__(A3_THREADS)
+__global__ void __launch_bounds__(A3_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
 k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
           const ValT* __restrict__ up, const ValT* __restrict__ g, ValT* __restrict__ grad_u,
           ValT* __restrict__ grad_x, int lda, int c0, int tz, int accumulate_x) {
@@ -137,51 +137,60 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
 #pragma unroll
                 for (int c = 0; c < LDA; ++c) u[c] = (AccT)up[(int64_t)p * lda + c0 + c];
             }
-            // y and z weights (and derivatives) of the particles of lanes ex+1, ex, ex-1, ex-2
-            AccT a[4][4], da[4][4], b[4][4], db[4][4];
+            // One pass per stencil x-offset k: the y and z weights (and
+            // derivatives) of the particle k-1 lanes to the right arrive by DPP,
+            // every component's column is contracted with them, and the three
+            // partial sums go back to that particle at once, where its x weights
+            // finish them.  Only one offset's weights are live at a time, so all
+            // components fit in one launch (the columns are read once per offset).
+            AccT gu[LDA];
 #pragma unroll
-            for (int o = 0; o < 4; ++o) {
-                a[0][o] = row_shl<1>(wy[o]); da[0][o] = row_shl<1>(dy[o]);
-                b[0][o] = row_shl<1>(wz[o]); db[0][o] = row_shl<1>(dz[o]);
-                a[1][o] = wy[o];             da[1][o] = dy[o];
-                b[1][o] = wz[o];             db[1][o] = dz[o];
-                a[2][o] = row_shr<1>(wy[o]); da[2][o] = row_shr<1>(dy[o]);
-                b[2][o] = row_shr<1>(wz[o]); db[2][o] = row_shr<1>(dz[o]);
-                a[3][o] = row_shr<2>(wy[o]); da[3][o] = row_shr<2>(dy[o]);
-                b[3][o] = row_shr<2>(wz[o]); db[3][o] = row_shr<2>(dz[o]);
-            }
+            for (int c = 0; c < LDA; ++c) gu[c] = AccT(0);
             AccT gx0 = AccT(0), gx1 = AccT(0), gx2 = AccT(0);
 #pragma unroll
-            for (int c = 0; c < LDA; ++c) {
-                AccT C0[4] = {0, 0, 0, 0}, Cy[4] = {0, 0, 0, 0}, Cz[4] = {0, 0, 0, 0};
+            for (int k = 0; k < 4; ++k) {
+                AccT a[4], da[4], b[4], db[4];
 #pragma unroll
-                for (int oz = 0; oz < 4; ++oz) {
-                    const ValT* f = ring + ((cz - 1 + oz) & 3) * (LDA * A3_PL) + c * A3_PL + row * 16 + ex;
-                    const AccT g0 = (AccT)f[0], g1 = (AccT)f[16], g2 = (AccT)f[32], g3 = (AccT)f[48];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        AccT t = a[k][0] * g0;
-                        t += a[k][1] * g1;
-                        t += a[k][2] * g2;
-                        t += a[k][3] * g3;
-                        AccT td = da[k][0] * g0;
-                        td += da[k][1] * g1;
-                        td += da[k][2] * g2;
-                        td += da[k][3] * g3;
-                        C0[k] += b[k][oz] * t;
-                        Cz[k] += db[k][oz] * t;
-                        Cy[k] += b[k][oz] * td;
-                    }
+                for (int o = 0; o < 4; ++o) {
+                    a[o] = k == 0 ? row_shl<1>(wy[o]) : k == 1 ? wy[o] : k == 2 ? row_shr<1>(wy[o]) : row_shr<2>(wy[o]);
+                    da[o] = k == 0 ? row_shl<1>(dy[o]) : k == 1 ? dy[o] : k == 2 ? row_shr<1>(dy[o]) : row_shr<2>(dy[o]);
+                    b[o] = k == 0 ? row_shl<1>(wz[o]) : k == 1 ? wz[o] : k == 2 ? row_shr<1>(wz[o]) : row_shr<2>(wz[o]);
+                    db[o] = k == 0 ? row_shl<1>(dz[o]) : k == 1 ? dz[o] : k == 2 ? row_shr<1>(dz[o]) : row_shr<2>(dz[o]);
                 }
-                // back to the particle lanes: C_k of lane P-1+k
-                const AccT s00 = row_shr<1>(C0[0]), s02 = row_shl<1>(C0[2]), s03 = row_shl<2>(C0[3]);
-                const AccT sy0 = row_shr<1>(Cy[0]), sy2 = row_shl<1>(Cy[2]), sy3 = row_shl<2>(Cy[3]);
-                const AccT sz0 = row_shr<1>(Cz[0]), sz2 = row_shl<1>(Cz[2]), sz3 = row_shl<2>(Cz[3]);
-                const AccT gu = wx[0] * s00 + wx[1] * C0[1] + wx[2] * s02 + wx[3] * s03;
-                gx0 += u[c] * (dx[0] * s00 + dx[1] * C0[1] + dx[2] * s02 + dx[3] * s03);
-                gx1 += u[c] * (wx[0] * sy0 + wx[1] * Cy[1] + wx[2] * sy2 + wx[3] * sy3);
-                gx2 += u[c] * (wx[0] * sz0 + wx[1] * Cz[1] + wx[2] * sz2 + wx[3] * sz3);
-                if (has && grad_u) grad_u[(int64_t)p * lda + c0 + c] = (ValT)gu;
+#pragma unroll
+                for (int c = 0; c < LDA; ++c) {
+                    AccT C0 = AccT(0), Cy = AccT(0), Cz = AccT(0);
+#pragma unroll
+                    for (int oz = 0; oz < 4; ++oz) {
+                        const ValT* f = ring + ((cz - 1 + oz) & 3) * (LDA * A3_PL) + c * A3_PL + row * 16 + ex;
+                        const AccT g0 = (AccT)f[0], g1 = (AccT)f[16], g2 = (AccT)f[32], g3 = (AccT)f[48];
+                        AccT t = a[0] * g0;
+                        t += a[1] * g1;
+                        t += a[2] * g2;
+                        t += a[3] * g3;
+                        AccT td = da[0] * g0;
+                        td += da[1] * g1;
+                        td += da[2] * g2;
+                        td += da[3] * g3;
+                        C0 += b[oz] * t;
+                        Cz += db[oz] * t;
+                        Cy += b[oz] * td;
+                    }
+                    // back to the particle: the sums of lane P-1+k
+                    const AccT s0 = k == 0 ? row_shr<1>(C0) : k == 1 ? C0 : k == 2 ? row_shl<1>(C0) : row_shl<2>(C0);
+                    const AccT sy = k == 0 ? row_shr<1>(Cy) : k == 1 ? Cy : k == 2 ? row_shl<1>(Cy) : row_shl<2>(Cy);
+                    const AccT sz = k == 0 ? row_shr<1>(Cz) : k == 1 ? Cz : k == 2 ? row_shl<1>(Cz) : row_shl<2>(Cz);
+                    gu[c] += wx[k] * s0;
+                    gx0 += u[c] * dx[k] * s0;
+                    gx1 += u[c] * wx[k] * sy;
+                    gx2 += u[c] * wx[k] * sz;
+                    // one component at a time (register pressure)
+                    asm volatile("" : "+v"(gx0), "+v"(gx1), "+v"(gx2)::"memory");
+                }
+            }
+            if (has && grad_u) {
+#pragma unroll
+                for (int c = 0; c < LDA; ++c) grad_u[(int64_t)p * lda + c0 + c] = (ValT)gu[c];
             }
             if (has && grad_x) {
                 ValT* o = grad_x + (int64_t)p * 3;
@@ -221,8 +230,13 @@ static void launch_a3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const V
 template <typename PosT, typename ValT>
 static int adjoint3d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda, const ValT* up, const ValT* g,
                            ValT* gu, ValT* gx, hipStream_t s) {
-    for (int c0 = 0; c0 < lda; ++c0) {
-        launch_a3<1, PosT, ValT, double>(plan, m, xp, up, g, gu, gx, lda, c0, c0 > 0 ? 1 : 0, s);
+    for (int c0 = 0; c0 < lda; c0 += 4) {
+        const int L = lda - c0 < 4 ? lda - c0 : 4;
+        const int acc = c0 > 0 ? 1 : 0;
+        if (L == 1) launch_a3<1, PosT, ValT, double>(plan, m, xp, up, g, gu, gx, lda, c0, acc, s);
+        else if (L == 2) launch_a3<2, PosT, ValT, double>(plan, m, xp, up, g, gu, gx, lda, c0, acc, s);
+        else if (L == 3) launch_a3<3, PosT, ValT, double>(plan, m, xp, up, g, gu, gx, lda, c0, acc, s);
+        else launch_a3<4, PosT, ValT, double>(plan, m, xp, up, g, gu, gx, lda, c0, acc, s);
         PPMG_HIP(hipGetLastError());
     }
     return 0;
