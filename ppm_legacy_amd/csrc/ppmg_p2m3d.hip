@@ -30,6 +30,7 @@ namespace ppmg {
 
 constexpr int P3_TT = 13;   // owned nodes per tile along x (16-lane DPP row = 13 + 3 halo cells)
 constexpr int P3_TY = 21;   // owned node rows per tile along y (24 cell rows: 6 chunks, 3 wavefronts)
+constexpr int P3_SERIAL = 4; // later rounds with at most this many active lanes go one particle at a time
 
 // LDS accumulator of one node component.  Every form accumulates in an FP64
 // slot with ds_add_f64 (fire-and-forget, no read-modify-write latency chain):
@@ -229,15 +230,54 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                     }
                 }
             };
-            // round 0 with the particle loaded during the previous step; later
-            // rounds (rare) load theirs, only lanes whose cell still has one
+            // round 0 with the particle loaded during the previous step
             if (__ballot(cnt > 0) != 0ull) round(xn, un);
-            for (int r = 1; __ballot(cnt > r) != 0ull; ++r) {
-                PosT x[3];
-                ValT u[LDA];
-                const int p = cnt > r ? (perm ? perm[first + r] : first + r) : 0;
-                load(p, cnt > r, x, u);
-                round(x, u);
+            // Later rounds load their own particles.  A round costs the same
+            // however few lanes still have a particle, so once at most P3_SERIAL
+            // lanes do, their remaining particles are deposited one at a time:
+            // the 64 lanes take the 64 nodes of the particle's stencil (distinct
+            // nodes, so the additions stay in a fixed order).
+            for (int r = 1;; ++r) {
+                uint64_t act = __ballot(cnt > r);
+                if (act == 0ull) break;
+                if (__popcll(act) > P3_SERIAL) {
+                    PosT x[3];
+                    ValT u[LDA];
+                    const int p = cnt > r ? (perm ? perm[first + r] : first + r) : 0;
+                    load(p, cnt > r, x, u);
+                    round(x, u);
+                    continue;
+                }
+                const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
+                slot_t* pl = ring + ((cz - 1 + oz) & 3) * PLANE;
+                const int qrow = ey4[j] - erow;      // first cell row of the chunk
+                do {
+                    const int src = __builtin_ctzll(act);
+                    act &= act - 1;
+                    const int sf = __builtin_amdgcn_readlane(first, src);
+                    const int sc = __builtin_amdgcn_readlane(cnt, src);
+                    // node of this lane's stencil offset, relative to the tile
+                    const int nx = (src & 15) - 3 + ox, ny = qrow + (src >> 4) - 3 + oy;
+                    const bool own = nx >= 0 && nx < nxo && ny >= 0 && ny < nyo;
+                    for (int rr = r; rr < sc; ++rr) {
+                        PosT x[3];
+                        ValT u[LDA], fr[3], wx[4], wy[4], wz[4];
+                        load(perm ? perm[sf + rr] : sf + rr, true, x, u);
+                        offsets(x, fr);
+                        mp4_unrolled<ValT>(fr[0], wx);
+                        mp4_unrolled<ValT>(fr[1], wy);
+                        mp4_unrolled<ValT>(fr[2], wz);
+                        auto pick = [](const ValT (&w)[4], int o) {
+                            return o == 0 ? w[0] : o == 1 ? w[1] : o == 2 ? w[2] : w[3];
+                        };
+                        const ValT wyz = pick(wy, oy) * pick(wz, oz), wxo = pick(wx, ox);
+                        if (own) {
+#pragma unroll
+                            for (int c = 0; c < LDA; ++c) A::add(pl + c * P3_PL + nx + P3_TT * ny, wyz * (wxo * u[c]));
+                        }
+                    }
+                } while (act != 0ull);
+                break;
             }
             load(pn, n1 > f1, xn, un);
             // end of a phase: the other phase's chunks share node rows with this one's
