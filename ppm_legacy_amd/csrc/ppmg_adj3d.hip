@@ -223,10 +223,10 @@ static void launch_a3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const V
                        tile_perm(plan), xp, up, g, gu, gx, lda, c0, tz, acc);
 }
 
-// One launch per component: the four weight sets (y, z and their
-// derivatives) of four neighbouring particles take most of the registers, and
-// one component at a time keeps the kernel at 3 wavefronts per SIMD; the
-// position gradient accumulates over the launches.
+// One launch per four components (the position gradient accumulates over
+// launches when lda > 4); inside a launch the components are contracted one
+// after another and the stencil x-offsets one per pass, which keeps the
+// kernel at 3 wavefronts per SIMD.
 template <typename PosT, typename ValT>
 static int adjoint3d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda, const ValT* up, const ValT* g,
                            ValT* gu, ValT* gx, hipStream_t s) {
