@@ -52,22 +52,28 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
 
     constexpr int NLD = (A3_PL * LDA + A3_THREADS - 1) / A3_THREADS;
     ValT stage[NLD];
+    // Byte offset within a field plane of each element this thread stages
+    // (the same in every plane; a plane fits in 2^31 bytes, checked at launch).
+    unsigned soff[NLD];
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) {
+        const int i = tid + j * A3_THREADS;
+        const int c = i % LDA;
+        const int node = i / LDA;
+        const int nx = node & 15, ry = node >> 4;
+        int gx = (tx0 - 1 + nx) % Nx;
+        if (gx < 0) gx += Nx;
+        int gy = (ty0 - 1 + ry) % Ny;
+        if (gy < 0) gy += Ny;
+        soff[j] = (unsigned)(((gy * Nx + gx) * lda + c0 + c) * (int)sizeof(ValT));
+    }
     auto fetch_plane = [&](int n) {
         int nz = n % Nz;
         if (nz < 0) nz += Nz;
+        const char* base = (const char*)(g + (int64_t)nz * plane * lda);
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
-            const int i = tid + j * A3_THREADS;
-            if (i < A3_PL * LDA) {
-                const int c = i % LDA;
-                const int node = i / LDA;
-                const int nx = node & 15, ry = node >> 4;
-                int gx = (tx0 - 1 + nx) % Nx;
-                if (gx < 0) gx += Nx;
-                int gy = (ty0 - 1 + ry) % Ny;
-                if (gy < 0) gy += Ny;
-                stage[j] = g[((int64_t)nz * plane + (int64_t)gy * Nx + gx) * lda + c0 + c];
-            }
+            if (tid + j * A3_THREADS < A3_PL * LDA) stage[j] = *(const ValT*)(base + soff[j]);
         }
     };
     auto store_plane = [&](int n) {
@@ -248,7 +254,8 @@ bool adjoint3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg
         if (!m.periodic[d] || m.N[d] < 4) return false;
         if (fg.lo[d] != 0 || fg.ext[d] != m.N[d]) return false;
     }
-    return true;
+    // the ring fetch addresses a field plane with 32-bit byte offsets
+    return (int64_t)m.N[0] * m.N[1] * fg.lda * 8 < ((int64_t)1 << 31);
 }
 
 // Uses the forward's binning in the plan (every particle is inside the sub:

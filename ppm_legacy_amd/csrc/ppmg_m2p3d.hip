@@ -42,9 +42,9 @@ constexpr int M3_PL = M3_RY * 16;       // node slots per ring plane and compone
 constexpr int M3_THREADS = 16 * M3_TY;  // 256
 
 // WPE: minimum wavefronts per SIMD the register allocation must allow (1: no
-// constraint).  FP64 needs 158 VGPRs and runs 3 wavefronts per SIMD as it is;
-// the FP32 ring (FP32c and FP32 forms) is held to 3 (134-140 VGPRs, no
-// spills).  Holding it to 4 spills inside the plane loop and is slower.
+// constraint).  FP64 needs 150 VGPRs at lda 3 and runs 3 wavefronts per SIMD;
+// the FP32 ring (FP32c and FP32 forms) is allowed 3 and comes out at 126-132
+// VGPRs (FP32c: 4 wavefronts), with no spills.
 template <int LDA, typename PosT, typename ValT, typename AccT, int WPE>
 __global__ void __launch_bounds__(M3_THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
@@ -65,23 +65,29 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     // plane cz+3 are in flight while plane cz is computed.
     constexpr int NLD = (M3_PL * LDA + M3_THREADS - 1) / M3_THREADS;
     ValT stage[NLD];
+    // Byte offset within a field plane of each element this thread stages
+    // (the same in every plane; a plane fits in 2^31 bytes, checked at launch).
+    unsigned soff[NLD];
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) {
+        const int i = tid + j * M3_THREADS;
+        const int c = i % LDA;
+        const int node = i / LDA;
+        const int nx = node & 15, ry = node >> 4;
+        int gx = (tx0 - 1 + nx) % Nx;
+        if (gx < 0) gx += Nx;
+        int gy = (ty0 - 1 + ry) % Ny;
+        if (gy < 0) gy += Ny;
+        soff[j] = (unsigned)(((gy * Nx + gx) * lda + c0 + c) * (int)sizeof(ValT));
+    }
     auto fetch_plane = [&](int n) {
         int nz = n % Nz;
         if (nz < 0) nz += Nz;
+        const char* base = (const char*)(field + (int64_t)nz * plane * lda);
         // component fastest in the field (coalesced), component-major in the ring
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
-            const int i = tid + j * M3_THREADS;
-            if (i < M3_PL * LDA) {
-                const int c = i % LDA;
-                const int node = i / LDA;
-                const int nx = node & 15, ry = node >> 4;
-                int gx = (tx0 - 1 + nx) % Nx;
-                if (gx < 0) gx += Nx;
-                int gy = (ty0 - 1 + ry) % Ny;
-                if (gy < 0) gy += Ny;
-                stage[j] = field[((int64_t)nz * plane + (int64_t)gy * Nx + gx) * lda + c0 + c];
-            }
+            if (tid + j * M3_THREADS < M3_PL * LDA) stage[j] = *(const ValT*)(base + soff[j]);
         }
     };
     auto store_plane = [&](int n) {
@@ -209,7 +215,8 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
         if (!m.periodic[d] || m.N[d] < 4) return false;
         if (fg.lo[d] != 0 || fg.ext[d] != m.N[d]) return false;
     }
-    return true;
+    // the ring fetch addresses a field plane with 32-bit byte offsets
+    return (int64_t)m.N[0] * m.N[1] * fg.lda * 8 < ((int64_t)1 << 31);
 }
 
 // One launch of the gather for components c0 .. c0+LL-1; the z chunk comes

@@ -128,6 +128,9 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     }
     __syncthreads();
     const int64_t plane_cells = (int64_t)Nx * Ny;
+    constexpr int WROW = P3_TT * LDA;              // values of one node row
+    constexpr int WRG = NT / WROW;                 // node rows written at once
+    static_assert(WRG >= 1, "a workgroup writes at least one node row at once");
 
     // The cell heads of the step after next are loaded while the current
     // (plane, chunk) step computes.
@@ -291,16 +294,30 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         const int n = cz - 1;
         slot_t* pl = ring + (n & 3) * PLANE;
         if (n >= z0 && n < z1) {
-            const int cntw = nyo * nxo * LDA;
-            ValT* out = field + (int64_t)n * plane_cells * lda + c0;
-            for (int i = tid; i < cntw; i += NT) {
-                int c = i % LDA;
-                int nd = i / LDA;
-                int x = nd % nxo, y = nd / nxo;
-                slot_t& a = pl[c * P3_PL + x + P3_TT * y];
-                const ValT v = (ValT)A::value(a);
-                A::zero(a);
-                out[((int64_t)(ty0 + y) * Nx + (tx0 + x)) * lda + c] = v;
+            // thread (wx, wc, wy0) writes node wx, component wc of rows wy0,
+            // wy0 + WRG, ...  A field plane fits in 2^31 bytes (checked at
+            // launch): 32-bit byte offsets.  The slots are recomputed here
+            // (the barrier keeps them from being hoisted and held in registers
+            // through the rounds).
+            int t = tid;
+            asm volatile("" : "+v"(t));
+            const int wy0 = t / WROW, wx = (t % WROW) / LDA, wc = t % LDA;
+            const bool wact = wy0 < WRG && wx < nxo;
+            const int wlds = wc * P3_PL + wx + P3_TT * wy0;
+            const unsigned wbyte =
+                (unsigned)(((ty0 + wy0) * Nx + tx0 + wx) * lda + c0 + wc) * (unsigned)sizeof(ValT);
+            const unsigned wstep = (unsigned)(WRG * Nx * lda) * (unsigned)sizeof(ValT);
+            const char* out = (const char*)(field + (int64_t)n * plane_cells * lda);
+            if (wact) {
+#pragma unroll
+                for (int k = 0; k < (TY + WRG - 1) / WRG; ++k) {
+                    if (wy0 + k * WRG < nyo) {
+                        slot_t& a = pl[wlds + k * WRG * P3_TT];
+                        const ValT v = (ValT)A::value(a);
+                        A::zero(a);
+                        *(ValT*)(out + (wbyte + (unsigned)k * wstep)) = v;
+                    }
+                }
             }
         } else {
             for (int i = tid; i < PLANE; i += NT) A::zero(pl[i]);
@@ -315,8 +332,9 @@ bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGe
         if (!m.periodic[d] || m.N[d] < 4) return false;
         if (fg.lo[d] != 0 || fg.ext[d] != m.N[d]) return false;
     }
-    (void)prec;
-    return true;
+    // the write-out addresses a field plane with 32-bit byte offsets
+    const int64_t vsize = prec == PPMG_PREC_F64 ? 8 : 4;
+    return (int64_t)m.N[0] * m.N[1] * fg.lda * vsize < ((int64_t)1 << 31);
 }
 
 // One launch of the deposit for components c0 .. c0+LL-1 with tiles of
