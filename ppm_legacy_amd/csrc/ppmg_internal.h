@@ -166,6 +166,18 @@ __device__ __forceinline__ void mp4_unrolled(T frac, T a[4]) {
     a[3] = T(2) + (T(4) + (T(2.5) + T(0.5) * x13) * x13) * x13;
 }
 
+// Weight o (0..3) of mp4_unrolled alone, without branches.  With
+// t = (frac + 1) - o the four unrolled polynomials are the inner form
+// 1 + (-2.5 + 1.5|t|) t^2 (o = 1, 2) and the outer form
+// 2 + (-4 + (2.5 - 0.5|t|)|t|)|t| (o = 0, 3), up to exact sign changes.
+template <typename T>
+__device__ __forceinline__ T mp4_at(T frac, int o) {
+    const T s = fabs((frac + T(1)) - T(o));
+    const T in = T(1) + (T(-2.5) + T(1.5) * s) * (s * s);
+    const T out = T(2) + (T(-4) + (T(2.5) - T(0.5) * s) * s) * s;
+    return (o == 1 || o == 2) ? in : out;
+}
+
 // M'4 by distance branch, the 2D form of ppm_interp_p2m.f:3620-3651.
 template <typename T>
 __device__ __forceinline__ T mp4_branch(T r) {

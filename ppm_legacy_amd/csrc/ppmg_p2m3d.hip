@@ -216,19 +216,18 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                     const int ly = ly0 + oy;
                     const bool own = xown && ly >= 0 && ly < nyo;
                     const int node = lx + P3_TT * ly;
+                    // per component: fold the y weight in first (4 products),
+                    // then one z contraction per z offset
 #pragma unroll
-                    for (int oz = 0; oz < 4; ++oz) {
-                        slot_t* pl = ring + ((cz - 1 + oz) & 3) * PLANE;
-                        ValT v[LDA];
+                    for (int c = 0; c < LDA; ++c) {
+                        const ValT H[4] = {WY[0] * G[0][c], WY[1] * G[1][c], WY[2] * G[2][c], WY[3] * G[3][c]};
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const ValT w = WY[k] * WZ[k][oz];
-#pragma unroll
-                            for (int c = 0; c < LDA; ++c) v[c] = k == 0 ? w * G[k][c] : v[c] + w * G[k][c];
-                        }
-                        if (own) {
-#pragma unroll
-                            for (int c = 0; c < LDA; ++c) A::add(pl + c * P3_PL + node, v[c]);
+                        for (int oz = 0; oz < 4; ++oz) {
+                            ValT v = WZ[0][oz] * H[0];
+                            v += WZ[1][oz] * H[1];
+                            v += WZ[2][oz] * H[2];
+                            v += WZ[3][oz] * H[3];
+                            if (own) A::add(ring + ((cz - 1 + oz) & 3) * PLANE + c * P3_PL + node, v);
                         }
                     }
                 }
@@ -264,16 +263,10 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                     const bool own = nx >= 0 && nx < nxo && ny >= 0 && ny < nyo;
                     for (int rr = r; rr < sc; ++rr) {
                         PosT x[3];
-                        ValT u[LDA], fr[3], wx[4], wy[4], wz[4];
+                        ValT u[LDA], fr[3];
                         load(perm ? perm[sf + rr] : sf + rr, true, x, u);
                         offsets(x, fr);
-                        mp4_unrolled<ValT>(fr[0], wx);
-                        mp4_unrolled<ValT>(fr[1], wy);
-                        mp4_unrolled<ValT>(fr[2], wz);
-                        auto pick = [](const ValT (&w)[4], int o) {
-                            return o == 0 ? w[0] : o == 1 ? w[1] : o == 2 ? w[2] : w[3];
-                        };
-                        const ValT wyz = pick(wy, oy) * pick(wz, oz), wxo = pick(wx, ox);
+                        const ValT wyz = mp4_at<ValT>(fr[1], oy) * mp4_at<ValT>(fr[2], oz), wxo = mp4_at<ValT>(fr[0], ox);
                         if (own) {
 #pragma unroll
                             for (int c = 0; c < LDA; ++c) A::add(pl + c * P3_PL + nx + P3_TT * ny, wyz * (wxo * u[c]));
