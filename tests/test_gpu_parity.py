@@ -259,6 +259,26 @@ def test_p2m_tile_heights(core, rows, N, monkeypatch):
     assert _rel(got, gen.cpu().numpy()) < 1e-13
 
 
+@pytest.mark.parametrize("rows", [9, 13, 29])
+@pytest.mark.parametrize("lda", [1, 2, 4])
+def test_p2m_tile_heights_components(core, rows, lda, monkeypatch):
+    """Tile heights against component counts: the write-out hands each thread
+    one (node, component) of one node row per pass, 64 to 256 threads over rows
+    of 13 x lda values.  Ragged tiles (N = 50) against the general path."""
+    monkeypatch.setenv("PPMG_P2M_ROWS", str(rows))
+    N = 50
+    xp = wl.advect(wl.jittered_lattice(N, seed=rows + lda), wl.fourier_modes(rows), N)[0]
+    up = wl.strengths(rows + lda, len(xp), lda)
+    mesh = core.Mesh(3, N + 1)
+    x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
+    f = torch.empty(mesh.compact_shape(lda), dtype=torch.float64, device="cuda")
+    core.p2m(mesh, x, u, f, [0] * 3, MP4, F64)
+    monkeypatch.setenv("PPMG_GENERAL", "1")
+    gen = torch.empty_like(f)
+    core.p2m(mesh, x, u, gen, [0] * 3, MP4, F64)
+    assert _rel(f.cpu().numpy(), gen.cpu().numpy()) < 1e-13
+
+
 def test_p2m_nonperiodic_axes(core, oracle):
     N = 10
     xp, up = _case(3, N, 800, 11, 2)
