@@ -353,14 +353,32 @@ __global__ void __launch_bounds__(256) k_rank_keys(const PosT* __restrict__ xp, 
     slot[p] = atomicAdd(&count[k], 1);
 }
 
+// Four particles per thread: keys and slots as 128-bit loads (plan buffers
+// are 256-byte aligned), then the four head loads in flight together.
 __global__ void __launch_bounds__(256) k_scatter(const int* __restrict__ key, const int* __restrict__ slot,
                                                  const int* __restrict__ start, int64_t np,
                                                  int* __restrict__ perm) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    int k = key[p];
-    if (k < 0) return;
-    perm[start[k] + slot[p]] = (int)p;
+    const int64_t p0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (p0 >= np) return;
+    int k[4], sl[4];
+    if (p0 + 4 <= np) {
+        const int4 kv = *reinterpret_cast<const int4*>(key + p0);
+        const int4 sv = *reinterpret_cast<const int4*>(slot + p0);
+        k[0] = kv.x; k[1] = kv.y; k[2] = kv.z; k[3] = kv.w;
+        sl[0] = sv.x; sl[1] = sv.y; sl[2] = sv.z; sl[3] = sv.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            k[i] = p0 + i < np ? key[p0 + i] : -1;
+            sl[i] = p0 + i < np ? slot[p0 + i] : 0;
+        }
+    }
+    int h[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) h[i] = k[i] >= 0 ? start[k[i]] : 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (k[i] >= 0) perm[h[i] + sl[i]] = (int)(p0 + i);
 }
 
 // Restore increasing particle index inside every cell (stability).
@@ -381,22 +399,33 @@ __global__ void __launch_bounds__(256) k_stabilize(const int* __restrict__ start
     }
 }
 
-// The same for the cells k_keys listed (those with two or more particles):
-// workgroup b sorts the cells listed by key workgroup b.
-__global__ void __launch_bounds__(256) k_stabilize_listed(const int* __restrict__ start,
-                                                          const int* __restrict__ multi,
-                                                          const int* __restrict__ multi_n, int* __restrict__ perm) {
-    if ((int)threadIdx.x >= multi_n[blockIdx.x]) return;
-    const int c = multi[(int64_t)blockIdx.x * 256 + threadIdx.x];
-    const int b = start[c], e = start[c + 1];
-    for (int i = b + 1; i < e; ++i) {
-        int v = perm[i];
-        int j = i - 1;
-        while (j >= b && perm[j] > v) {
-            perm[j + 1] = perm[j];
-            --j;
+// The same for the cells k_keys listed (those with two or more particles).
+// A key workgroup lists a few cells at most, so one wavefront takes the lists
+// of STAB_GROUP consecutive key workgroups, spread over its lanes.
+constexpr int STAB_GROUP = 8;
+__global__ void __launch_bounds__(64) k_stabilize_listed(const int* __restrict__ start,
+                                                         const int* __restrict__ multi,
+                                                         const int* __restrict__ multi_n, int64_t nlists,
+                                                         int* __restrict__ perm) {
+    const int64_t l0 = (int64_t)blockIdx.x * STAB_GROUP;
+    int off[STAB_GROUP + 1];
+    off[0] = 0;
+#pragma unroll
+    for (int j = 0; j < STAB_GROUP; ++j) off[j + 1] = off[j] + (l0 + j < nlists ? multi_n[l0 + j] : 0);
+    for (int q = (int)threadIdx.x; q < off[STAB_GROUP]; q += 64) {
+        int j = 0;
+        while (q >= off[j + 1]) ++j;
+        const int c = multi[(l0 + j) * 256 + (q - off[j])];
+        const int b = start[c], e = start[c + 1];
+        for (int i = b + 1; i < e; ++i) {
+            const int v = perm[i];
+            int k = i - 1;
+            while (k >= b && perm[k] > v) {
+                perm[k + 1] = perm[k];
+                --k;
+            }
+            perm[k + 1] = v;
         }
-        perm[j + 1] = v;
     }
 }
 
@@ -495,7 +524,7 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
         }
         if (exclusive_scan(count, ncells, start, plan->scan_tmp, np, s)) return PPMG_ERROR_FATAL;
         if (!sorted) {
-            hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 256)), dim3(256), 0, s, key, slot, start, np,
+            hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 1024)), dim3(256), 0, s, key, slot, start, np,
                                plan->perm.as<int>());
             // the key pass listed the cells with two or more particles, unless a
             // wrong prediction sent it down the cell-order path
@@ -503,8 +532,8 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
                 hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, start, ncells,
                                    plan->perm.as<int>());
             else
-                hipLaunchKernelGGL(k_stabilize_listed, dim3(nblk(np, 256)), dim3(256), 0, s, start, multi, multi_n,
-                                   plan->perm.as<int>());
+                hipLaunchKernelGGL(k_stabilize_listed, dim3(nblk(nblk(np, 256), STAB_GROUP)), dim3(64), 0, s, start,
+                                   multi, multi_n, (int64_t)nblk(np, 256), plan->perm.as<int>());
         }
     }
     PPMG_HIP(hipGetLastError());
@@ -594,7 +623,7 @@ int rank_particles(ppmg_plan* plan, const RankGeom& g, int prec, int64_t np, con
     int64_t inside = np - stats[0];
     if (exclusive_scan(plan->count.as<int>(), ncells, lhbx, plan->scan_tmp, inside, s)) return PPMG_ERROR_FATAL;
     if (np > 0) {
-        hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 256)), dim3(256), 0, s, plan->key.as<int>(),
+        hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 1024)), dim3(256), 0, s, plan->key.as<int>(),
                            plan->slot.as<int>(), lhbx, np, lpdx);
         hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, lhbx, ncells, lpdx);
         hipLaunchKernelGGL(k_to_one_based, dim3(nblk(inside, 256)), dim3(256), 0, s, lpdx, inside);
