@@ -105,6 +105,25 @@ def build_fortran(verbose=True):
     return lib
 
 
+def build_c_client(verbose=True):
+    """Compile the C client of include/ppmg.h (tests/c_client) with gcc
+    against the built library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "c_client", "ppmg_c_check.c")
+    if not os.path.exists(src):
+        return None
+    exe = os.path.join(root, "tests", "c_client", "ppmg_c_check")
+    cmd = ["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", "-I", os.path.join(root, "include"), src,
+           "-L", HERE, "-lppmg", "-lm", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../ppm_legacy_amd",
+           "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0 or r.stderr.strip():
+        raise RuntimeError(f"C client build failed or warned:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {exe}")
+    return exe
+
+
 if __name__ == "__main__":
     build()
     if "--fortran" in sys.argv:
