@@ -279,6 +279,52 @@ def test_p2m_tile_heights_components(core, rows, lda, monkeypatch):
     assert _rel(f.cpu().numpy(), gen.cpu().numpy()) < 1e-13
 
 
+@pytest.mark.parametrize("prec", [F64, F32C])
+def test_tiled_paths_noncubic_box(core, oracle, prec, monkeypatch):
+    """The three tiled kernels on a non-cubic mesh over a non-unit box: 40 x 29
+    x 18 cells on [0,2] x [-1,0.5] x [0.25,1] (a different cell count, spacing
+    and origin per axis, ragged tiles in x and y, a z extent under one chunk).
+    P2M and M2P against the oracle, the adjoint against the oracle and the
+    general per-particle path."""
+    Ns = (40, 29, 18)
+    mn, mx = (0.0, -1.0, 0.25), (2.0, 0.5, 1.0)
+    lda = 3
+    rng = np.random.default_rng(77)
+    def box(n):
+        return np.stack([mn[d] + (mx[d] - mn[d]) * rng.uniform(0.0, 1.0, n) for d in range(3)], 1)
+    xp, xq = box(12000), box(5000)
+    up = rng.normal(size=(len(xp), lda))
+    t = oracle.topo(3, [n + 1 for n in Ns], mn, mx)
+    info, ref = oracle.p2m(t, MP4, xp, up, [1, 1, 1])
+    assert info == 0
+    info, uref = oracle.m2p(t, MP4, xq, ref, [1, 1, 1])
+    assert info == 0
+    mesh = core.Mesh(3, [n + 1 for n in Ns], mn, mx)
+    vdt = _val_dtype(prec)
+    tol = 1e-12 if prec == F64 else 1e-6
+    x, u = _dev(xp, _pos_dtype(prec)), _dev(up, vdt)
+    field = torch.empty(mesh.compact_shape(lda), dtype=vdt, device="cuda")
+    assert field.shape == (18, 29, 40, lda)
+    plan = core.p2m(mesh, x, u, field, [0] * 3, MP4, prec)
+    assert _rel(field.double().cpu().numpy(), oracle.unique_nodes(ref, [1, 1, 1], 3)) < tol
+    out = torch.empty((len(xq), lda), dtype=vdt, device="cuda")
+    core.m2p(mesh, _dev(xq, _pos_dtype(prec)), field, [0] * 3, out, MP4, prec)
+    assert _rel(out.double().cpu().numpy(), uref) < tol
+    gcot = rng.normal(size=field.shape)
+    info, gu_ref, gx_ref = oracle.p2m_adjoint(t, MP4, xp, up, gcot)
+    assert info == 0
+    gc = _dev(gcot, vdt)
+    gu, gx = core.p2m_adjoint(mesh, x, u, gc, [0] * 3, plan, MP4, prec)
+    tol_adj = 1e-12 if prec == F64 else 2e-6
+    assert _rel(gu.double().cpu().numpy(), gu_ref) < tol_adj
+    assert _rel(gx.double().cpu().numpy(), gx_ref) < tol_adj
+    monkeypatch.setenv("PPMG_GENERAL", "1")
+    gu2, gx2 = core.p2m_adjoint(mesh, x, u, gc, [0] * 3, plan, MP4, prec)
+    monkeypatch.delenv("PPMG_GENERAL")
+    assert _rel(gu.double().cpu().numpy(), gu2.double().cpu().numpy()) < (1e-13 if prec == F64 else 1e-6)
+    assert _rel(gx.double().cpu().numpy(), gx2.double().cpu().numpy()) < (1e-13 if prec == F64 else 1e-6)
+
+
 def test_p2m_nonperiodic_axes(core, oracle):
     N = 10
     xp, up = _case(3, N, 800, 11, 2)
