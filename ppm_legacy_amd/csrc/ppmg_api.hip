@@ -282,7 +282,7 @@ int ppmg_p2m(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     const char* gen = std::getenv("PPMG_GENERAL");
     const bool force_general = gen && gen[0] && gen[0] != '0';
     if (!force_general && p2m3d_fast_applicable(m, kernel, prec, fg)) {
-        if (p2m3d_fast(plan, m, prec, np, xp, lda, up, field->data, s)) return g_status.info;
+        if (p2m3d_fast(plan, m, prec, np, xp, fg, up, field->data, s)) return g_status.info;
     } else if (p2m_general(plan, m, kernel, prec, np, xp, lda, up, fg, field->data, nullptr, s)) {
         return g_status.info;
     }
@@ -317,7 +317,7 @@ int ppmg_m2p(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     const char* gen = std::getenv("PPMG_GENERAL");
     const bool force_general = gen && gen[0] && gen[0] != '0';
     if (!force_general && m2p3d_fast_applicable(m, kernel, fg)) {
-        int rc = m2p3d_fast(plan, m, prec, np, xp, lda, field->data, up, s);
+        int rc = m2p3d_fast(plan, m, prec, np, xp, fg, field->data, up, s);
         if (rc < 0) return g_status.info;
         if (rc == 0) {
             mark(plan, s);

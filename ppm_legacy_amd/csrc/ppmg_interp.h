@@ -18,20 +18,24 @@ int create_part_general(ppmg_plan* plan, const DevMesh& m, int prec, const Field
                         const double* cutoff, const double* cw, int resetpos, int64_t maxnp, int64_t* np_out,
                         void* xp, void* up, const FieldGeom* fw, const void* wfield, void* wp, hipStream_t s);
 
-// 3D M'4 tiled deposit (ppmg_p2m3d.hip): fully periodic mesh, compact layout.
+// 3D M'4 tiled deposit (ppmg_p2m3d.hip): fully periodic mesh; per axis the
+// compact layout or a stored box holding the whole real range 0..N.
 bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg);
-int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
-               void* field, hipStream_t s);
+int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
+               const void* up, void* field, hipStream_t s);
 
-// 3D M'4 tiled gather (ppmg_m2p3d.hip): bins the particles into the plan first;
-// returns 1 without touching up when a particle lies outside the sub.
-// Tiled 3D M'4 adjoint (ppmg_adj3d.hip), on the forward's binning.
+// 3D M'4 tiled gather (ppmg_m2p3d.hip), same layouts as the deposit: bins the
+// particles into the plan first; returns 1 without touching up when a particle
+// lies outside the sub.
+bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
+int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
+               const void* field, void* up, hipStream_t s);
+
+// Tiled 3D M'4 adjoint (ppmg_adj3d.hip), on the forward's binning; compact
+// layout only.
 bool adjoint3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
 int adjoint3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
                    const void* g, void* gu, void* gx, hipStream_t s);
-bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
-int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* field,
-               void* up, hipStream_t s);
 
 // Ghost mapping of a padded field (ppmg_ghost.hip).  A block is the part of
 // one periodic image of the real nodes that the stored box holds: src is its

@@ -45,10 +45,22 @@ constexpr int M3_THREADS = 16 * M3_TY;  // 256
 // constraint).  FP64 needs 150 VGPRs at lda 3 and runs 3 wavefronts per SIMD;
 // the FP32 ring (FP32c and FP32 forms) is allowed 3 and comes out at 126-132
 // VGPRs (FP32c: 4 wavefronts), with no spills.
+// Stored node read for node v of a periodic axis with N unique nodes: the
+// compact layout wraps; a stored box holding node 0 and node N reads with
+// ghost_get's node map (ppm_map_field_ghost_get.f:257-280: node 0 and the
+// ghosts below it take node v+N, node N and the ghosts above it v-N), as the
+// general path does.  Stencils reach -1..N+1; the ring of a tile wider than
+// the mesh also loads columns beyond 2N that no stencil reads, which wrap.
+__device__ __forceinline__ int stored_node(int v, int N, int swap) {
+    if (swap && v > -N && v < 2 * N) return v <= 0 ? v + N : (v >= N ? v - N : v);
+    v %= N;
+    return v < 0 ? v + N : v;
+}
+
 template <int LDA, typename PosT, typename ValT, typename AccT, int WPE>
 __global__ void __launch_bounds__(M3_THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
-          const ValT* __restrict__ field, ValT* __restrict__ up, int lda, int c0, int tz) {
+          const ValT* __restrict__ field, FieldGeom fg, ValT* __restrict__ up, int lda, int c0, int tz) {
     __shared__ ValT ring[4 * LDA * M3_PL];   // [plane][c][ry][nx]
     const int tid = threadIdx.x;
     const int ex = tid & 15, row = tid >> 4;
@@ -74,16 +86,13 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         const int c = i % LDA;
         const int node = i / LDA;
         const int nx = node & 15, ry = node >> 4;
-        int gx = (tx0 - 1 + nx) % Nx;
-        if (gx < 0) gx += Nx;
-        int gy = (ty0 - 1 + ry) % Ny;
-        if (gy < 0) gy += Ny;
-        soff[j] = (unsigned)(((gy * Nx + gx) * lda + c0 + c) * (int)sizeof(ValT));
+        const int gx = stored_node(tx0 - 1 + nx, Nx, fg.swap[0]) - fg.lo[0];
+        const int gy = stored_node(ty0 - 1 + ry, Ny, fg.swap[1]) - fg.lo[1];
+        soff[j] = (unsigned)(((gy * fg.ext[0] + gx) * lda + c0 + c) * (int)sizeof(ValT));
     }
     auto fetch_plane = [&](int n) {
-        int nz = n % Nz;
-        if (nz < 0) nz += Nz;
-        const char* base = (const char*)(field + (int64_t)nz * plane * lda);
+        const int nz = stored_node(n, Nz, fg.swap[2]) - fg.lo[2];
+        const char* base = (const char*)(field + (int64_t)nz * fg.ext[0] * fg.ext[1] * lda);
         // component fastest in the field (coalesced), component-major in the ring
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
@@ -209,21 +218,25 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     }
 }
 
+// Per axis the compact layout or a stored box holding the whole real range
+// 0..N (read with the ghost_get node map).
 bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
     if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return false;
     for (int d = 0; d < 3; ++d) {
         if (!m.periodic[d] || m.N[d] < 4) return false;
-        if (fg.lo[d] != 0 || fg.ext[d] != m.N[d]) return false;
+        const bool compact = fg.lo[d] == 0 && fg.ext[d] == m.N[d];
+        const bool boxed = fg.lo[d] <= 0 && fg.lo[d] + fg.ext[d] - 1 >= m.N[d];
+        if (!compact && !boxed) return false;
     }
-    // the ring fetch addresses a field plane with 32-bit byte offsets
-    return (int64_t)m.N[0] * m.N[1] * fg.lda * 8 < ((int64_t)1 << 31);
+    // the ring fetch addresses a stored plane with 32-bit byte offsets
+    return (int64_t)fg.ext[0] * fg.ext[1] * fg.lda * 8 < ((int64_t)1 << 31);
 }
 
 // One launch of the gather for components c0 .. c0+LL-1; the z chunk comes
 // from the resident workgroup count (each chunk loads 3 extra node planes).
 template <int LL, typename PosT, typename ValT, typename AccT>
-static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda, int c0, const ValT* field, ValT* up,
-                      hipStream_t s) {
+static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
+                      const ValT* field, ValT* up, hipStream_t s) {
     auto kernel = k_m3_tile<LL, PosT, ValT, AccT, sizeof(ValT) == 4 ? 3 : 1>;
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(kernel, M3_THREADS));
@@ -231,15 +244,16 @@ static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda
     const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 1);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
     hipLaunchKernelGGL(kernel, grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
-                       tile_perm(plan), xp, field, up, lda, c0, tz);
+                       tile_perm(plan), xp, field, fg, up, fg.lda, c0, tz);
 }
 
 template <typename PosT, typename ValT, typename AccT>
-static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda, const ValT* field, ValT* up,
-                       hipStream_t s) {
+static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, const ValT* field,
+                       ValT* up, hipStream_t s) {
+    const int lda = fg.lda;
     for (int c0 = 0; c0 < lda; c0 += 4) {
         int L = lda - c0 < 4 ? lda - c0 : 4;
-#define TL(LL) launch_m3<LL, PosT, ValT, AccT>(plan, m, xp, lda, c0, field, up, s)
+#define TL(LL) launch_m3<LL, PosT, ValT, AccT>(plan, m, xp, fg, c0, field, up, s)
         if (L == 1) TL(1);
         else if (L == 2) TL(2);
         else if (L == 3) TL(3);
@@ -258,20 +272,20 @@ static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int ld
 // issues at the FP32 rate, so the FP32-compensated form keeps FP32 only where
 // it saves bytes (field in HBM and LDS, output) and sums its 64 products in
 // FP64 registers -- more accurate than compensated FP32 at no extra cost.
-int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* field,
-               void* up, hipStream_t s) {
+int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
+               const void* field, void* up, hipStream_t s) {
     int64_t nbad = 0;
     if (bin_particles(plan, m, prec, np, xp, s, &nbad)) return PPMG_ERROR_FATAL;
     if (nbad > 0) return 1;
     mark(plan, s);
     if (np <= 0) return 0;
     if (prec == PPMG_PREC_F64)
-        return m2p3d_typed<double, double, double>(plan, m, (const double*)xp, lda, (const double*)field,
+        return m2p3d_typed<double, double, double>(plan, m, (const double*)xp, fg, (const double*)field,
                                                    (double*)up, s);
     if (prec == PPMG_PREC_F32C)
-        return m2p3d_typed<double, float, double>(plan, m, (const double*)xp, lda, (const float*)field, (float*)up,
+        return m2p3d_typed<double, float, double>(plan, m, (const double*)xp, fg, (const float*)field, (float*)up,
                                                   s);
-    return m2p3d_typed<float, float, double>(plan, m, (const float*)xp, lda, (const float*)field, (float*)up, s);
+    return m2p3d_typed<float, float, double>(plan, m, (const float*)xp, fg, (const float*)field, (float*)up, s);
 }
 
 }  // namespace ppmg

@@ -89,7 +89,8 @@ __device__ __forceinline__ void shift_in(const ValT (&wx)[4], const ValT (&wz)[4
 template <int LDA, typename PosT, typename ValT, int TY, int NW>
 __global__ void __launch_bounds__(64 * NW)
 k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
-          const ValT* __restrict__ up, int lda, int c0, int tz, ValT* __restrict__ field) {
+          const ValT* __restrict__ up, FieldGeom fg, int c0, int tz, ValT* __restrict__ field) {
+    const int lda = fg.lda;
     using A = LdsAcc<ValT>;
     using slot_t = typename A::slot_t;
     constexpr int NT = 64 * NW;
@@ -288,8 +289,8 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         slot_t* pl = ring + (n & 3) * PLANE;
         if (n >= z0 && n < z1) {
             // thread (wx, wc, wy0) writes node wx, component wc of rows wy0,
-            // wy0 + WRG, ...  A field plane fits in 2^31 bytes (checked at
-            // launch): 32-bit byte offsets.  The slots are recomputed here
+            // wy0 + WRG, ... at its place in the stored box.  A stored plane
+            // fits in 2^31 bytes (checked at launch): 32-bit byte offsets.  The slots are recomputed here
             // (the barrier keeps them from being hoisted and held in registers
             // through the rounds).
             int t = tid;
@@ -297,10 +298,12 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             const int wy0 = t / WROW, wx = (t % WROW) / LDA, wc = t % LDA;
             const bool wact = wy0 < WRG && wx < nxo;
             const int wlds = wc * P3_PL + wx + P3_TT * wy0;
+            const int ex0 = fg.ext[0];
             const unsigned wbyte =
-                (unsigned)(((ty0 + wy0) * Nx + tx0 + wx) * lda + c0 + wc) * (unsigned)sizeof(ValT);
-            const unsigned wstep = (unsigned)(WRG * Nx * lda) * (unsigned)sizeof(ValT);
-            const char* out = (const char*)(field + (int64_t)n * plane_cells * lda);
+                (unsigned)(((ty0 + wy0 - fg.lo[1]) * ex0 + tx0 + wx - fg.lo[0]) * lda + c0 + wc) *
+                (unsigned)sizeof(ValT);
+            const unsigned wstep = (unsigned)(WRG * ex0 * lda) * (unsigned)sizeof(ValT);
+            const char* out = (const char*)(field + (int64_t)(n - fg.lo[2]) * ex0 * fg.ext[1] * lda);
             if (wact) {
 #pragma unroll
                 for (int k = 0; k < (TY + WRG - 1) / WRG; ++k) {
@@ -319,15 +322,20 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     }
 }
 
+// Per axis the stored box is either the compact layout (the N unique nodes,
+// lo = 0, ext = N) or holds the whole real range 0..N (lo <= 0 <= N <=
+// lo+ext-1: the reference's 1-g..Nm+g box is lo = -g, ext = N+1+2g).
 bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg) {
     if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return false;
     for (int d = 0; d < 3; ++d) {
         if (!m.periodic[d] || m.N[d] < 4) return false;
-        if (fg.lo[d] != 0 || fg.ext[d] != m.N[d]) return false;
+        const bool compact = fg.lo[d] == 0 && fg.ext[d] == m.N[d];
+        const bool boxed = fg.lo[d] <= 0 && fg.lo[d] + fg.ext[d] - 1 >= m.N[d];
+        if (!compact && !boxed) return false;
     }
-    // the write-out addresses a field plane with 32-bit byte offsets
+    // the write-out addresses a stored plane with 32-bit byte offsets
     const int64_t vsize = prec == PPMG_PREC_F64 ? 8 : 4;
-    return (int64_t)m.N[0] * m.N[1] * fg.lda * vsize < ((int64_t)1 << 31);
+    return (int64_t)fg.ext[0] * fg.ext[1] * fg.lda * vsize < ((int64_t)1 << 31);
 }
 
 // One launch of the deposit for components c0 .. c0+LL-1 with tiles of
@@ -335,15 +343,15 @@ bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGe
 // device's resident workgroup count (queried once per instantiation): each
 // chunk marches 3 extra cell planes.
 template <int LL, typename PosT, typename ValT, int TY, int NW>
-static void launch_p3_shape(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, int lda, int c0,
-                            ValT* field, hipStream_t s) {
+static void launch_p3_shape(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const FieldGeom& fg,
+                            int c0, ValT* field, hipStream_t s) {
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT, TY, NW>, 64 * NW));
     const int tx = (m.N[0] + P3_TT - 1) / P3_TT, ty = (m.N[1] + TY - 1) / TY;
     const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 3);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
     hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT, TY, NW>), grid, dim3(64 * NW), 0, s, m, plan->start.as<int>(),
-                       tile_perm(plan), xp, up, lda, c0, tz, field);
+                       tile_perm(plan), xp, up, fg, c0, tz, field);
 }
 
 // Tile height: PPMG_P2M_ROWS (9, 13, 21 or 29 node rows: 1, 2, 3 or 4
@@ -358,24 +366,132 @@ static int p3_rows() {
 }
 
 template <int LL, typename PosT, typename ValT>
-static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, int lda, int c0, ValT* field,
+static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const FieldGeom& fg, int c0,
+                      ValT* field,
                       hipStream_t s) {
     switch (p3_rows()) {
-        case 9: launch_p3_shape<LL, PosT, ValT, 9, 1>(plan, m, xp, up, lda, c0, field, s); break;
-        case 13: launch_p3_shape<LL, PosT, ValT, 13, 2>(plan, m, xp, up, lda, c0, field, s); break;
-        case 29: launch_p3_shape<LL, PosT, ValT, 29, 4>(plan, m, xp, up, lda, c0, field, s); break;
-        default: launch_p3_shape<LL, PosT, ValT, 21, 3>(plan, m, xp, up, lda, c0, field, s); break;
+        case 9: launch_p3_shape<LL, PosT, ValT, 9, 1>(plan, m, xp, up, fg, c0, field, s); break;
+        case 13: launch_p3_shape<LL, PosT, ValT, 13, 2>(plan, m, xp, up, fg, c0, field, s); break;
+        case 29: launch_p3_shape<LL, PosT, ValT, 29, 4>(plan, m, xp, up, fg, c0, field, s); break;
+        default: launch_p3_shape<LL, PosT, ValT, 21, 3>(plan, m, xp, up, fg, c0, field, s); break;
     }
 }
 
+// Stored nodes besides the unique ones, for axes whose stored box holds the
+// whole real range 0..N.  Per axis: R = stored real nodes (N compact, N+1
+// boxed), O = ext - R stored nodes outside 0..N.
+
+// Real nodes with index N on a boxed axis: the fold leaves node N equal to
+// node 0, so they take the value the tiles wrote there.  Enumerated as the
+// face i = N, then j = N (i < N), then k = N (i, j < N).
+template <typename ValT>
+__global__ void __launch_bounds__(256) k_p3_dup(FieldGeom fg, int3 n, int3 r, int3 b, ValT* __restrict__ field,
+                                                int64_t ndup) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ndup) return;
+    const int64_t fx = b.x ? (int64_t)r.y * r.z : 0;
+    const int64_t fy = b.y ? (int64_t)(r.x - b.x) * r.z : 0;
+    int i, j, k;
+    if (t < fx) {
+        i = n.x;
+        j = (int)(t % r.y);
+        k = (int)(t / r.y);
+    } else if (t < fx + fy) {
+        const int64_t q = t - fx;
+        i = (int)(q % (r.x - b.x));
+        j = n.y;
+        k = (int)(q / (r.x - b.x));
+    } else {
+        const int64_t q = t - fx - fy;
+        i = (int)(q % (r.x - b.x));
+        j = (int)((q / (r.x - b.x)) % (r.y - b.y));
+        k = n.z;
+    }
+    auto at = [&](int x, int y, int z) {
+        return (((int64_t)(z - fg.lo[2]) * fg.ext[1] + (y - fg.lo[1])) * fg.ext[0] + (x - fg.lo[0])) * fg.lda;
+    };
+    const int64_t dst = at(i, j, k);
+    const int64_t src = at(i == n.x ? 0 : i, j == n.y ? 0 : j, k == n.z ? 0 : k);
+    for (int c = 0; c < fg.lda; ++c) field[dst + c] = field[src + c];
+}
+
+// Stored nodes outside 0..N on some axis (ghost layers): the raw sum of the
+// particles whose stencil reaches the node with no periodic wrap on any axis,
+// which is what ppm_map_field_ghost_put leaves in ghost nodes (the general
+// path's rule for nodes that are not real).  Enumerated as the planes outside
+// 0..N in z, then the rows outside in y (z real), then the nodes outside in x
+// (y, z real).  Weights and products in the value precision, sums in FP64.
+template <typename PosT, typename ValT, int LL>
+__global__ void __launch_bounds__(256) k_p3_shell(DevMesh m, FieldGeom fg, int3 r, const int* __restrict__ start,
+                                                  const int* __restrict__ perm, const PosT* __restrict__ xp,
+                                                  const ValT* __restrict__ up, int c0, ValT* __restrict__ field,
+                                                  int64_t nshell) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nshell) return;
+    const int e0 = fg.ext[0], e1 = fg.ext[1], e2 = fg.ext[2];
+    const int o0 = e0 - r.x, o1 = e1 - r.y, o2 = e2 - r.z;
+    // q-th stored node of axis d outside the real range: below 0, then above N
+    auto outer = [&](int d, int q) { return q < -fg.lo[d] ? fg.lo[d] + q : m.N[d] + 1 + (q + fg.lo[d]); };
+    const int64_t ra = (int64_t)o2 * e1 * e0, rb = (int64_t)r.z * o1 * e0;
+    int n[3];
+    if (t < ra) {
+        n[0] = fg.lo[0] + (int)(t % e0);
+        n[1] = fg.lo[1] + (int)((t / e0) % e1);
+        n[2] = outer(2, (int)(t / ((int64_t)e0 * e1)));
+    } else if (t < ra + rb) {
+        const int64_t q = t - ra;
+        n[0] = fg.lo[0] + (int)(q % e0);
+        n[1] = outer(1, (int)((q / e0) % o1));
+        n[2] = (int)(q / ((int64_t)e0 * o1));
+    } else {
+        const int64_t q = t - ra - rb;
+        n[0] = outer(0, (int)(q % o0));
+        n[1] = (int)((q / o0) % r.y);
+        n[2] = (int)(q / ((int64_t)o0 * r.y));
+    }
+    double acc[LL];
+#pragma unroll
+    for (int c = 0; c < LL; ++c) acc[c] = 0.0;
+    for (int oz = 0; oz < 4; ++oz) {
+        const int cz = n[2] + 1 - oz;
+        if (cz < 0 || cz >= m.nbin[2]) continue;
+        for (int oy = 0; oy < 4; ++oy) {
+            const int cy = n[1] + 1 - oy;
+            if (cy < 0 || cy >= m.nbin[1]) continue;
+            for (int ox = 0; ox < 4; ++ox) {
+                const int cx = n[0] + 1 - ox;
+                if (cx < 0 || cx >= m.nbin[0]) continue;
+                const int key = cx + m.nbin[0] * (cy + m.nbin[1] * cz);
+                const int b = start[key], e = start[key + 1];
+                for (int q = b; q < e; ++q) {
+                    const int p = perm ? perm[q] : q;
+                    ValT fr[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        int ci;
+                        fr[d] = (ValT)cell_offset<PosT>(xp[(int64_t)p * 3 + d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
+                    }
+                    const ValT w = (mp4_at<ValT>(fr[0], ox) * mp4_at<ValT>(fr[1], oy)) * mp4_at<ValT>(fr[2], oz);
+#pragma unroll
+                    for (int c = 0; c < LL; ++c) acc[c] += (double)(w * up[(int64_t)p * fg.lda + c0 + c]);
+                }
+            }
+        }
+    }
+    ValT* out = field + (((int64_t)(n[2] - fg.lo[2]) * e1 + (n[1] - fg.lo[1])) * e0 + (n[0] - fg.lo[0])) * fg.lda + c0;
+#pragma unroll
+    for (int c = 0; c < LL; ++c) out[c] = (ValT)acc[c];
+}
+
 template <typename PosT, typename ValT>
-static int p2m3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT* xp, int lda, const ValT* up,
-                       ValT* field, hipStream_t s) {
+static int p2m3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT* xp, const FieldGeom& fg,
+                       const ValT* up, ValT* field, hipStream_t s) {
     (void)np;
     mark(plan, s);
+    const int lda = fg.lda;
     for (int c0 = 0; c0 < lda; c0 += 4) {
         int L = lda - c0 < 4 ? lda - c0 : 4;
-#define TL(LL) launch_p3<LL, PosT, ValT>(plan, m, xp, up, lda, c0, field, s)
+#define TL(LL) launch_p3<LL, PosT, ValT>(plan, m, xp, up, fg, c0, field, s)
         if (L == 1) TL(1);
         else if (L == 2) TL(2);
         else if (L == 3) TL(3);
@@ -383,17 +499,44 @@ static int p2m3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT
 #undef TL
         PPMG_HIP(hipGetLastError());
     }
+    // the rest of a stored box that holds the real range: node N, then ghosts
+    int R[3], B[3];
+    for (int d = 0; d < 3; ++d) {
+        B[d] = !(fg.lo[d] == 0 && fg.ext[d] == m.N[d]);
+        R[d] = B[d] ? m.N[d] + 1 : m.N[d];
+    }
+    const int3 n3 = make_int3(m.N[0], m.N[1], m.N[2]), r3 = make_int3(R[0], R[1], R[2]),
+               b3 = make_int3(B[0], B[1], B[2]);
+    const int64_t ndup = (int64_t)R[0] * R[1] * R[2] - (int64_t)(R[0] - B[0]) * (R[1] - B[1]) * (R[2] - B[2]);
+    if (ndup > 0)
+        hipLaunchKernelGGL(k_p3_dup<ValT>, dim3((unsigned)((ndup + 255) / 256)), dim3(256), 0, s, fg, n3, r3, b3,
+                           field, ndup);
+    const int64_t nshell = (int64_t)fg.ext[0] * fg.ext[1] * fg.ext[2] - (int64_t)R[0] * R[1] * R[2];
+    if (nshell > 0) {
+        const unsigned g = (unsigned)((nshell + 255) / 256);
+        for (int c0 = 0; c0 < lda; c0 += 4) {
+            const int L = lda - c0 < 4 ? lda - c0 : 4;
+#define SH(LL) hipLaunchKernelGGL((k_p3_shell<PosT, ValT, LL>), dim3(g), dim3(256), 0, s, m, fg, r3, \
+                                  plan->start.as<int>(), tile_perm(plan), xp, up, c0, field, nshell)
+            if (L == 1) SH(1);
+            else if (L == 2) SH(2);
+            else if (L == 3) SH(3);
+            else SH(4);
+#undef SH
+        }
+    }
+    PPMG_HIP(hipGetLastError());
     mark(plan, s);
     return 0;
 }
 
-int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
-               void* field, hipStream_t s) {
+int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
+               const void* up, void* field, hipStream_t s) {
     if (prec == PPMG_PREC_F64)
-        return p2m3d_typed<double, double>(plan, m, np, (const double*)xp, lda, (const double*)up, (double*)field, s);
+        return p2m3d_typed<double, double>(plan, m, np, (const double*)xp, fg, (const double*)up, (double*)field, s);
     if (prec == PPMG_PREC_F32C)
-        return p2m3d_typed<double, float>(plan, m, np, (const double*)xp, lda, (const float*)up, (float*)field, s);
-    return p2m3d_typed<float, float>(plan, m, np, (const float*)xp, lda, (const float*)up, (float*)field, s);
+        return p2m3d_typed<double, float>(plan, m, np, (const double*)xp, fg, (const float*)up, (float*)field, s);
+    return p2m3d_typed<float, float>(plan, m, np, (const float*)xp, fg, (const float*)up, (float*)field, s);
 }
 
 }  // namespace ppmg

@@ -188,7 +188,10 @@ def test_p2m_variants(core, oracle, dim, kernel, prec, lda):
     assert _rel(field.cpu().numpy(), ref) < TOL[prec]
 
 
-def test_p2m_compact_equals_padded_and_is_deterministic(core):
+def test_p2m_compact_equals_padded_and_is_deterministic(core, monkeypatch):
+    """Compact and padded (g = 1) layouts both run the tiled deposit; the
+    padded one adds node N (equal to node 0 bitwise) and the raw ghost shell.
+    Both against the general node gather, and bitwise reproducible."""
     N = 16
     xp, up = _case(3, N, 4000, 7, 3)
     mesh = core.Mesh(3, N + 1)
@@ -199,11 +202,17 @@ def test_p2m_compact_equals_padded_and_is_deterministic(core):
     core.p2m(mesh, x, u, compact, [0] * 3)
     again = torch.zeros_like(compact)
     core.p2m(mesh, x, u, again, [0] * 3)
-    # compact runs the tiled 3D M'4 kernel, padded the general node gather
-    assert _rel(compact.cpu().numpy(), padded[1:1 + N, 1:1 + N, 1:1 + N].cpu().numpy()) < 1e-13
+    assert torch.equal(padded[1:1 + N, 1:1 + N, 1:1 + N], compact)
+    assert torch.equal(padded[1:1 + N, 1:1 + N, 1 + N], padded[1:1 + N, 1:1 + N, 1])   # node N = node 0
+    assert torch.equal(padded[1 + N, 1:2 + N, 1:2 + N], padded[1, 1:2 + N, 1:2 + N])
     assert torch.equal(compact, again)                      # bitwise reproducible
     ref = bf.p2m_periodic(xp, up, N)
     assert _rel(compact.cpu().numpy(), ref) < 1e-12
+    monkeypatch.setenv("PPMG_GENERAL", "1")
+    gen = torch.zeros_like(padded)
+    core.p2m(mesh, x, u, gen, [-1] * 3)
+    monkeypatch.delenv("PPMG_GENERAL")
+    assert _rel(padded.cpu().numpy(), gen.cpu().numpy()) < 1e-13   # ghosts included
 
 
 @pytest.mark.parametrize("prec", [F64, F32C, F32])
@@ -277,6 +286,38 @@ def test_p2m_tile_heights_components(core, rows, lda, monkeypatch):
     gen = torch.empty_like(f)
     core.p2m(mesh, x, u, gen, [0] * 3, MP4, F64)
     assert _rel(f.cpu().numpy(), gen.cpu().numpy()) < 1e-13
+
+
+@pytest.mark.parametrize("N", [5, 12, 40])
+@pytest.mark.parametrize("g", [0, 1, 2])
+def test_tiled_paths_padded_layout(core, N, g, monkeypatch):
+    """The tiled kernels on the reference's stored box (lo = -g, ext = N+1+2g)
+    against the general paths: P2M on every stored node (node N equal to node
+    0, ghost layers the raw unwrapped sums), and M2P of a random stored field
+    whose node N, node 0 and ghost layers all differ, which the gather reads
+    with ghost_get's node map.  N = 5 and 12 give tiles wider than the mesh."""
+    rng = np.random.default_rng(1000 * N + g)
+    lda = 2
+    xp = rng.uniform(0.0, 1.0, size=(max(8 * N ** 3, 2000), 3))
+    up = rng.normal(size=(len(xp), lda))
+    mesh = core.Mesh(3, N + 1)
+    lo = [-g] * 3
+    x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
+    fast = torch.full(mesh.padded_shape(lda, [g] * 3), 7.0, dtype=torch.float64, device="cuda")
+    core.p2m(mesh, x, u, fast, lo)
+    field = torch.from_numpy(rng.normal(size=mesh.padded_shape(lda, [g] * 3))).cuda()
+    out = torch.empty((len(xp), lda), dtype=torch.float64, device="cuda")
+    core.m2p(mesh, x, field, lo, out)
+    monkeypatch.setenv("PPMG_GENERAL", "1")
+    gen = torch.full_like(fast, 7.0)
+    core.p2m(mesh, x, u, gen, lo)
+    out_gen = torch.empty_like(out)
+    core.m2p(mesh, x, field, lo, out_gen)
+    monkeypatch.delenv("PPMG_GENERAL")
+    assert _rel(fast.cpu().numpy(), gen.cpu().numpy()) < 1e-13
+    s = slice(g, g + N)
+    assert torch.equal(fast[s, s, g + N], fast[s, s, g])          # node N = node 0 (x)
+    assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < 1e-13
 
 
 @pytest.mark.parametrize("prec", [F64, F32C])
