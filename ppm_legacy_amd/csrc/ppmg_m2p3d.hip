@@ -45,14 +45,17 @@ constexpr int M3_THREADS = 16 * M3_TY;  // 256
 // constraint).  FP64 needs 150 VGPRs at lda 3 and runs 3 wavefronts per SIMD;
 // the FP32 ring (FP32c and FP32 forms) is allowed 3 and comes out at 126-132
 // VGPRs (FP32c: 4 wavefronts), with no spills.
-// Stored node read for node v of a periodic axis with N unique nodes: the
-// compact layout wraps; a stored box holding node 0 and node N reads with
-// ghost_get's node map (ppm_map_field_ghost_get.f:257-280: node 0 and the
-// ghosts below it take node v+N, node N and the ghosts above it v-N), as the
-// general path does.  Stencils reach -1..N+1; the ring of a tile wider than
-// the mesh also loads columns beyond 2N that no stencil reads, which wrap.
-__device__ __forceinline__ int stored_node(int v, int N, int swap) {
-    if (swap && v > -N && v < 2 * N) return v <= 0 ? v + N : (v >= N ? v - N : v);
+// Stored node read for node v along axis d, as the general path reads it:
+// on a periodic axis the compact layout wraps, and a stored box holding node
+// 0 and node N reads with ghost_get's node map (ppm_map_field_ghost_get.f:
+// 257-280: node 0 and the ghosts below it take node v+N, node N and the ghosts
+// above it v-N); an open axis reads node v itself.  Stencils reach -1..N+2;
+// the ring of a tile also loads columns beyond that which no stencil reads:
+// they wrap (periodic) or are clamped into the box (open).
+__device__ __forceinline__ int stored_node(const DevMesh& m, const FieldGeom& fg, int d, int v) {
+    const int N = m.N[d];
+    if (!m.periodic[d]) return min(max(v, fg.lo[d]), fg.lo[d] + fg.ext[d] - 1);
+    if (fg.swap[d] && v > -N && v < 2 * N) return v <= 0 ? v + N : (v >= N ? v - N : v);
     v %= N;
     return v < 0 ? v + N : v;
 }
@@ -64,7 +67,8 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     __shared__ ValT ring[4 * LDA * M3_PL];   // [plane][c][ry][nx]
     const int tid = threadIdx.x;
     const int ex = tid & 15, row = tid >> 4;
-    const int Nx = m.N[0], Ny = m.N[1], Nz = m.N[2];
+    // tiles cover the cells: N per periodic axis, N+1 (0..N) per open one
+    const int Nx = m.nbin[0], Ny = m.nbin[1], Nz = m.nbin[2];
     const int ntx = (Nx + M3_TX - 1) / M3_TX, nty = (Ny + M3_TY - 1) / M3_TY;
     const int64_t tile = xcd_tile(blockIdx.x, (int64_t)gridDim.x);
     const int tx0 = (int)(tile % ntx) * M3_TX, ty0 = (int)((tile / ntx) % nty) * M3_TY;
@@ -86,12 +90,12 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         const int c = i % LDA;
         const int node = i / LDA;
         const int nx = node & 15, ry = node >> 4;
-        const int gx = stored_node(tx0 - 1 + nx, Nx, fg.swap[0]) - fg.lo[0];
-        const int gy = stored_node(ty0 - 1 + ry, Ny, fg.swap[1]) - fg.lo[1];
+        const int gx = stored_node(m, fg, 0, tx0 - 1 + nx) - fg.lo[0];
+        const int gy = stored_node(m, fg, 1, ty0 - 1 + ry) - fg.lo[1];
         soff[j] = (unsigned)(((gy * fg.ext[0] + gx) * lda + c0 + c) * (int)sizeof(ValT));
     }
     auto fetch_plane = [&](int n) {
-        const int nz = stored_node(n, Nz, fg.swap[2]) - fg.lo[2];
+        const int nz = stored_node(m, fg, 2, n) - fg.lo[2];
         const char* base = (const char*)(field + (int64_t)nz * fg.ext[0] * fg.ext[1] * lda);
         // component fastest in the field (coalesced), component-major in the ring
 #pragma unroll
@@ -218,15 +222,21 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     }
 }
 
-// Per axis the compact layout or a stored box holding the whole real range
-// 0..N (read with the ghost_get node map).
+// Per periodic axis the compact layout or a stored box holding the whole
+// real range 0..N (read with the ghost_get node map); per open axis a box
+// holding every node a stencil reaches, -1..N+2 (ghost size 2 or more).
 bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
     if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return false;
     for (int d = 0; d < 3; ++d) {
-        if (!m.periodic[d] || m.N[d] < 4) return false;
-        const bool compact = fg.lo[d] == 0 && fg.ext[d] == m.N[d];
-        const bool boxed = fg.lo[d] <= 0 && fg.lo[d] + fg.ext[d] - 1 >= m.N[d];
-        if (!compact && !boxed) return false;
+        if (m.N[d] < 4) return false;
+        const int hi = fg.lo[d] + fg.ext[d] - 1;
+        if (m.periodic[d]) {
+            const bool compact = fg.lo[d] == 0 && fg.ext[d] == m.N[d];
+            const bool boxed = fg.lo[d] <= 0 && hi >= m.N[d];
+            if (!compact && !boxed) return false;
+        } else if (fg.lo[d] > -1 || hi < m.N[d] + 2) {
+            return false;
+        }
     }
     // the ring fetch addresses a stored plane with 32-bit byte offsets
     return (int64_t)fg.ext[0] * fg.ext[1] * fg.lda * 8 < ((int64_t)1 << 31);
@@ -240,9 +250,9 @@ static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const F
     auto kernel = k_m3_tile<LL, PosT, ValT, AccT, sizeof(ValT) == 4 ? 3 : 1>;
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(kernel, M3_THREADS));
-    const int tx = (m.N[0] + M3_TX - 1) / M3_TX, ty = (m.N[1] + M3_TY - 1) / M3_TY;
-    const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 1);
-    dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
+    const int tx = (m.nbin[0] + M3_TX - 1) / M3_TX, ty = (m.nbin[1] + M3_TY - 1) / M3_TY;
+    const int tz = pick_z_chunk((int64_t)tx * ty, m.nbin[2], slots.load(), 1);
+    dim3 grid((unsigned)((int64_t)tx * ty * ((m.nbin[2] + tz - 1) / tz)));
     hipLaunchKernelGGL(kernel, grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
                        tile_perm(plan), xp, field, fg, up, fg.lda, c0, tz);
 }

@@ -103,32 +103,42 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     __shared__ slot_t ring[4 * PLANE];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int Nx = m.N[0], Ny = m.N[1], Nz = m.N[2];
-    const int ntx = (Nx + P3_TT - 1) / P3_TT, nty = (Ny + TY - 1) / TY;
+    // nodes the tiles produce per axis: the N unique nodes of a periodic axis,
+    // the N+1 real nodes 0..N of an open one
+    const int Rx = m.N[0] + !m.periodic[0], Ry = m.N[1] + !m.periodic[1], Rz = m.N[2] + !m.periodic[2];
+    const int ntx = (Rx + P3_TT - 1) / P3_TT, nty = (Ry + TY - 1) / TY;
     const int64_t tile = xcd_tile(blockIdx.x, (int64_t)gridDim.x);
     const int tx0 = (int)(tile % ntx) * P3_TT, ty0 = (int)((tile / ntx) % nty) * TY;
     const int z0 = (int)(tile / ((int64_t)ntx * nty)) * tz;
-    const int z1 = min(z0 + tz, Nz);
-    const int nxo = min(P3_TT, Nx - tx0), nyo = min(TY, Ny - ty0);
+    const int z1 = min(z0 + tz, Rz);
+    const int nxo = min(P3_TT, Rx - tx0), nyo = min(TY, Ry - ty0);
     for (int i = tid; i < 4 * PLANE; i += NT) A::zero(ring[i]);
 
     const int ex = lane & 15, erow = lane >> 4;
     const int lx = ex - 3;                          // node this lane owns after the row sum
     const bool xown = lx >= 0 && lx < nxo;
-    int cx = tx0 - 2 + ex;
-    cx %= Nx; if (cx < 0) cx += Nx;
+    // cell c of the window along axis d: a periodic axis wraps, an open one
+    // has no cell outside 0..nbin-1 (-1)
+    auto wcell = [&](int c, int d) {
+        const int nb = m.nbin[d];
+        if (m.periodic[d]) {
+            c %= nb;
+            return c < 0 ? c + nb : c;
+        }
+        return c >= 0 && c < nb ? c : -1;
+    };
+    const int cx = wcell(tx0 - 2 + ex, 0);
     int keyxy[NC], ey4[NC];
 #pragma unroll
     for (int j = 0; j < NC; ++j) {
         const int q = NW == 1 ? j : 2 * wave + j;
         int ey = q * 4 + erow;
         ey4[j] = ey;
-        int cy = ty0 - 2 + ey;
-        cy %= Ny; if (cy < 0) cy += Ny;
-        keyxy[j] = cx + Nx * cy;
+        const int cy = wcell(ty0 - 2 + ey, 1);
+        keyxy[j] = cx < 0 || cy < 0 ? -1 : cx + m.nbin[0] * cy;
     }
     __syncthreads();
-    const int64_t plane_cells = (int64_t)Nx * Ny;
+    const int64_t plane_cells = (int64_t)m.nbin[0] * m.nbin[1];
     constexpr int WROW = P3_TT * LDA;              // values of one node row
     constexpr int WRG = NT / WROW;                 // node rows written at once
     static_assert(WRG >= 1, "a workgroup writes at least one node row at once");
@@ -138,9 +148,8 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     auto heads = [&](int cz, int j, int& f, int& n) {
         f = 0;
         n = 0;
-        if (cz <= z1) {
-            int czw = cz % Nz;
-            if (czw < 0) czw += Nz;
+        const int czw = cz <= z1 ? wcell(cz, 2) : -1;
+        if (czw >= 0 && keyxy[j] >= 0) {
             const int key = (int)(plane_cells * czw + keyxy[j]);
             f = start[key];
             n = start[key + 1];   // end, subtracted where used: the load stays in flight
@@ -322,14 +331,15 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     }
 }
 
-// Per axis the stored box is either the compact layout (the N unique nodes,
-// lo = 0, ext = N) or holds the whole real range 0..N (lo <= 0 <= N <=
-// lo+ext-1: the reference's 1-g..Nm+g box is lo = -g, ext = N+1+2g).
+// Per axis the stored box is either the compact layout of a periodic axis
+// (the N unique nodes, lo = 0, ext = N) or holds the whole real range 0..N
+// (lo <= 0 <= N <= lo+ext-1: the reference's 1-g..Nm+g box is lo = -g,
+// ext = N+1+2g), on periodic and open axes alike.
 bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg) {
     if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return false;
     for (int d = 0; d < 3; ++d) {
-        if (!m.periodic[d] || m.N[d] < 4) return false;
-        const bool compact = fg.lo[d] == 0 && fg.ext[d] == m.N[d];
+        if (m.N[d] < 4) return false;
+        const bool compact = m.periodic[d] && fg.lo[d] == 0 && fg.ext[d] == m.N[d];
         const bool boxed = fg.lo[d] <= 0 && fg.lo[d] + fg.ext[d] - 1 >= m.N[d];
         if (!compact && !boxed) return false;
     }
@@ -347,9 +357,10 @@ static void launch_p3_shape(ppmg_plan* plan, const DevMesh& m, const PosT* xp, c
                             int c0, ValT* field, hipStream_t s) {
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT, TY, NW>, 64 * NW));
-    const int tx = (m.N[0] + P3_TT - 1) / P3_TT, ty = (m.N[1] + TY - 1) / TY;
-    const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 3);
-    dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
+    const int Rx = m.N[0] + !m.periodic[0], Ry = m.N[1] + !m.periodic[1], Rz = m.N[2] + !m.periodic[2];
+    const int tx = (Rx + P3_TT - 1) / P3_TT, ty = (Ry + TY - 1) / TY;
+    const int tz = pick_z_chunk((int64_t)tx * ty, Rz, slots.load(), 3);
+    dim3 grid((unsigned)((int64_t)tx * ty * ((Rz + tz - 1) / tz)));
     hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT, TY, NW>), grid, dim3(64 * NW), 0, s, m, plan->start.as<int>(),
                        tile_perm(plan), xp, up, fg, c0, tz, field);
 }
@@ -381,9 +392,11 @@ static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const V
 // whole real range 0..N.  Per axis: R = stored real nodes (N compact, N+1
 // boxed), O = ext - R stored nodes outside 0..N.
 
-// Real nodes with index N on a boxed axis: the fold leaves node N equal to
-// node 0, so they take the value the tiles wrote there.  Enumerated as the
-// face i = N, then j = N (i < N), then k = N (i, j < N).
+// Real nodes with index N on a boxed periodic axis (n.d = N; -1 on other
+// axes): the fold leaves node N equal to node 0, so they take the value the
+// tiles wrote there.  Enumerated as the face i = N, then j = N (i < N), then
+// k = N (i, j < N), over the real nodes (r per axis, b = 1 where node N is a
+// copy).
 template <typename ValT>
 __global__ void __launch_bounds__(256) k_p3_dup(FieldGeom fg, int3 n, int3 r, int3 b, ValT* __restrict__ field,
                                                 int64_t ndup) {
@@ -499,13 +512,16 @@ static int p2m3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT
 #undef TL
         PPMG_HIP(hipGetLastError());
     }
-    // the rest of a stored box that holds the real range: node N, then ghosts
-    int R[3], B[3];
+    // the rest of a stored box that holds the real range: node N of periodic
+    // axes (open axes' node N is real and came from the tiles), then ghosts
+    int R[3], B[3], D[3];
     for (int d = 0; d < 3; ++d) {
-        B[d] = !(fg.lo[d] == 0 && fg.ext[d] == m.N[d]);
-        R[d] = B[d] ? m.N[d] + 1 : m.N[d];
+        const bool compact = m.periodic[d] && fg.lo[d] == 0 && fg.ext[d] == m.N[d];
+        B[d] = m.periodic[d] && !compact;
+        R[d] = compact ? m.N[d] : m.N[d] + 1;
+        D[d] = B[d] ? m.N[d] : -1;
     }
-    const int3 n3 = make_int3(m.N[0], m.N[1], m.N[2]), r3 = make_int3(R[0], R[1], R[2]),
+    const int3 n3 = make_int3(D[0], D[1], D[2]), r3 = make_int3(R[0], R[1], R[2]),
                b3 = make_int3(B[0], B[1], B[2]);
     const int64_t ndup = (int64_t)R[0] * R[1] * R[2] - (int64_t)(R[0] - B[0]) * (R[1] - B[1]) * (R[2] - B[2]);
     if (ndup > 0)

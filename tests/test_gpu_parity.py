@@ -320,6 +320,53 @@ def test_tiled_paths_padded_layout(core, N, g, monkeypatch):
     assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < 1e-13
 
 
+@pytest.mark.parametrize("bc", [[1, 1, 1, 1, 2, 2], [2, 2, 1, 1, 1, 1], [2, 2, 2, 2, 2, 2]],
+                         ids=["open_z", "open_x", "all_open"])
+@pytest.mark.parametrize("N", [5, 12, 40])
+def test_tiled_paths_open_axes(core, oracle, bc, N, monkeypatch):
+    """Meshes with non-periodic axes on the tiled kernels: on an open axis the
+    tiles make the N+1 real nodes 0..N with no wrap, ghost nodes keep their raw
+    sums, and the gather reads node v itself.  Against the general paths on
+    every stored node (P2M with ghost sizes 0..2, M2P of a random stored field
+    with ghost size 2), and P2M against the oracle.  A tenth of the particles
+    sit exactly on a face of an open axis (cell N holds the upper face)."""
+    rng = np.random.default_rng(7 * N + sum(bc))
+    lda = 3
+    npart = max(6 * N ** 3, 3000)
+    xp = rng.uniform(0.0, 1.0, size=(npart, 3))
+    for d in range(3):
+        if bc[2 * d] != 1:
+            face = rng.uniform(size=npart) < 0.05
+            xp[face, d] = 1.0
+            face = rng.uniform(size=npart) < 0.05
+            xp[face, d] = 0.0
+    up = rng.normal(size=(npart, lda))
+    mesh = core.Mesh(3, N + 1, bcdef=bc)
+    x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
+    t = oracle.topo(3, [N + 1] * 3, bcdef=bc)
+    for g in (0, 1, 2):
+        lo = [-g] * 3
+        fast = torch.full(mesh.padded_shape(lda, [g] * 3), 7.0, dtype=torch.float64, device="cuda")
+        core.p2m(mesh, x, u, fast, lo)
+        monkeypatch.setenv("PPMG_GENERAL", "1")
+        gen = torch.full_like(fast, 7.0)
+        core.p2m(mesh, x, u, gen, lo)
+        monkeypatch.delenv("PPMG_GENERAL")
+        assert _rel(fast.cpu().numpy(), gen.cpu().numpy()) < 1e-13, g
+        if g == 2:
+            info, ref = oracle.p2m(t, MP4, xp, up, [2, 2, 2])
+            assert info == 0
+            assert _rel(fast.cpu().numpy(), ref) < 1e-12
+    field = torch.from_numpy(rng.normal(size=mesh.padded_shape(lda, [2] * 3))).cuda()
+    out = torch.empty((npart, lda), dtype=torch.float64, device="cuda")
+    core.m2p(mesh, x, field, [-2] * 3, out)
+    monkeypatch.setenv("PPMG_GENERAL", "1")
+    out_gen = torch.empty_like(out)
+    core.m2p(mesh, x, field, [-2] * 3, out_gen)
+    monkeypatch.delenv("PPMG_GENERAL")
+    assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < 1e-13
+
+
 @pytest.mark.parametrize("prec", [F64, F32C])
 def test_tiled_paths_noncubic_box(core, oracle, prec, monkeypatch):
     """The three tiled kernels on a non-cubic mesh over a non-unit box: 40 x 29
