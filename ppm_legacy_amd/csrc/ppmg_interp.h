@@ -18,8 +18,8 @@ int create_part_general(ppmg_plan* plan, const DevMesh& m, int prec, const Field
                         const double* cutoff, const double* cw, int resetpos, int64_t maxnp, int64_t* np_out,
                         void* xp, void* up, const FieldGeom* fw, const void* wfield, void* wp, hipStream_t s);
 
-// 3D M'4 tiled deposit (ppmg_p2m3d.hip): fully periodic mesh; per axis the
-// compact layout or a stored box holding the whole real range 0..N.
+// 3D M'4 tiled deposit (ppmg_p2m3d.hip): per axis (periodic or open) the
+// compact layout of a periodic axis or a stored box holding the real range 0..N.
 bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg);
 int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
                const void* up, void* field, hipStream_t s);

@@ -2,9 +2,10 @@
 // with DPP row exchange.
 //
 // Replaces the gather loop of ppm_interp_m2p (ppm_interp_m2p.f:895-1240) for
-// fully periodic meshes stored in the compact layout; the periodic ghost fill
-// the reference gets from ppm_map_field_ghost (ghost_get) is the index wrap of
-// the node planes loaded around the tile.
+// 3D M'4 on periodic and open axes, reading the compact layout or a stored box;
+// the periodic ghost fill the reference gets from ppm_map_field_ghost
+// (ghost_get) is the node map of the planes loaded around the tile
+// (stored_node).
 //
 // A workgroup (4 wavefronts, 16 rows of 16 lanes) owns the particles of a
 // 13 x 16 cell tile in x,y over 32 cell planes in z.  The 16 x 19 nodes those

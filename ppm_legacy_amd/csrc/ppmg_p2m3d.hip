@@ -1,8 +1,10 @@
 // ppmg_p2m3d.hip -- 3D M'4 P2M fast path for gfx950: tiled z-march deposit.
 //
 // Replaces the scatter loop of ppm_interp_p2m (ppm_interp_p2m.f:783-1377,
-// 2540-3055) plus its periodic ghost fold (:3670-3716) for fully periodic
-// meshes written in the compact layout (unique nodes only).
+// 2540-3055) plus its periodic ghost fold (:3670-3716) for 3D M'4 on periodic
+// and open axes, written in the compact layout (the unique nodes of periodic
+// axes) or into a stored box holding the real range (the reference's
+// field_up(1-g:Nm+g)), whose node N and ghost layers two small kernels fill.
 //
 // Work decomposition (three wavefronts per workgroup):
 //   * a workgroup owns a 13 x 21 node tile in x,y and a chunk of node planes in z;
