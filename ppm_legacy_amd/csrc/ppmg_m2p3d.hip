@@ -53,15 +53,16 @@ constexpr int M3_THREADS = 16 * M3_TY;  // 256
 // above it v-N); an open axis reads node v itself.  Stencils reach -1..N+2;
 // the ring of a tile also loads columns beyond that which no stencil reads:
 // they wrap (periodic) or are clamped into the box (open).
+template <bool OPEN>
 __device__ __forceinline__ int stored_node(const DevMesh& m, const FieldGeom& fg, int d, int v) {
     const int N = m.N[d];
-    if (!m.periodic[d]) return min(max(v, fg.lo[d]), fg.lo[d] + fg.ext[d] - 1);
+    if (OPEN && !m.periodic[d]) return min(max(v, fg.lo[d]), fg.lo[d] + fg.ext[d] - 1);
     if (fg.swap[d] && v > -N && v < 2 * N) return v <= 0 ? v + N : (v >= N ? v - N : v);
     v %= N;
     return v < 0 ? v + N : v;
 }
 
-template <int LDA, typename PosT, typename ValT, typename AccT, int WPE>
+template <int LDA, typename PosT, typename ValT, typename AccT, int WPE, bool OPEN>
 __global__ void __launch_bounds__(M3_THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
           const ValT* __restrict__ field, FieldGeom fg, ValT* __restrict__ up, int lda, int c0, int tz) {
@@ -91,12 +92,12 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         const int c = i % LDA;
         const int node = i / LDA;
         const int nx = node & 15, ry = node >> 4;
-        const int gx = stored_node(m, fg, 0, tx0 - 1 + nx) - fg.lo[0];
-        const int gy = stored_node(m, fg, 1, ty0 - 1 + ry) - fg.lo[1];
+        const int gx = stored_node<OPEN>(m, fg, 0, tx0 - 1 + nx) - fg.lo[0];
+        const int gy = stored_node<OPEN>(m, fg, 1, ty0 - 1 + ry) - fg.lo[1];
         soff[j] = (unsigned)(((gy * fg.ext[0] + gx) * lda + c0 + c) * (int)sizeof(ValT));
     }
     auto fetch_plane = [&](int n) {
-        const int nz = stored_node(m, fg, 2, n) - fg.lo[2];
+        const int nz = stored_node<OPEN>(m, fg, 2, n) - fg.lo[2];
         const char* base = (const char*)(field + (int64_t)nz * fg.ext[0] * fg.ext[1] * lda);
         // component fastest in the field (coalesced), component-major in the ring
 #pragma unroll
@@ -245,10 +246,10 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
 
 // One launch of the gather for components c0 .. c0+LL-1; the z chunk comes
 // from the resident workgroup count (each chunk loads 3 extra node planes).
-template <int LL, typename PosT, typename ValT, typename AccT>
-static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
-                      const ValT* field, ValT* up, hipStream_t s) {
-    auto kernel = k_m3_tile<LL, PosT, ValT, AccT, sizeof(ValT) == 4 ? 3 : 1>;
+template <int LL, typename PosT, typename ValT, typename AccT, bool OPEN>
+static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
+                           const ValT* field, ValT* up, hipStream_t s) {
+    auto kernel = k_m3_tile<LL, PosT, ValT, AccT, sizeof(ValT) == 4 ? 3 : 1, OPEN>;
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(kernel, M3_THREADS));
     const int tx = (m.nbin[0] + M3_TX - 1) / M3_TX, ty = (m.nbin[1] + M3_TY - 1) / M3_TY;
@@ -256,6 +257,16 @@ static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const F
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.nbin[2] + tz - 1) / tz)));
     hipLaunchKernelGGL(kernel, grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
                        tile_perm(plan), xp, field, fg, up, fg.lda, c0, tz);
+}
+
+// OPEN: some axis is open (a fully periodic mesh compiles without the clamp).
+template <int LL, typename PosT, typename ValT, typename AccT>
+static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
+                      const ValT* field, ValT* up, hipStream_t s) {
+    if (m.periodic[0] && m.periodic[1] && m.periodic[2])
+        launch_m3_open<LL, PosT, ValT, AccT, false>(plan, m, xp, fg, c0, field, up, s);
+    else
+        launch_m3_open<LL, PosT, ValT, AccT, true>(plan, m, xp, fg, c0, field, up, s);
 }
 
 template <typename PosT, typename ValT, typename AccT>

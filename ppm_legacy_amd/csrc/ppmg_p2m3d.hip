@@ -88,7 +88,7 @@ __device__ __forceinline__ void shift_in(const ValT (&wx)[4], const ValT (&wz)[4
 // every node receives its additions in an order fixed by the program and the
 // binning, as with one wavefront.  A taller tile (more wavefronts) re-reads
 // fewer halo cells: the window is 16 x (TY+3) cells for 13 x TY nodes.
-template <int LDA, typename PosT, typename ValT, int TY, int NW>
+template <int LDA, typename PosT, typename ValT, int TY, int NW, bool OPEN>
 __global__ void __launch_bounds__(64 * NW)
 k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
           const ValT* __restrict__ up, FieldGeom fg, int c0, int tz, ValT* __restrict__ field) {
@@ -120,10 +120,11 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const int lx = ex - 3;                          // node this lane owns after the row sum
     const bool xown = lx >= 0 && lx < nxo;
     // cell c of the window along axis d: a periodic axis wraps, an open one
-    // has no cell outside 0..nbin-1 (-1)
+    // has no cell outside 0..nbin-1 (-1).  OPEN: some axis is open (a fully
+    // periodic mesh compiles without the checks).
     auto wcell = [&](int c, int d) {
         const int nb = m.nbin[d];
-        if (m.periodic[d]) {
+        if (!OPEN || m.periodic[d]) {
             c %= nb;
             return c < 0 ? c + nb : c;
         }
@@ -137,7 +138,7 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         int ey = q * 4 + erow;
         ey4[j] = ey;
         const int cy = wcell(ty0 - 2 + ey, 1);
-        keyxy[j] = cx < 0 || cy < 0 ? -1 : cx + m.nbin[0] * cy;
+        keyxy[j] = OPEN && (cx < 0 || cy < 0) ? -1 : cx + m.nbin[0] * cy;
     }
     __syncthreads();
     const int64_t plane_cells = (int64_t)m.nbin[0] * m.nbin[1];
@@ -151,7 +152,7 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         f = 0;
         n = 0;
         const int czw = cz <= z1 ? wcell(cz, 2) : -1;
-        if (czw >= 0 && keyxy[j] >= 0) {
+        if (czw >= 0 && (!OPEN || keyxy[j] >= 0)) {
             const int key = (int)(plane_cells * czw + keyxy[j]);
             f = start[key];
             n = start[key + 1];   // end, subtracted where used: the load stays in flight
@@ -354,16 +355,16 @@ bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGe
 // 13 x TY nodes and NW wavefronts per tile.  The z chunk is chosen from the
 // device's resident workgroup count (queried once per instantiation): each
 // chunk marches 3 extra cell planes.
-template <int LL, typename PosT, typename ValT, int TY, int NW>
+template <int LL, typename PosT, typename ValT, int TY, int NW, bool OPEN>
 static void launch_p3_shape(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const FieldGeom& fg,
                             int c0, ValT* field, hipStream_t s) {
     static std::atomic<int> slots{0};
-    if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT, TY, NW>, 64 * NW));
+    if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT, TY, NW, OPEN>, 64 * NW));
     const int Rx = m.N[0] + !m.periodic[0], Ry = m.N[1] + !m.periodic[1], Rz = m.N[2] + !m.periodic[2];
     const int tx = (Rx + P3_TT - 1) / P3_TT, ty = (Ry + TY - 1) / TY;
     const int tz = pick_z_chunk((int64_t)tx * ty, Rz, slots.load(), 3);
     dim3 grid((unsigned)((int64_t)tx * ty * ((Rz + tz - 1) / tz)));
-    hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT, TY, NW>), grid, dim3(64 * NW), 0, s, m, plan->start.as<int>(),
+    hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT, TY, NW, OPEN>), grid, dim3(64 * NW), 0, s, m, plan->start.as<int>(),
                        tile_perm(plan), xp, up, fg, c0, tz, field);
 }
 
@@ -378,16 +379,24 @@ static int p3_rows() {
     return r == 9 || r == 13 || r == 21 || r == 29 ? r : P3_TY;
 }
 
+template <int LL, typename PosT, typename ValT, bool OPEN>
+static void launch_p3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const FieldGeom& fg,
+                           int c0, ValT* field, hipStream_t s) {
+    switch (p3_rows()) {
+        case 9: launch_p3_shape<LL, PosT, ValT, 9, 1, OPEN>(plan, m, xp, up, fg, c0, field, s); break;
+        case 13: launch_p3_shape<LL, PosT, ValT, 13, 2, OPEN>(plan, m, xp, up, fg, c0, field, s); break;
+        case 29: launch_p3_shape<LL, PosT, ValT, 29, 4, OPEN>(plan, m, xp, up, fg, c0, field, s); break;
+        default: launch_p3_shape<LL, PosT, ValT, 21, 3, OPEN>(plan, m, xp, up, fg, c0, field, s); break;
+    }
+}
+
 template <int LL, typename PosT, typename ValT>
 static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const FieldGeom& fg, int c0,
-                      ValT* field,
-                      hipStream_t s) {
-    switch (p3_rows()) {
-        case 9: launch_p3_shape<LL, PosT, ValT, 9, 1>(plan, m, xp, up, fg, c0, field, s); break;
-        case 13: launch_p3_shape<LL, PosT, ValT, 13, 2>(plan, m, xp, up, fg, c0, field, s); break;
-        case 29: launch_p3_shape<LL, PosT, ValT, 29, 4>(plan, m, xp, up, fg, c0, field, s); break;
-        default: launch_p3_shape<LL, PosT, ValT, 21, 3>(plan, m, xp, up, fg, c0, field, s); break;
-    }
+                      ValT* field, hipStream_t s) {
+    if (m.periodic[0] && m.periodic[1] && m.periodic[2])
+        launch_p3_open<LL, PosT, ValT, false>(plan, m, xp, up, fg, c0, field, s);
+    else
+        launch_p3_open<LL, PosT, ValT, true>(plan, m, xp, up, fg, c0, field, s);
 }
 
 // Stored nodes besides the unique ones, for axes whose stored box holds the
