@@ -40,7 +40,52 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=256, help="cells per axis of the CPU sample")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--slab", action="store_true",
+                    help="only time one GPU's share of an 8-way z split of the 512^3 cycle: a 512x512x64-cell "
+                         "slab with open z faces in the reference's stored box (ghost size 2)")
     return ap.parse_args()
+
+
+def slab_cycle(core, torch, steps, warmup, lda, seed):
+    """P2M + M2P on a 512 x 512 x 64-cell slab (h = 1/512, z faces open, the
+    field in the reference's 1-g..Nm+g box with g = 2), 16.8M particles on a
+    jittered lattice for each leg."""
+    NX, NY, NZ = 512, 512, 64
+    h = 1.0 / NX
+
+    def lattice(s):
+        gen = torch.Generator(device="cuda").manual_seed(s)
+        n = NX * NY * NZ
+        idx = torch.arange(n, device="cuda", dtype=torch.int64)
+        cells = torch.stack([idx % NX, (idx // NX) % NY, idx // (NX * NY)], 1).double()
+        jit = 0.6 * (torch.rand((n, 3), dtype=torch.float64, device="cuda", generator=gen) - 0.5)
+        return ((cells + 0.5 + jit) * h).contiguous()
+
+    x, xq = lattice(seed), lattice(seed + 1)
+    u = torch.randn((x.shape[0], lda), dtype=torch.float64, device="cuda")
+    mesh = core.Mesh(3, [NX + 1, NY + 1, NZ + 1], [0.0, 0.0, 0.0], [1.0, 1.0, NZ * h], bcdef=[1, 1, 1, 1, 2, 2])
+    field = torch.empty(mesh.padded_shape(lda, [2, 2, 2]), dtype=torch.float64, device="cuda")
+    out = torch.empty((xq.shape[0], lda), dtype=torch.float64, device="cuda")
+    pp, pm = core.Plan(), core.Plan()
+    for _ in range(warmup):
+        core.p2m(mesh, x, u, field, [-2] * 3, 2, core.PREC_F64, pp)
+        core.m2p(mesh, xq, field, [-2] * 3, out, 2, core.PREC_F64, pm)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        core.p2m(mesh, x, u, field, [-2] * 3, 2, core.PREC_F64, pp)
+        core.m2p(mesh, xq, field, [-2] * 3, out, 2, core.PREC_F64, pm)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return {"metric": "particles/s per GPU, 3D M'4 remesh (P2M+M2P) on one GPU's z-slab of 512^3",
+            "value": x.shape[0] / (ms * 1e-3), "unit": "particles/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "dtype": "f64",
+            "data": "synthetic (jittered lattices)",
+            "config": {"workload": "512x512x64-cell slab, z faces open, stored box with ghost size 2",
+                       "particles": x.shape[0], "lda": lda}}
 
 
 def make_inputs(core, wl, N, lda, seed, torch):
@@ -166,7 +211,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+                                   "(sharding over several GPUs is not part of it)"}))
         sys.exit(2)
     import torch
     from ppm_legacy_amd import core, workloads as wl
@@ -179,6 +224,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8, "launches": 3}))
+        return
+    if args.slab:
+        print(json.dumps(slab_cycle(core, torch, args.steps, args.warmup, args.lda, args.seed)))
         return
     N, lda = args.n, args.lda
     np_ = N ** 3
