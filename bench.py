@@ -47,9 +47,10 @@ def parse():
 
 
 def slab_cycle(core, torch, steps, warmup, lda, seed):
-    """P2M + M2P on a 512 x 512 x 64-cell slab (h = 1/512, z faces open, the
-    field in the reference's 1-g..Nm+g box with g = 2), 16.8M particles on a
-    jittered lattice for each leg."""
+    """P2M + M2P on a 512 x 512 x 64-cell slab (h = 1/512, z faces open),
+    16.8M particles on a jittered lattice for each leg.  The field holds the
+    unique nodes of the periodic x and y axes (compact) and, along z, the
+    reference's 1-g..Nm+g box with g = 2."""
     NX, NY, NZ = 512, 512, 64
     h = 1.0 / NX
 
@@ -64,19 +65,20 @@ def slab_cycle(core, torch, steps, warmup, lda, seed):
     x, xq = lattice(seed), lattice(seed + 1)
     u = torch.randn((x.shape[0], lda), dtype=torch.float64, device="cuda")
     mesh = core.Mesh(3, [NX + 1, NY + 1, NZ + 1], [0.0, 0.0, 0.0], [1.0, 1.0, NZ * h], bcdef=[1, 1, 1, 1, 2, 2])
-    field = torch.empty(mesh.padded_shape(lda, [2, 2, 2]), dtype=torch.float64, device="cuda")
+    lo = [0, 0, -2]
+    field = torch.empty((NZ + 5, NY, NX, lda), dtype=torch.float64, device="cuda")
     out = torch.empty((xq.shape[0], lda), dtype=torch.float64, device="cuda")
     pp, pm = core.Plan(), core.Plan()
     for _ in range(warmup):
-        core.p2m(mesh, x, u, field, [-2] * 3, 2, core.PREC_F64, pp)
-        core.m2p(mesh, xq, field, [-2] * 3, out, 2, core.PREC_F64, pm)
+        core.p2m(mesh, x, u, field, lo, 2, core.PREC_F64, pp)
+        core.m2p(mesh, xq, field, lo, out, 2, core.PREC_F64, pm)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(steps):
-        core.p2m(mesh, x, u, field, [-2] * 3, 2, core.PREC_F64, pp)
-        core.m2p(mesh, xq, field, [-2] * 3, out, 2, core.PREC_F64, pm)
+        core.p2m(mesh, x, u, field, lo, 2, core.PREC_F64, pp)
+        core.m2p(mesh, xq, field, lo, out, 2, core.PREC_F64, pm)
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
@@ -84,7 +86,7 @@ def slab_cycle(core, torch, steps, warmup, lda, seed):
             "value": x.shape[0] / (ms * 1e-3), "unit": "particles/s", "n_gpus": 1, "steps": steps,
             "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "dtype": "f64",
             "data": "synthetic (jittered lattices)",
-            "config": {"workload": "512x512x64-cell slab, z faces open, stored box with ghost size 2",
+            "config": {"workload": "512x512x64-cell slab, z faces open, x/y compact, z box with ghost size 2",
                        "particles": x.shape[0], "lda": lda}}
 
 

@@ -123,12 +123,16 @@ k_p2m_gather(DevMesh m, FieldGeom f, ValT* __restrict__ field, const int* __rest
 // Stored index of node v along axis d for a read, with ghost_get's periodic
 // node map (ppm_map_field_ghost_get.f:257-280: node 0 and the ghosts below it
 // take node v+N, node N and the ghosts above it take v-N); -1 if not stored.
-__device__ __forceinline__ int read_index(const DevMesh& m, const FieldGeom& f, int d, int v) {
+// `filled` is false for a node that ghost_get leaves as stored (see
+// real_on_open_axis): on a stored box it is then read at v itself.
+__device__ __forceinline__ int read_index(const DevMesh& m, const FieldGeom& f, int d, int v, bool filled = true) {
     if (m.periodic[d]) {
         int N = m.N[d];
         if (f.swap[d]) {
-            if (v <= 0) v += N;
-            else if (v >= N) v -= N;
+            if (filled) {
+                if (v <= 0) v += N;
+                else if (v >= N) v -= N;
+            }
         } else {
             v %= N;
             if (v < 0) v += N;
@@ -136,6 +140,16 @@ __device__ __forceinline__ int read_index(const DevMesh& m, const FieldGeom& f, 
     }
     int l = v - f.lo[d];
     return (l >= 0 && l < f.ext[d]) ? l : -1;
+}
+
+// ghost_get copies shifted images of the sub's real nodes only
+// (ppm_map_field_ghost_init.f:234-447, ppm_mesh_block_intersect.f:207-279):
+// on a mesh with non-periodic axes it fills a node only when the node's
+// coordinate on every non-periodic axis is a real node 0..N.  Such a node takes
+// the periodic node map on the periodic axes; any other node keeps what is
+// stored there, on every axis.
+__device__ __forceinline__ bool real_on_open_axis(const DevMesh& m, int d, int v) {
+    return m.periodic[d] || (v >= 0 && v <= m.N[d]);
 }
 
 // M2P gather: one thread per particle (ppm_interp_m2p.f:895-1240, 3245-4106).
@@ -166,13 +180,19 @@ k_m2p_gather(DevMesh m, FieldGeom f, const ValT* __restrict__ field, const PosT*
     bool bad = false;
 #pragma unroll
     for (int ox = 0; ox < KS; ++ox) {
-        int ix = read_index(m, f, 0, nb[0] + ox);
+        const int vx = nb[0] + ox;
+        const bool rx = real_on_open_axis(m, 0, vx);
 #pragma unroll
         for (int oy = 0; oy < KS; ++oy) {
-            int iy = read_index(m, f, 1, nb[1] + oy);
+            const int vy = nb[1] + oy;
+            const bool rxy = rx && real_on_open_axis(m, 1, vy);
 #pragma unroll
             for (int oz = 0; oz < KZ; ++oz) {
-                int iz = DIM == 3 ? read_index(m, f, 2, nb[2] + oz) : 0;
+                const int vz = nb[2] + oz;
+                const bool filled = rxy && (DIM == 2 || real_on_open_axis(m, 2, vz));
+                const int ix = read_index(m, f, 0, vx, filled);
+                const int iy = read_index(m, f, 1, vy, filled);
+                const int iz = DIM == 3 ? read_index(m, f, 2, vz, filled) : 0;
                 if (ix < 0 || iy < 0 || iz < 0) {
                     bad = true;
                     continue;

@@ -50,7 +50,8 @@ constexpr int M3_THREADS = 16 * M3_TY;  // 256
 // on a periodic axis the compact layout wraps, and a stored box holding node
 // 0 and node N reads with ghost_get's node map (ppm_map_field_ghost_get.f:
 // 257-280: node 0 and the ghosts below it take node v+N, node N and the ghosts
-// above it v-N); an open axis reads node v itself.  Stencils reach -1..N+2;
+// above it v-N); an open axis reads node v itself (open axes come here only
+// beside periodic axes in the compact layout).  Stencils reach -1..N+2;
 // the ring of a tile also loads columns beyond that which no stencil reads:
 // they wrap (periodic) or are clamped into the box (open).
 template <bool OPEN>
@@ -227,8 +228,17 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
 // Per periodic axis the compact layout or a stored box holding the whole
 // real range 0..N (read with the ghost_get node map); per open axis a box
 // holding every node a stencil reaches, -1..N+2 (ghost size 2 or more).
+// An open axis next to a periodic axis stored as a box is left to the general
+// path: there ghost_get maps a node only when its open coordinates are real
+// nodes, which the per-axis offsets of the ring fetch do not express.
 bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
     if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return false;
+    bool open = false, swapped = false;
+    for (int d = 0; d < 3; ++d) {
+        open = open || !m.periodic[d];
+        swapped = swapped || fg.swap[d];
+    }
+    if (open && swapped) return false;
     for (int d = 0; d < 3; ++d) {
         if (m.N[d] < 4) return false;
         const int hi = fg.lo[d] + fg.ext[d] - 1;

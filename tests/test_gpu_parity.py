@@ -357,7 +357,8 @@ def test_tiled_paths_open_axes(core, oracle, bc, N, monkeypatch):
             info, ref = oracle.p2m(t, MP4, xp, up, [2, 2, 2])
             assert info == 0
             assert _rel(fast.cpu().numpy(), ref) < 1e-12
-    field = torch.from_numpy(rng.normal(size=mesh.padded_shape(lda, [2] * 3))).cuda()
+    fr = rng.normal(size=mesh.padded_shape(lda, [2] * 3))
+    field = torch.from_numpy(fr).cuda()
     out = torch.empty((npart, lda), dtype=torch.float64, device="cuda")
     core.m2p(mesh, x, field, [-2] * 3, out)
     monkeypatch.setenv("PPMG_GENERAL", "1")
@@ -365,6 +366,114 @@ def test_tiled_paths_open_axes(core, oracle, bc, N, monkeypatch):
     core.m2p(mesh, x, field, [-2] * 3, out_gen)
     monkeypatch.delenv("PPMG_GENERAL")
     assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < 1e-13
+    # the oracle runs ghost_get first: beside an open axis it fills only the
+    # nodes whose open coordinates are real nodes, the rest read as stored
+    info, qref = oracle.m2p(t, MP4, xp, fr, [2] * 3)
+    assert info == 0
+    assert _rel(out.cpu().numpy(), qref) < 1e-12
+    if any(b == 1 for b in bc):
+        # compact periodic axes beside boxed open ones: the tiled gather
+        # against the general one on the same layout
+        lo = [0 if bc[2 * d] == 1 else -2 for d in range(3)]
+        shape = tuple(reversed([N if bc[2 * d] == 1 else N + 5 for d in range(3)])) + (lda,)
+        fm = torch.from_numpy(rng.normal(size=shape)).cuda()
+        core.m2p(mesh, x, fm, lo, out)
+        pm = torch.full(shape, 7.0, dtype=torch.float64, device="cuda")
+        core.p2m(mesh, x, u, pm, lo)
+        monkeypatch.setenv("PPMG_GENERAL", "1")
+        core.m2p(mesh, x, fm, lo, out_gen)
+        pg = torch.full_like(pm, 7.0)
+        core.p2m(mesh, x, u, pg, lo)
+        monkeypatch.delenv("PPMG_GENERAL")
+        assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < 1e-13
+        assert _rel(pm.cpu().numpy(), pg.cpu().numpy()) < 1e-13
+
+
+@pytest.mark.parametrize("bc", [[1] * 6, [2] * 6, [1, 1, 1, 1, 2, 2]], ids=["periodic", "all_open", "open_z"])
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_smallest_meshes(core, oracle, bc, N, monkeypatch):
+    """Meshes of 2-4 cells per axis, where every stencil wraps onto itself
+    (ppm_topo_mkfield.f:421 allows Nm >= 2).  N = 4 is the smallest mesh the
+    tiled kernels take, N = 2 and 3 run the general paths.  P2M and M2P on
+    the reference's stored box (ghost size 2) against the oracle; on a
+    periodic mesh also the compact layout against the numpy brute force;
+    at N = 4 the tiled kernels against the general ones."""
+    rng = np.random.default_rng(40 + 3 * N + sum(bc))
+    lda = 3
+    xp = rng.uniform(0.0, 1.0, size=(600, 3))
+    for d in range(3):
+        if bc[2 * d] != 1:
+            xp[rng.uniform(size=len(xp)) < 0.05, d] = 1.0
+            xp[rng.uniform(size=len(xp)) < 0.05, d] = 0.0
+    up = rng.normal(size=(len(xp), lda))
+    t = oracle.topo(3, [N + 1] * 3, bcdef=bc)
+    mesh = core.Mesh(3, N + 1, bcdef=bc)
+    x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
+    g = [2] * 3
+    info, ref = oracle.p2m(t, MP4, xp, up, g)
+    assert info == 0
+    field = torch.full(mesh.padded_shape(lda, g), 7.0, dtype=torch.float64, device="cuda")
+    core.p2m(mesh, x, u, field, [-2] * 3)
+    assert _rel(field.cpu().numpy(), ref) < 1e-12
+    fr = rng.normal(size=oracle.field_shape(t, lda, g))
+    info, qref = oracle.m2p(t, MP4, xp, fr, g)
+    assert info == 0
+    q = torch.empty((len(xp), lda), dtype=torch.float64, device="cuda")
+    core.m2p(mesh, x, _dev(fr, torch.float64), [-2] * 3, q)
+    assert _rel(q.cpu().numpy(), qref) < 1e-12
+    if bc == [1] * 6:
+        comp = torch.empty(mesh.compact_shape(lda), dtype=torch.float64, device="cuda")
+        core.p2m(mesh, x, u, comp, [0] * 3)
+        assert _rel(comp.cpu().numpy(), bf.p2m_periodic(xp, up, N)) < 1e-12
+        uniq = rng.normal(size=(N, N, N, lda))
+        core.m2p(mesh, x, _dev(uniq, torch.float64), [0] * 3, q)
+        assert _rel(q.cpu().numpy(), bf.m2p_periodic(xp, uniq, N)) < 1e-12
+    if N == 4:
+        monkeypatch.setenv("PPMG_GENERAL", "1")
+        gen = torch.full_like(field, 7.0)
+        core.p2m(mesh, x, u, gen, [-2] * 3)
+        qg = torch.empty_like(q)
+        core.m2p(mesh, x, _dev(fr, torch.float64), [-2] * 3, qg)
+        monkeypatch.delenv("PPMG_GENERAL")
+        assert _rel(field.cpu().numpy(), gen.cpu().numpy()) < 1e-13
+        q2 = torch.empty_like(q)
+        core.m2p(mesh, x, _dev(fr, torch.float64), [-2] * 3, q2)
+        assert _rel(q2.cpu().numpy(), qg.cpu().numpy()) < 1e-13
+
+
+def test_empty_inputs(core, oracle):
+    """No particles: every entry point returns without error and matches the
+    oracle's empty results (P2M and remesh leave a zero field, the ranking
+    gives empty boxes, create_part of a zero field selects nothing)."""
+    N = 8
+    mesh = core.Mesh(3, N + 1)
+    t = oracle.topo(3, [N + 1] * 3)
+    x = torch.zeros((0, 3), dtype=torch.float64, device="cuda")
+    u = torch.zeros((0, 2), dtype=torch.float64, device="cuda")
+    field = torch.full(mesh.compact_shape(2), 3.0, dtype=torch.float64, device="cuda")
+    plan = core.p2m(mesh, x, u, field, [0] * 3)
+    assert torch.all(field == 0)
+    q = torch.zeros((0, 2), dtype=torch.float64, device="cuda")
+    core.m2p(mesh, x, field, [0] * 3, q)
+    gu, gx = core.p2m_adjoint(mesh, x, u, field, [0] * 3, plan)
+    assert gu.shape == (0, 2) and gx.shape == (0, 3)
+    lp, lh = core.util_rank(x, [0.0] * 3, [1.0] * 3, [4, 4, 4], [0] * 6)
+    lp_ref, lh_ref, ic, _, _ = oracle.rank(np.zeros((0, 3)), [0.0] * 3, [1.0] * 3, [4, 4, 4], [0] * 6)
+    assert ic == 0 and lp.numel() == 0 and np.array_equal(lh.cpu().numpy(), lh_ref)
+    npbx = core.util_sort(x, [0.0] * 3, [1.0] * 3, [4, 4, 4])
+    assert torch.all(npbx == 0)
+    core.impose_part_bc(mesh, x)
+    w = core.comp_weights(mesh, x)
+    assert all(wd.shape == (0, 4) for wd in w)
+    g = [2] * 3
+    info, fref = oracle.remesh(t, MP4, np.zeros((0, 3)), np.zeros((0, 2)), g, [np.zeros((0, 4))] * 3)
+    assert info == 0
+    rf = torch.full(mesh.padded_shape(2, g), 5.0, dtype=torch.float64, device="cuda")
+    core.remesh(mesh, x, u, w, rf, [-2] * 3)
+    assert np.array_equal(rf.cpu().numpy(), fref)
+    zero = torch.zeros(mesh.padded_shape(2, [1] * 3), dtype=torch.float64, device="cuda")
+    xo, uo = core.create_part(mesh, zero, [-1] * 3, [0.5, 1.0])
+    assert xo.shape == (0, 3) and uo.shape == (0, 2)
 
 
 @pytest.mark.parametrize("prec", [F64, F32C])
@@ -484,6 +593,33 @@ def test_m2p_variants(core, oracle, dim, kernel, prec, lda):
     up = torch.empty((2000, lda), dtype=_val_dtype(prec), device="cuda")
     core.m2p(mesh, _dev(xp, _pos_dtype(prec)), _dev(f, _val_dtype(prec)), [-2] * dim, up, kernel, prec)
     assert _rel(up.cpu().numpy(), ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("dim,bc", [(2, [1, 1, 2, 2]), (2, [2, 2, 1, 1]), (3, [2, 2, 1, 1, 1, 1]),
+                                    (3, [1, 1, 2, 2, 2, 2])])
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+def test_m2p_mixed_axes_vs_oracle(core, oracle, dim, bc, kernel):
+    """M2P on a mesh with periodic and open axes, from a random stored box
+    whose ghosts all differ: ghost_get maps a node only when its open
+    coordinates are real nodes 0..N, so ghost nodes beside an open face are
+    read as stored on every axis.  A tenth of the particles sit on the open
+    faces, where the stencils reach those nodes."""
+    N = 9
+    g = [2] * dim
+    rng = np.random.default_rng(60 + dim + kernel + sum(bc))
+    xp = rng.uniform(0, 1, size=(3000, dim))
+    for d in range(dim):
+        if bc[2 * d] != 1:
+            xp[rng.uniform(size=len(xp)) < 0.05, d] = 1.0
+            xp[rng.uniform(size=len(xp)) < 0.05, d] = 0.0
+    t = oracle.topo(dim, [N + 1] * dim, bcdef=bc)
+    f = rng.normal(size=oracle.field_shape(t, 2, g))
+    info, ref = oracle.m2p(t, kernel, xp, f, g)
+    assert info == 0
+    mesh = core.Mesh(dim, N + 1, bcdef=bc)
+    up = torch.empty((len(xp), 2), dtype=torch.float64, device="cuda")
+    core.m2p(mesh, _dev(xp, torch.float64), _dev(f, torch.float64), [-2] * dim, up, kernel)
+    assert _rel(up.cpu().numpy(), ref) < 1e-12
 
 
 def test_m2p_compact_layout(core):
