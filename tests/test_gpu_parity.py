@@ -843,6 +843,81 @@ def test_create_part_slave_field(core, oracle):
     assert xn is None and np.array_equal(un.cpu().numpy(), uref) and np.array_equal(wn.cpu().numpy(), w)
 
 
+@pytest.mark.parametrize("bc", [[1, 1, 1, 1, 2, 2], [2, 2, 1, 1, 1, 1], [2] * 6], ids=["open_z", "open_x", "all_open"])
+def test_remesh_create_part_bc_on_open_axes(core, oracle, bc):
+    """comp_weights + remesh, create_part and impose_part_bc on meshes with
+    non-periodic axes, against the oracle: remesh folds only the periodic
+    axes, create_part takes node N of an open axis (a real node there) but
+    not of a periodic one, impose_part_bc moves only periodic coordinates."""
+    N = 10
+    rng = np.random.default_rng(70 + sum(bc))
+    t = oracle.topo(3, [N + 1] * 3, bcdef=bc)
+    mesh = core.Mesh(3, N + 1, bcdef=bc)
+    xp = rng.uniform(0, 1, size=(1500, 3))
+    up = rng.normal(size=(len(xp), 2))
+    info, wref = oracle.comp_weights(t, MP4, xp)
+    assert info == 0
+    x = _dev(xp, torch.float64)
+    w = core.comp_weights(mesh, x, MP4)
+    for d in range(3):
+        assert np.max(np.abs(w[d].cpu().numpy() - wref[d])) < 1e-15
+    info, fref = oracle.remesh(t, MP4, xp, up, [2] * 3, wref)
+    assert info == 0
+    field = torch.zeros(mesh.padded_shape(2, [2] * 3), dtype=torch.float64, device="cuda")
+    core.remesh(mesh, x, _dev(up, torch.float64), w, field, [-2] * 3, MP4)
+    assert _rel(field.cpu().numpy(), fref) < 1e-12
+    f = rng.normal(size=oracle.field_shape(t, 3, [1] * 3))
+    info, xref, uref = oracle.create_part(t, f, [1] * 3, [0.7, 1.9])
+    assert info == 0 and len(xref) > 0
+    xc, uc = core.create_part(mesh, _dev(f, torch.float64), [-1] * 3, [0.7, 1.9])
+    assert np.array_equal(xc.cpu().numpy(), xref)
+    assert np.array_equal(uc.cpu().numpy(), uref)
+    xb = rng.uniform(-0.5, 1.5, size=(2000, 3))
+    ref = oracle.impose_part_bc(t, xb)
+    xd = _dev(xb, torch.float64)
+    core.impose_part_bc(mesh, xd)
+    assert np.array_equal(xd.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+def test_adjoint_open_mesh(core, oracle, kernel):
+    """The adjoint on a mesh with every axis open (the cotangent on the stored
+    box, ghosts included), which has no periodic oracle form.  Strength
+    gradient: <P2M(u), g> = <u, dL/du> on every stored node.  Position
+    gradient: central differences of the oracle's P2M of one particle,
+    L_p(x) = <P2M({x}, {u_p}), g>, with step 1e-6 h (both kernels are C0 at
+    worst only on nodes, which random positions miss)."""
+    N = 10
+    bc = [2] * 6
+    rng = np.random.default_rng(80 + kernel)
+    xp = rng.uniform(0.05, 0.95, size=(600, 3))
+    up = rng.normal(size=(len(xp), 2))
+    t = oracle.topo(3, [N + 1] * 3, bcdef=bc)
+    mesh = core.Mesh(3, N + 1, bcdef=bc)
+    gc = rng.normal(size=mesh.padded_shape(2, [2] * 3))
+    x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
+    field = torch.empty(mesh.padded_shape(2, [2] * 3), dtype=torch.float64, device="cuda")
+    plan = core.p2m(mesh, x, u, field, [-2] * 3, kernel)
+    gu, gx = core.p2m_adjoint(mesh, x, u, _dev(gc, torch.float64), [-2] * 3, plan, kernel)
+    gu, gx = gu.cpu().numpy(), gx.cpu().numpy()
+    lhs = float(np.sum(field.cpu().numpy() * gc))
+    rhs = float(np.sum(up * gu))
+    assert abs(lhs - rhs) <= 1e-12 * np.sum(np.abs(field.cpu().numpy() * gc))
+    h = 1e-6 / N
+    scale = np.max(np.abs(gx))
+    for p in range(0, len(xp), 25):
+        for d in range(3):
+            xs = []
+            for s in (1.0, -1.0):
+                xq = xp[p:p + 1].copy()
+                xq[0, d] += s * h
+                info, fp = oracle.p2m(t, kernel, xq, up[p:p + 1], [2] * 3)
+                assert info == 0
+                xs.append(float(np.sum(fp * gc)))
+            fd = (xs[0] - xs[1]) / (2 * h)
+            assert abs(gx[p, d] - fd) <= 1e-6 * scale, (p, d, gx[p, d], fd)
+
+
 def test_impose_part_bc(core, oracle):
     t = oracle.topo(3, [9, 9, 9], [-1.0, 0.0, 0.0], [1.0, 1.0, 2.0])
     rng = np.random.default_rng(12)
