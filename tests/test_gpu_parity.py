@@ -288,9 +288,14 @@ def test_p2m_tile_heights_components(core, rows, lda, monkeypatch):
     assert _rel(f.cpu().numpy(), gen.cpu().numpy()) < 1e-13
 
 
+# tiled against general on the same inputs: only the summation order differs
+GEN_TOL = {F64: 1e-13, F32C: 1e-6, F32: 1e-5}
+
+
+@pytest.mark.parametrize("prec", [F64, F32C, F32])
 @pytest.mark.parametrize("N", [5, 12, 40])
 @pytest.mark.parametrize("g", [0, 1, 2])
-def test_tiled_paths_padded_layout(core, N, g, monkeypatch):
+def test_tiled_paths_padded_layout(core, N, g, prec, monkeypatch):
     """The tiled kernels on the reference's stored box (lo = -g, ext = N+1+2g)
     against the general paths: P2M on every stored node (node N equal to node
     0, ghost layers the raw unwrapped sums), and M2P of a random stored field
@@ -302,34 +307,37 @@ def test_tiled_paths_padded_layout(core, N, g, monkeypatch):
     up = rng.normal(size=(len(xp), lda))
     mesh = core.Mesh(3, N + 1)
     lo = [-g] * 3
-    x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
-    fast = torch.full(mesh.padded_shape(lda, [g] * 3), 7.0, dtype=torch.float64, device="cuda")
-    core.p2m(mesh, x, u, fast, lo)
-    field = torch.from_numpy(rng.normal(size=mesh.padded_shape(lda, [g] * 3))).cuda()
-    out = torch.empty((len(xp), lda), dtype=torch.float64, device="cuda")
-    core.m2p(mesh, x, field, lo, out)
+    vdt = _val_dtype(prec)
+    x, u = _dev(xp, _pos_dtype(prec)), _dev(up, vdt)
+    fast = torch.full(mesh.padded_shape(lda, [g] * 3), 7.0, dtype=vdt, device="cuda")
+    core.p2m(mesh, x, u, fast, lo, MP4, prec)
+    field = _dev(rng.normal(size=mesh.padded_shape(lda, [g] * 3)), vdt)
+    out = torch.empty((len(xp), lda), dtype=vdt, device="cuda")
+    core.m2p(mesh, x, field, lo, out, MP4, prec)
     monkeypatch.setenv("PPMG_GENERAL", "1")
     gen = torch.full_like(fast, 7.0)
-    core.p2m(mesh, x, u, gen, lo)
+    core.p2m(mesh, x, u, gen, lo, MP4, prec)
     out_gen = torch.empty_like(out)
-    core.m2p(mesh, x, field, lo, out_gen)
+    core.m2p(mesh, x, field, lo, out_gen, MP4, prec)
     monkeypatch.delenv("PPMG_GENERAL")
-    assert _rel(fast.cpu().numpy(), gen.cpu().numpy()) < 1e-13
+    assert _rel(fast.cpu().numpy(), gen.cpu().numpy()) < GEN_TOL[prec]
     s = slice(g, g + N)
     assert torch.equal(fast[s, s, g + N], fast[s, s, g])          # node N = node 0 (x)
-    assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < 1e-13
+    assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < GEN_TOL[prec]
 
 
+@pytest.mark.parametrize("prec", [F64, F32C, F32])
 @pytest.mark.parametrize("bc", [[1, 1, 1, 1, 2, 2], [2, 2, 1, 1, 1, 1], [2, 2, 2, 2, 2, 2]],
                          ids=["open_z", "open_x", "all_open"])
 @pytest.mark.parametrize("N", [5, 12, 40])
-def test_tiled_paths_open_axes(core, oracle, bc, N, monkeypatch):
+def test_tiled_paths_open_axes(core, oracle, bc, N, prec, monkeypatch):
     """Meshes with non-periodic axes on the tiled kernels: on an open axis the
     tiles make the N+1 real nodes 0..N with no wrap, ghost nodes keep their raw
     sums, and the gather reads node v itself.  Against the general paths on
     every stored node (P2M with ghost sizes 0..2, M2P of a random stored field
-    with ghost size 2), and P2M against the oracle.  A tenth of the particles
-    sit exactly on a face of an open axis (cell N holds the upper face)."""
+    with ghost size 2), and against the oracle (FP32 against its float form).
+    A tenth of the particles sit exactly on a face of an open axis (cell N
+    holds the upper face)."""
     rng = np.random.default_rng(7 * N + sum(bc))
     lda = 3
     npart = max(6 * N ** 3, 3000)
@@ -341,52 +349,54 @@ def test_tiled_paths_open_axes(core, oracle, bc, N, monkeypatch):
             face = rng.uniform(size=npart) < 0.05
             xp[face, d] = 0.0
     up = rng.normal(size=(npart, lda))
+    vdt = _val_dtype(prec)
+    odt = np.float32 if prec == F32 else np.float64
     mesh = core.Mesh(3, N + 1, bcdef=bc)
-    x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
+    x, u = _dev(xp, _pos_dtype(prec)), _dev(up, vdt)
     t = oracle.topo(3, [N + 1] * 3, bcdef=bc)
     for g in (0, 1, 2):
         lo = [-g] * 3
-        fast = torch.full(mesh.padded_shape(lda, [g] * 3), 7.0, dtype=torch.float64, device="cuda")
-        core.p2m(mesh, x, u, fast, lo)
+        fast = torch.full(mesh.padded_shape(lda, [g] * 3), 7.0, dtype=vdt, device="cuda")
+        core.p2m(mesh, x, u, fast, lo, MP4, prec)
         monkeypatch.setenv("PPMG_GENERAL", "1")
         gen = torch.full_like(fast, 7.0)
-        core.p2m(mesh, x, u, gen, lo)
+        core.p2m(mesh, x, u, gen, lo, MP4, prec)
         monkeypatch.delenv("PPMG_GENERAL")
-        assert _rel(fast.cpu().numpy(), gen.cpu().numpy()) < 1e-13, g
+        assert _rel(fast.cpu().numpy(), gen.cpu().numpy()) < GEN_TOL[prec], g
         if g == 2:
-            info, ref = oracle.p2m(t, MP4, xp, up, [2, 2, 2])
+            info, ref = oracle.p2m(t, MP4, xp.astype(odt), up.astype(odt), [2, 2, 2], dtype=odt)
             assert info == 0
-            assert _rel(fast.cpu().numpy(), ref) < 1e-12
+            assert _rel(fast.cpu().numpy(), ref) < TOL[prec]
     fr = rng.normal(size=mesh.padded_shape(lda, [2] * 3))
-    field = torch.from_numpy(fr).cuda()
-    out = torch.empty((npart, lda), dtype=torch.float64, device="cuda")
-    core.m2p(mesh, x, field, [-2] * 3, out)
+    field = _dev(fr, vdt)
+    out = torch.empty((npart, lda), dtype=vdt, device="cuda")
+    core.m2p(mesh, x, field, [-2] * 3, out, MP4, prec)
     monkeypatch.setenv("PPMG_GENERAL", "1")
     out_gen = torch.empty_like(out)
-    core.m2p(mesh, x, field, [-2] * 3, out_gen)
+    core.m2p(mesh, x, field, [-2] * 3, out_gen, MP4, prec)
     monkeypatch.delenv("PPMG_GENERAL")
-    assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < 1e-13
+    assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < GEN_TOL[prec]
     # the oracle runs ghost_get first: beside an open axis it fills only the
     # nodes whose open coordinates are real nodes, the rest read as stored
-    info, qref = oracle.m2p(t, MP4, xp, fr, [2] * 3)
+    info, qref = oracle.m2p(t, MP4, xp.astype(odt), fr.astype(odt), [2] * 3, dtype=odt)
     assert info == 0
-    assert _rel(out.cpu().numpy(), qref) < 1e-12
+    assert _rel(out.cpu().numpy(), qref) < TOL[prec]
     if any(b == 1 for b in bc):
         # compact periodic axes beside boxed open ones: the tiled gather
         # against the general one on the same layout
         lo = [0 if bc[2 * d] == 1 else -2 for d in range(3)]
         shape = tuple(reversed([N if bc[2 * d] == 1 else N + 5 for d in range(3)])) + (lda,)
-        fm = torch.from_numpy(rng.normal(size=shape)).cuda()
-        core.m2p(mesh, x, fm, lo, out)
-        pm = torch.full(shape, 7.0, dtype=torch.float64, device="cuda")
-        core.p2m(mesh, x, u, pm, lo)
+        fm = _dev(rng.normal(size=shape), vdt)
+        core.m2p(mesh, x, fm, lo, out, MP4, prec)
+        pm = torch.full(shape, 7.0, dtype=vdt, device="cuda")
+        core.p2m(mesh, x, u, pm, lo, MP4, prec)
         monkeypatch.setenv("PPMG_GENERAL", "1")
-        core.m2p(mesh, x, fm, lo, out_gen)
+        core.m2p(mesh, x, fm, lo, out_gen, MP4, prec)
         pg = torch.full_like(pm, 7.0)
-        core.p2m(mesh, x, u, pg, lo)
+        core.p2m(mesh, x, u, pg, lo, MP4, prec)
         monkeypatch.delenv("PPMG_GENERAL")
-        assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < 1e-13
-        assert _rel(pm.cpu().numpy(), pg.cpu().numpy()) < 1e-13
+        assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < GEN_TOL[prec]
+        assert _rel(pm.cpu().numpy(), pg.cpu().numpy()) < GEN_TOL[prec]
 
 
 @pytest.mark.parametrize("bc", [[1] * 6, [2] * 6, [1, 1, 1, 1, 2, 2]], ids=["periodic", "all_open", "open_z"])
