@@ -63,7 +63,25 @@ __device__ __forceinline__ int stored_node(const DevMesh& m, const FieldGeom& fg
     return v < 0 ? v + N : v;
 }
 
-template <int LDA, typename PosT, typename ValT, typename AccT, int WPE, bool OPEN>
+// Beside an open axis, ghost_get fills a stored node only when the node's
+// coordinate on every open axis is a real node 0..N (its blocks are shifted
+// images of the sub's real nodes, ppm_map_field_ghost_init.f:234-447); any
+// other node keeps what is stored there.
+__device__ __forceinline__ bool m3_real_on_open(const DevMesh& m, int d, int v) {
+    return m.periodic[d] || (v >= 0 && v <= m.N[d]);
+}
+
+// Stored node read as stored: v itself on a stored box (clamped into it for
+// the ring columns no stencil reads); a compact periodic axis wraps.
+template <bool OPEN>
+__device__ __forceinline__ int raw_node(const DevMesh& m, const FieldGeom& fg, int d, int v) {
+    if (fg.swap[d] || !m.periodic[d]) return min(max(v, fg.lo[d]), fg.lo[d] + fg.ext[d] - 1);
+    return stored_node<OPEN>(m, fg, d, v);
+}
+
+// MIX: some open axis beside a periodic axis stored as a box, where whether a
+// node takes the periodic node map depends on all three of its coordinates.
+template <int LDA, typename PosT, typename ValT, typename AccT, int WPE, bool OPEN, bool MIX>
 __global__ void __launch_bounds__(M3_THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
           const ValT* __restrict__ field, FieldGeom fg, ValT* __restrict__ up, int lda, int c0, int tz) {
@@ -86,6 +104,8 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     ValT stage[NLD];
     // Byte offset within a field plane of each element this thread stages
     // (the same in every plane; a plane fits in 2^31 bytes, checked at launch).
+    // MIX: bit 31 marks an element whose x or y is not a real node of an open
+    // axis, which every plane reads as stored.
     unsigned soff[NLD];
 #pragma unroll
     for (int j = 0; j < NLD; ++j) {
@@ -96,10 +116,35 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         const int gx = stored_node<OPEN>(m, fg, 0, tx0 - 1 + nx) - fg.lo[0];
         const int gy = stored_node<OPEN>(m, fg, 1, ty0 - 1 + ry) - fg.lo[1];
         soff[j] = (unsigned)(((gy * fg.ext[0] + gx) * lda + c0 + c) * (int)sizeof(ValT));
+        if (MIX && !(m3_real_on_open(m, 0, tx0 - 1 + nx) && m3_real_on_open(m, 1, ty0 - 1 + ry)))
+            soff[j] |= 0x80000000u;
     }
+    // MIX: byte offset of element i of a plane read as stored
+    auto raw_off = [&](int i) -> unsigned {
+        const int c = i % LDA;
+        const int node = i / LDA;
+        const int nx = node & 15, ry = node >> 4;
+        const int gx = raw_node<OPEN>(m, fg, 0, tx0 - 1 + nx) - fg.lo[0];
+        const int gy = raw_node<OPEN>(m, fg, 1, ty0 - 1 + ry) - fg.lo[1];
+        return (unsigned)(((gy * fg.ext[0] + gx) * lda + c0 + c) * (int)sizeof(ValT));
+    };
     auto fetch_plane = [&](int n) {
         const int nz = stored_node<OPEN>(m, fg, 2, n) - fg.lo[2];
         const char* base = (const char*)(field + (int64_t)nz * fg.ext[0] * fg.ext[1] * lda);
+        if (MIX) {
+            const bool rz = m3_real_on_open(m, 2, n);
+            const int nzr = raw_node<OPEN>(m, fg, 2, n) - fg.lo[2];
+            const char* braw = (const char*)(field + (int64_t)nzr * fg.ext[0] * fg.ext[1] * lda);
+#pragma unroll
+            for (int j = 0; j < NLD; ++j) {
+                const int i = tid + j * M3_THREADS;
+                if (i < M3_PL * LDA) {
+                    if (rz && (soff[j] >> 31) == 0u) stage[j] = *(const ValT*)(base + soff[j]);
+                    else stage[j] = *(const ValT*)(braw + raw_off(i));
+                }
+            }
+            return;
+        }
         // component fastest in the field (coalesced), component-major in the ring
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
@@ -228,9 +273,9 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
 // Per periodic axis the compact layout or a stored box holding the whole
 // real range 0..N (read with the ghost_get node map); per open axis a box
 // holding every node a stencil reaches, -1..N+2 (ghost size 2 or more).
-// An open axis next to a periodic axis stored as a box is left to the general
-// path: there ghost_get maps a node only when its open coordinates are real
-// nodes, which the per-axis offsets of the ring fetch do not express.
+// Beside an open axis a periodic axis stored as a box must hold -1..N+2 too:
+// there ghost_get maps a node only when its open coordinates are real nodes,
+// and the others are read as stored.
 bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
     if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return false;
     bool open = false, swapped = false;
@@ -238,7 +283,10 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
         open = open || !m.periodic[d];
         swapped = swapped || fg.swap[d];
     }
-    if (open && swapped) return false;
+    // there a periodic box must hold every node a stencil reaches as stored
+    if (open && swapped)
+        for (int d = 0; d < 3; ++d)
+            if (fg.swap[d] && (fg.lo[d] > -1 || fg.lo[d] + fg.ext[d] - 1 < m.N[d] + 2)) return false;
     for (int d = 0; d < 3; ++d) {
         if (m.N[d] < 4) return false;
         const int hi = fg.lo[d] + fg.ext[d] - 1;
@@ -256,10 +304,10 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
 
 // One launch of the gather for components c0 .. c0+LL-1; the z chunk comes
 // from the resident workgroup count (each chunk loads 3 extra node planes).
-template <int LL, typename PosT, typename ValT, typename AccT, bool OPEN>
+template <int LL, typename PosT, typename ValT, typename AccT, bool OPEN, bool MIX>
 static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
                            const ValT* field, ValT* up, hipStream_t s) {
-    auto kernel = k_m3_tile<LL, PosT, ValT, AccT, sizeof(ValT) == 4 ? 3 : 1, OPEN>;
+    auto kernel = k_m3_tile<LL, PosT, ValT, AccT, sizeof(ValT) == 4 ? 3 : 1, OPEN, MIX>;
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(kernel, M3_THREADS));
     const int tx = (m.nbin[0] + M3_TX - 1) / M3_TX, ty = (m.nbin[1] + M3_TY - 1) / M3_TY;
@@ -269,14 +317,17 @@ static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, co
                        tile_perm(plan), xp, field, fg, up, fg.lda, c0, tz);
 }
 
-// OPEN: some axis is open (a fully periodic mesh compiles without the clamp).
+// OPEN: some axis is open (a fully periodic mesh compiles without the clamp);
+// MIX: and some periodic axis is stored as a box.
 template <int LL, typename PosT, typename ValT, typename AccT>
 static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
                       const ValT* field, ValT* up, hipStream_t s) {
     if (m.periodic[0] && m.periodic[1] && m.periodic[2])
-        launch_m3_open<LL, PosT, ValT, AccT, false>(plan, m, xp, fg, c0, field, up, s);
+        launch_m3_open<LL, PosT, ValT, AccT, false, false>(plan, m, xp, fg, c0, field, up, s);
+    else if (fg.swap[0] || fg.swap[1] || fg.swap[2])
+        launch_m3_open<LL, PosT, ValT, AccT, true, true>(plan, m, xp, fg, c0, field, up, s);
     else
-        launch_m3_open<LL, PosT, ValT, AccT, true>(plan, m, xp, fg, c0, field, up, s);
+        launch_m3_open<LL, PosT, ValT, AccT, true, false>(plan, m, xp, fg, c0, field, up, s);
 }
 
 template <typename PosT, typename ValT, typename AccT>
