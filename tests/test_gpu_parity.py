@@ -990,6 +990,44 @@ def test_registry_host_arrays_like_the_reference(oracle):
     assert ppm.ppm_finalize() == 0
 
 
+@pytest.mark.parametrize("bc", [[1, 1, 1, 1, 2, 2], [2, 2, 2, 2, 1, 1]], ids=["open_z", "open_xy"])
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+def test_registry_mixed_axes(oracle, bc, kernel):
+    """The registry entry points (host arrays, the Fortran drop-in's route) on
+    meshes with open and periodic axes: P2M, and M2P of a random stored field
+    whose ghosts all differ, against the oracle, in the double and single
+    specifics.  Particles sit on the open faces too."""
+    from ppm_legacy_amd import ppm
+    assert ppm.ppm_init(3, ppm.ppm_kind_double, -15, None, 0) == 0
+    N = 14
+    g = [2, 2, 2]
+    info, topo, meshid = ppm.ppm_mktopo([0.0] * 3, [1.0] * 3, bc, [N + 1] * 3, g)
+    assert info == 0
+    rng = np.random.default_rng(90 + kernel + sum(bc))
+    xp = rng.uniform(0, 1, size=(5000, 3))
+    for d in range(3):
+        if bc[2 * d] != 1:
+            xp[rng.uniform(size=len(xp)) < 0.05, d] = 1.0
+            xp[rng.uniform(size=len(xp)) < 0.05, d] = 0.0
+    up = rng.normal(size=(len(xp), 2))
+    t = oracle.topo(3, [N + 1] * 3, bcdef=bc)
+    shape = (N + 1 + 4,) * 3 + (2,)
+    fr = rng.normal(size=shape)
+    for dt, tol in ((np.float64, 1e-12), (np.float32, 1e-5)):
+        x, u = xp.astype(dt), up.astype(dt)
+        field = np.zeros(shape, dt)
+        assert ppm.ppm_interp_p2m(x, len(x), u, topo, meshid, kernel, g, field) == 0
+        info, ref = oracle.p2m(t, kernel, x, u, g, dtype=dt)
+        assert info == 0
+        assert _rel(field, ref) < tol
+        back = np.zeros_like(u)
+        assert ppm.ppm_interp_m2p(x, len(x), back, topo, meshid, kernel, g, fr.astype(dt).copy()) == 0
+        info, bref = oracle.m2p(t, kernel, x, fr.astype(dt), g, dtype=dt)
+        assert info == 0
+        assert _rel(back, bref) < tol
+    assert ppm.ppm_finalize() == 0
+
+
 # ---------------------------------------------------------- ghost mapping --
 @pytest.mark.parametrize("dim,bc", [(3, [1] * 6), (3, [1, 1, 2, 2, 1, 1]), (2, [1] * 4), (2, [2, 2, 1, 1])])
 @pytest.mark.parametrize("g", [0, 1, 3])
