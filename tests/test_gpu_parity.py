@@ -608,15 +608,16 @@ def test_m2p_variants(core, oracle, dim, kernel, prec, lda):
 @pytest.mark.parametrize("dim,bc", [(2, [1, 1, 2, 2]), (2, [2, 2, 1, 1]), (3, [2, 2, 1, 1, 1, 1]),
                                     (3, [1, 1, 2, 2, 2, 2])])
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
-def test_m2p_mixed_axes_vs_oracle(core, oracle, dim, bc, kernel):
-    """M2P on a mesh with periodic and open axes, from a random stored box
-    whose ghosts all differ: ghost_get maps a node only when its open
-    coordinates are real nodes 0..N, so ghost nodes beside an open face are
-    read as stored on every axis.  A tenth of the particles sit on the open
-    faces, where the stencils reach those nodes."""
+@pytest.mark.parametrize("gs", [2, 3])
+def test_p2m_m2p_mixed_axes_vs_oracle(core, oracle, dim, bc, kernel, gs):
+    """P2M, and M2P from a random stored box whose ghosts all differ, on a
+    mesh with periodic and open axes, against the oracle.  ghost_get maps a
+    node only when its open coordinates are real nodes 0..N, so ghost nodes
+    beside an open face are read as stored on every axis.  A tenth of the
+    particles sit on the open faces, where the stencils reach those nodes."""
     N = 9
-    g = [2] * dim
-    rng = np.random.default_rng(60 + dim + kernel + sum(bc))
+    g = [gs] * dim
+    rng = np.random.default_rng(60 + dim + kernel + sum(bc) + gs)
     xp = rng.uniform(0, 1, size=(3000, dim))
     for d in range(dim):
         if bc[2 * d] != 1:
@@ -627,9 +628,15 @@ def test_m2p_mixed_axes_vs_oracle(core, oracle, dim, bc, kernel):
     info, ref = oracle.m2p(t, kernel, xp, f, g)
     assert info == 0
     mesh = core.Mesh(dim, N + 1, bcdef=bc)
+    x = _dev(xp, torch.float64)
     up = torch.empty((len(xp), 2), dtype=torch.float64, device="cuda")
-    core.m2p(mesh, _dev(xp, torch.float64), _dev(f, torch.float64), [-2] * dim, up, kernel)
+    core.m2p(mesh, x, _dev(f, torch.float64), [-gs] * dim, up, kernel)
     assert _rel(up.cpu().numpy(), ref) < 1e-12
+    info, fref = oracle.p2m(t, kernel, xp, up.cpu().numpy(), g)
+    assert info == 0
+    field = torch.full(fref.shape, 7.0, dtype=torch.float64, device="cuda")
+    core.p2m(mesh, x, up, field, [-gs] * dim, kernel)
+    assert _rel(field.cpu().numpy(), fref) < 1e-12
 
 
 def test_m2p_compact_layout(core):
