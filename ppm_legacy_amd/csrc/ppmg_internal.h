@@ -124,6 +124,16 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t ntiles) {
     return x * q + (x < r ? x : r) + j;
 }
 
+// ghost_get copies shifted images of the sub's real nodes only
+// (ppm_map_field_ghost_init.f:234-447, ppm_mesh_block_intersect.f:207-279):
+// on a mesh with non-periodic axes it fills a node only when the node's
+// coordinate on every non-periodic axis is a real node 0..N.  Such a node takes
+// the periodic node map on the periodic axes; any other node keeps what is
+// stored there, on every axis.  True when node v of axis d allows the map.
+__device__ __forceinline__ bool real_on_open_axis(const DevMesh& m, int d, int v) {
+    return m.periodic[d] || (v >= 0 && v <= m.N[d]);
+}
+
 // ------------------------------------------------------ DPP row shifts --
 // Move a value K lanes up (row_shr: lane l reads lane l-K) or down (row_shl:
 // lane l reads lane l+K) inside each 16-lane row; bound_ctrl makes lanes whose

@@ -142,16 +142,6 @@ __device__ __forceinline__ int read_index(const DevMesh& m, const FieldGeom& f, 
     return (l >= 0 && l < f.ext[d]) ? l : -1;
 }
 
-// ghost_get copies shifted images of the sub's real nodes only
-// (ppm_map_field_ghost_init.f:234-447, ppm_mesh_block_intersect.f:207-279):
-// on a mesh with non-periodic axes it fills a node only when the node's
-// coordinate on every non-periodic axis is a real node 0..N.  Such a node takes
-// the periodic node map on the periodic axes; any other node keeps what is
-// stored there, on every axis.
-__device__ __forceinline__ bool real_on_open_axis(const DevMesh& m, int d, int v) {
-    return m.periodic[d] || (v >= 0 && v <= m.N[d]);
-}
-
 // M2P gather: one thread per particle (ppm_interp_m2p.f:895-1240, 3245-4106).
 template <int DIM, int KS, typename PosT, typename ValT, int LDA>
 __global__ void __launch_bounds__(256)

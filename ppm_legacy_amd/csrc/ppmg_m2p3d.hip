@@ -63,14 +63,6 @@ __device__ __forceinline__ int stored_node(const DevMesh& m, const FieldGeom& fg
     return v < 0 ? v + N : v;
 }
 
-// Beside an open axis, ghost_get fills a stored node only when the node's
-// coordinate on every open axis is a real node 0..N (its blocks are shifted
-// images of the sub's real nodes, ppm_map_field_ghost_init.f:234-447); any
-// other node keeps what is stored there.
-__device__ __forceinline__ bool m3_real_on_open(const DevMesh& m, int d, int v) {
-    return m.periodic[d] || (v >= 0 && v <= m.N[d]);
-}
-
 // Stored node read as stored: v itself on a stored box (clamped into it for
 // the ring columns no stencil reads); a compact periodic axis wraps.
 template <bool OPEN>
@@ -116,7 +108,7 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         const int gx = stored_node<OPEN>(m, fg, 0, tx0 - 1 + nx) - fg.lo[0];
         const int gy = stored_node<OPEN>(m, fg, 1, ty0 - 1 + ry) - fg.lo[1];
         soff[j] = (unsigned)(((gy * fg.ext[0] + gx) * lda + c0 + c) * (int)sizeof(ValT));
-        if (MIX && !(m3_real_on_open(m, 0, tx0 - 1 + nx) && m3_real_on_open(m, 1, ty0 - 1 + ry)))
+        if (MIX && !(real_on_open_axis(m, 0, tx0 - 1 + nx) && real_on_open_axis(m, 1, ty0 - 1 + ry)))
             soff[j] |= 0x80000000u;
     }
     // MIX: byte offset of element i of a plane read as stored
@@ -132,7 +124,7 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         const int nz = stored_node<OPEN>(m, fg, 2, n) - fg.lo[2];
         const char* base = (const char*)(field + (int64_t)nz * fg.ext[0] * fg.ext[1] * lda);
         if (MIX) {
-            const bool rz = m3_real_on_open(m, 2, n);
+            const bool rz = real_on_open_axis(m, 2, n);
             const int nzr = raw_node<OPEN>(m, fg, 2, n) - fg.lo[2];
             const char* braw = (const char*)(field + (int64_t)nzr * fg.ext[0] * fg.ext[1] * lda);
 #pragma unroll
