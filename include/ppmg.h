@@ -120,10 +120,13 @@ int ppmg_m2p(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
              const ppmg_field* field, void* up, ppmg_plan* plan, void* stream);
 
 /* Adjoint of ppmg_p2m (no reference counterpart).  g = dL/dm on the stored
- * nodes of a layout that stores every unique node once (compact periodic, or
- * any box on non-periodic axes).  grad_up = dL/du (np, lda), grad_xp = dL/dx
- * (np, dim); either may be NULL.  Uses the permutation and cell keys that the
- * last ppmg_p2m on this plan left (same mesh, kernel, np and xp required). */
+ * nodes of any layout ppmg_p2m writes (compact, or a box with ghosts, on
+ * periodic and non-periodic axes): each particle collects g from every stored
+ * node the forward writes it into (a real node and its periodic copies, such
+ * as node N beside node 0, and a ghost node's raw sum).  grad_up = dL/du
+ * (np, lda), grad_xp = dL/dx (np, dim); either may be NULL.  Uses the
+ * permutation and cell keys that the last ppmg_p2m on this plan left (same
+ * mesh, kernel, np and xp required). */
 int ppmg_p2m_adjoint(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void* xp,
                      int lda, const void* up, const ppmg_field* g, void* grad_up, void* grad_xp,
                      ppmg_plan* plan, void* stream);

@@ -352,15 +352,6 @@ int ppmg_p2m_adjoint(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, co
     FieldGeom fg;
     if (make_field(g, m, &fg)) return g_status.info;
     if (fg.lda != lda) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "cotangent lda differs from lda");
-    int nper = 0;
-    for (int d = 0; d < m.dim; ++d) nper += m.periodic[d];
-    if (nper != 0 && nper != m.dim)
-        return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "adjoint supports fully periodic or fully non-periodic meshes");
-    if (nper == m.dim)
-        for (int d = 0; d < m.dim; ++d)
-            if (fg.lo[d] != 0 || fg.ext[d] != m.N[d])
-                return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT,
-                            "periodic adjoint needs the compact cotangent layout (lo = 0, ext = Nm - 1)");
     if (np == 0) return 0;
     mark(plan, as_stream(stream), true);
     // PPMG_GENERAL=1 forces the general per-particle path (used to cross-check the tiled one)

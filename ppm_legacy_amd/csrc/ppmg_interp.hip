@@ -205,9 +205,34 @@ k_m2p_gather(DevMesh m, FieldGeom f, const ValT* __restrict__ field, const PosT*
     for (int c = 0; c < LDA; ++c) u[c] = (ValT)acc[0][c].value();
 }
 
+// Stored indices along axis d of the real nodes that receive stencil node v
+// in the forward P2M: on a periodic axis its image r = v mod N and, for r = 0,
+// node N (both real, both given every periodic image's sum); on an open axis
+// v itself if it is a real node 0..N.  Indices outside the box are dropped.
+__device__ __forceinline__ int real_images(const DevMesh& m, const FieldGeom& f, int d, int v, int (&out)[2]) {
+    int n = 0;
+    if (m.periodic[d]) {
+        const int N = m.N[d];
+        int r = v % N;
+        if (r < 0) r += N;
+        int l = r - f.lo[d];
+        if (l >= 0 && l < f.ext[d]) out[n++] = l;
+        l = N - f.lo[d];
+        if (r == 0 && l >= 0 && l < f.ext[d]) out[n++] = l;
+    } else if (v >= 0 && v <= m.N[d]) {
+        const int l = v - f.lo[d];
+        if (l >= 0 && l < f.ext[d]) out[n++] = l;
+    }
+    return n;
+}
+
 // Adjoint of P2M (no reference counterpart), one thread per particle in the
 // forward's sorted order, reusing its permutation and cell keys:
-//   dL/du_p = sum_n W(x0_p - n) g_n,  dL/dx_p,d = sum_n (g_n . u_p) dW/dx_d.
+//   dL/du_p = sum_n W(x0_p - n) g_n,  dL/dx_p,d = sum_n (g_n . u_p) dW/dx_d,
+// where n runs over the stored nodes the forward writes particle p into: for
+// each stencil node v, every real node congruent to v on the periodic axes
+// (v on the open ones), and v itself where it is a stored node outside the
+// real range (a ghost node keeps the raw sum).
 template <int DIM, int KS, typename PosT, typename ValT, int LDA>
 __global__ void __launch_bounds__(256)
 k_adjoint(DevMesh m, FieldGeom f, const ValT* __restrict__ g, const PosT* __restrict__ xp,
@@ -251,19 +276,50 @@ k_adjoint(DevMesh m, FieldGeom f, const ValT* __restrict__ g, const PosT* __rest
     for (int d = 0; d < 3; ++d) gx[d].init();
     constexpr int KZ = DIM == 3 ? KS : 1;
     for (int oz = 0; oz < KZ; ++oz) {
-        int iz = DIM == 3 ? read_index(m, f, 2, cell[2] - BASE + oz) : 0;
+        const int vz = cell[2] - BASE + oz;
+        int iz[2] = {0, 0};
+        const int nz = DIM == 3 ? real_images(m, f, 2, vz, iz) : 1;
+        const bool rz = DIM == 2 || (vz >= 0 && vz <= m.N[2]);
+        const int lz = DIM == 3 ? vz - f.lo[2] : 0;
         for (int oy = 0; oy < KS; ++oy) {
-            int iy = read_index(m, f, 1, cell[1] - BASE + oy);
+            const int vy = cell[1] - BASE + oy;
+            int iy[2] = {0, 0};
+            const int ny = real_images(m, f, 1, vy, iy);
+            const bool ry = rz && vy >= 0 && vy <= m.N[1];
+            const int ly = vy - f.lo[1];
             for (int ox = 0; ox < KS; ++ox) {
-                int ix = read_index(m, f, 0, cell[0] - BASE + ox);
-                if (ix < 0 || iy < 0 || iz < 0) continue;   // node not an output: zero cotangent
-                const ValT* gn = g + ((int64_t)ix + (int64_t)f.ext[0] * (iy + (int64_t)f.ext[1] * iz)) * f.lda + c0;
+                const int vx = cell[0] - BASE + ox;
+                int ix[2] = {0, 0};
+                const int nx = real_images(m, f, 0, vx, ix);
+                const int lx = vx - f.lo[0];
+                ValT gs[LDA];
+#pragma unroll
+                for (int c = 0; c < LDA; ++c) gs[c] = ValT(0);
+                bool any = false;
+                for (int a2 = 0; a2 < nz; ++a2)
+                    for (int a1 = 0; a1 < ny; ++a1)
+                        for (int a0 = 0; a0 < nx; ++a0) {
+                            const ValT* gn =
+                                g + ((int64_t)ix[a0] + (int64_t)f.ext[0] * (iy[a1] + (int64_t)f.ext[1] * iz[a2])) * f.lda + c0;
+#pragma unroll
+                            for (int c = 0; c < LDA; ++c) gs[c] += gn[c];
+                            any = true;
+                        }
+                // v outside the real range: the ghost node v, if stored, holds its raw sum
+                if (!(ry && vx >= 0 && vx <= m.N[0]) && lx >= 0 && lx < f.ext[0] && ly >= 0 && ly < f.ext[1] &&
+                    lz >= 0 && lz < f.ext[2]) {
+                    const ValT* gn = g + ((int64_t)lx + (int64_t)f.ext[0] * (ly + (int64_t)f.ext[1] * lz)) * f.lda + c0;
+#pragma unroll
+                    for (int c = 0; c < LDA; ++c) gs[c] += gn[c];
+                    any = true;
+                }
+                if (!any) continue;   // p writes no stored node here: zero cotangent
                 ValT w = a[0][ox] * a[1][oy] * a[2][oz];
                 ValT gdu = ValT(0);
 #pragma unroll
                 for (int c = 0; c < LDA; ++c) {
-                    gu[c].add(w * gn[c]);
-                    gdu += gn[c] * uu[c];
+                    gu[c].add(w * gs[c]);
+                    gdu += gs[c] * uu[c];
                 }
                 gx[0].add(gdu * (da[0][ox] * a[1][oy] * a[2][oz]));
                 gx[1].add(gdu * (a[0][ox] * da[1][oy] * a[2][oz]));

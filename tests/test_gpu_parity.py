@@ -896,17 +896,20 @@ def test_remesh_create_part_bc_on_open_axes(core, oracle, bc):
     assert np.array_equal(xd.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("bc", [[1] * 6, [2] * 6, [1, 1, 1, 1, 2, 2], [2, 2, 1, 1, 1, 1]],
+                         ids=["periodic_box", "all_open", "open_z", "open_x"])
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
-def test_adjoint_open_mesh(core, oracle, kernel):
-    """The adjoint on a mesh with every axis open (the cotangent on the stored
-    box, ghosts included), which has no periodic oracle form.  Strength
-    gradient: <P2M(u), g> = <u, dL/du> on every stored node.  Position
+def test_adjoint_stored_boxes(core, oracle, bc, kernel):
+    """The adjoint with the cotangent on the reference's stored box (ghosts
+    and node N included), periodic, open and mixed: each particle collects g
+    from every stored node the forward writes it into.  Strength gradient:
+    <P2M(u), g> = <u, dL/du> over every stored node, and per particle
+    dL/du_p,c = <P2M({x_p}, {e_c}), g> with the oracle's P2M.  Position
     gradient: central differences of the oracle's P2M of one particle,
     L_p(x) = <P2M({x}, {u_p}), g>, with step 1e-6 h (both kernels are C0 at
     worst only on nodes, which random positions miss)."""
     N = 10
-    bc = [2] * 6
-    rng = np.random.default_rng(80 + kernel)
+    rng = np.random.default_rng(80 + kernel + sum(bc))
     xp = rng.uniform(0.05, 0.95, size=(600, 3))
     up = rng.normal(size=(len(xp), 2))
     t = oracle.topo(3, [N + 1] * 3, bcdef=bc)
@@ -920,6 +923,13 @@ def test_adjoint_open_mesh(core, oracle, kernel):
     lhs = float(np.sum(field.cpu().numpy() * gc))
     rhs = float(np.sum(up * gu))
     assert abs(lhs - rhs) <= 1e-12 * np.sum(np.abs(field.cpu().numpy() * gc))
+    for p in range(0, len(xp), 50):
+        for c in range(2):
+            e = np.zeros((1, 2))
+            e[0, c] = 1.0
+            info, fp = oracle.p2m(t, kernel, xp[p:p + 1], e, [2] * 3)
+            assert info == 0
+            assert abs(gu[p, c] - np.sum(fp * gc)) <= 1e-12 * np.max(np.abs(gu)), (p, c)
     h = 1e-6 / N
     scale = np.max(np.abs(gx))
     for p in range(0, len(xp), 25):
