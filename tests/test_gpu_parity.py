@@ -639,6 +639,26 @@ def test_p2m_m2p_mixed_axes_vs_oracle(core, oracle, dim, bc, kernel, gs):
     assert _rel(field.cpu().numpy(), fref) < 1e-12
 
 
+def test_m2p_stencil_outside_stored_field_is_9101(core):
+    """A particle on the upper face of an open axis lies in cell N, and its
+    M'4 stencil reaches node N+2, which a box with ghost size 1 does not
+    store: the reference would read outside field_up, the library counts
+    the read on the device and returns an error.  Interior particles alone
+    read inside the box and succeed."""
+    from ppm_legacy_amd import _lib
+    N = 8
+    mesh = core.Mesh(3, N + 1, bcdef=[1, 1, 1, 1, 2, 2])
+    field = torch.ones(mesh.padded_shape(1, [1] * 3), dtype=torch.float64, device="cuda")
+    inner = torch.tensor([[0.5, 0.5, 0.5], [0.1, 0.9, 0.45]], dtype=torch.float64, device="cuda")
+    up = torch.empty((2, 1), dtype=torch.float64, device="cuda")
+    core.m2p(mesh, inner, field, [-1] * 3, up)
+    assert torch.allclose(up, torch.ones_like(up), rtol=0, atol=1e-14)
+    face = torch.tensor([[0.5, 0.5, 0.5], [0.5, 0.5, 1.0]], dtype=torch.float64, device="cuda")
+    with pytest.raises(_lib.PpmgError) as e:
+        core.m2p(mesh, face, field, [-1] * 3, up)
+    assert e.value.code == _lib.ERR_FIELD_BOUNDS
+
+
 def test_m2p_compact_layout(core):
     N = 16
     rng = np.random.default_rng(9)
