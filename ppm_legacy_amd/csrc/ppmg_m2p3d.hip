@@ -50,8 +50,8 @@ constexpr int M3_THREADS = 16 * M3_TY;  // 256
 // on a periodic axis the compact layout wraps, and a stored box holding node
 // 0 and node N reads with ghost_get's node map (ppm_map_field_ghost_get.f:
 // 257-280: node 0 and the ghosts below it take node v+N, node N and the ghosts
-// above it v-N); an open axis reads node v itself (open axes come here only
-// beside periodic axes in the compact layout).  Stencils reach -1..N+2;
+// above it v-N); an open axis reads node v itself (beside a periodic box the
+// MIX form reads some nodes through raw_node instead).  Stencils reach -1..N+2;
 // the ring of a tile also loads columns beyond that which no stencil reads:
 // they wrap (periodic) or are clamped into the box (open).
 template <bool OPEN>
