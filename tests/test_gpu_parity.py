@@ -919,7 +919,8 @@ def test_remesh_create_part_bc_on_open_axes(core, oracle, bc):
 @pytest.mark.parametrize("bc", [[1] * 6, [2] * 6, [1, 1, 1, 1, 2, 2], [2, 2, 1, 1, 1, 1]],
                          ids=["periodic_box", "all_open", "open_z", "open_x"])
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
-def test_adjoint_stored_boxes(core, oracle, bc, kernel):
+@pytest.mark.parametrize("prec", [F64, F32C])
+def test_adjoint_stored_boxes(core, oracle, bc, kernel, prec):
     """The adjoint with the cotangent on the reference's stored box (ghosts
     and node N included), periodic, open and mixed: each particle collects g
     from every stored node the forward writes it into.  Strength gradient:
@@ -927,29 +928,38 @@ def test_adjoint_stored_boxes(core, oracle, bc, kernel):
     dL/du_p,c = <P2M({x_p}, {e_c}), g> with the oracle's P2M.  Position
     gradient: central differences of the oracle's P2M of one particle,
     L_p(x) = <P2M({x}, {u_p}), g>, with step 1e-6 h (both kernels are C0 at
-    worst only on nodes, which random positions miss)."""
+    worst only on nodes, which random positions miss).  FP32c (values in FP32,
+    positions in FP64) against the same FP64 references on the FP32-rounded
+    strengths and cotangent: 1e-6 for the sums, 1e-5 of the gradient scale
+    for the differences."""
     N = 10
     rng = np.random.default_rng(80 + kernel + sum(bc))
     xp = rng.uniform(0.05, 0.95, size=(600, 3))
     up = rng.normal(size=(len(xp), 2))
+    gc = rng.normal(size=(N + 5,) * 3 + (2,))
+    if prec == F32C:
+        up = up.astype(np.float32).astype(np.float64)
+        gc = gc.astype(np.float32).astype(np.float64)
+    tol_sum, tol_fd = (1e-12, 1e-6) if prec == F64 else (1e-6, 1e-5)
+    vdt = _val_dtype(prec)
     t = oracle.topo(3, [N + 1] * 3, bcdef=bc)
     mesh = core.Mesh(3, N + 1, bcdef=bc)
-    gc = rng.normal(size=mesh.padded_shape(2, [2] * 3))
-    x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
-    field = torch.empty(mesh.padded_shape(2, [2] * 3), dtype=torch.float64, device="cuda")
-    plan = core.p2m(mesh, x, u, field, [-2] * 3, kernel)
-    gu, gx = core.p2m_adjoint(mesh, x, u, _dev(gc, torch.float64), [-2] * 3, plan, kernel)
-    gu, gx = gu.cpu().numpy(), gx.cpu().numpy()
-    lhs = float(np.sum(field.cpu().numpy() * gc))
+    x, u = _dev(xp, torch.float64), _dev(up, vdt)
+    field = torch.empty(mesh.padded_shape(2, [2] * 3), dtype=vdt, device="cuda")
+    plan = core.p2m(mesh, x, u, field, [-2] * 3, kernel, prec)
+    gu, gx = core.p2m_adjoint(mesh, x, u, _dev(gc, vdt), [-2] * 3, plan, kernel, prec)
+    gu, gx = gu.cpu().numpy().astype(np.float64), gx.cpu().numpy().astype(np.float64)
+    fnp = field.cpu().numpy().astype(np.float64)
+    lhs = float(np.sum(fnp * gc))
     rhs = float(np.sum(up * gu))
-    assert abs(lhs - rhs) <= 1e-12 * np.sum(np.abs(field.cpu().numpy() * gc))
+    assert abs(lhs - rhs) <= tol_sum * np.sum(np.abs(fnp * gc))
     for p in range(0, len(xp), 50):
         for c in range(2):
             e = np.zeros((1, 2))
             e[0, c] = 1.0
             info, fp = oracle.p2m(t, kernel, xp[p:p + 1], e, [2] * 3)
             assert info == 0
-            assert abs(gu[p, c] - np.sum(fp * gc)) <= 1e-12 * np.max(np.abs(gu)), (p, c)
+            assert abs(gu[p, c] - np.sum(fp * gc)) <= tol_sum * np.max(np.abs(gu)), (p, c)
     h = 1e-6 / N
     scale = np.max(np.abs(gx))
     for p in range(0, len(xp), 25):
@@ -962,7 +972,7 @@ def test_adjoint_stored_boxes(core, oracle, bc, kernel):
                 assert info == 0
                 xs.append(float(np.sum(fp * gc)))
             fd = (xs[0] - xs[1]) / (2 * h)
-            assert abs(gx[p, d] - fd) <= 1e-6 * scale, (p, d, gx[p, d], fd)
+            assert abs(gx[p, d] - fd) <= tol_fd * scale, (p, d, gx[p, d], fd)
 
 
 def test_impose_part_bc(core, oracle):
