@@ -31,11 +31,21 @@ void clear_error() {
 }
 int debug_level() { return g_debug.load(); }
 
+// Fault injection for the error path of device allocations: with
+// PPMG_FAIL_ALLOC_ABOVE=<bytes> set, every new scratch allocation larger than
+// that many bytes fails as a refused hipMalloc would (read only when a buffer
+// has to grow, so calls that reuse their scratch never look at it).
+static bool injected_alloc_failure(size_t n) {
+    const char* e = std::getenv("PPMG_FAIL_ALLOC_ABOVE");
+    if (!e || !*e) return false;
+    return n > (size_t)std::strtoull(e, nullptr, 10);
+}
+
 int DevBuf::ensure(size_t n) {
     if (n <= bytes && p) return 0;
     release();
     size_t want = n < 256 ? 256 : n;
-    hipError_t e = hipMalloc(&p, want);
+    hipError_t e = injected_alloc_failure(want) ? hipErrorOutOfMemory : hipMalloc(&p, want);
     if (e != hipSuccess) {
         p = nullptr;
         bytes = 0;
@@ -139,6 +149,22 @@ static int make_field(const ppmg_field* in, const DevMesh& m, FieldGeom* f) {
 }
 
 static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Two descriptors that bin and weigh particles identically (bitwise equal
+// origin and inverse spacing).
+static bool same_mesh(const DevMesh& a, const DevMesh& b) {
+    if (a.dim != b.dim) return false;
+    for (int d = 0; d < 3; ++d) {
+        if (a.N[d] != b.N[d] || a.nbin[d] != b.nbin[d] || a.periodic[d] != b.periodic[d] ||
+            a.upper_open[d] != b.upper_open[d])
+            return false;
+        if (std::memcmp(&a.minv[d], &b.minv[d], sizeof(double)) != 0 ||
+            std::memcmp(&a.maxv[d], &b.maxv[d], sizeof(double)) != 0 ||
+            std::memcmp(&a.dxi[d], &b.dxi[d], sizeof(double)) != 0)
+            return false;
+    }
+    return true;
+}
 
 static int check_plan(ppmg_plan* plan) {
     if (!plan) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "plan is NULL");
@@ -349,6 +375,12 @@ int ppmg_p2m_adjoint(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, co
     DevMesh m;
     if (make_mesh(mesh, prec, Spacing::MK, &m)) return g_status.info;
     if (m.dim != plan->dim) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "dimension differs from the forward");
+    // The adjoint decodes the forward's cell keys and heads: the mesh must be
+    // the one the forward binned on (cells, bins, boundary conditions, origin
+    // and spacing), else keys would be read with the wrong stride or past the
+    // heads array.  Checked before any launch.
+    if (!same_mesh(m, plan->mesh))
+        return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "mesh differs from the one the forward binned on");
     FieldGeom fg;
     if (make_field(g, m, &fg)) return g_status.info;
     if (fg.lda != lda) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "cotangent lda differs from lda");

@@ -152,6 +152,22 @@ k_m2p_gather(DevMesh m, FieldGeom f, const ValT* __restrict__ field, const PosT*
     if (p >= np) return;
     constexpr int BASE = KS == 4 ? 1 : 0;
     const PosT* x = xp + p * DIM;
+    // A position that is not a number, or so far away that its node index
+    // does not fit an int, reads no node: it is counted with the particles
+    // whose stencil leaves the stored field, and its values are NaN.
+    bool sane = true;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const PosT x0 = scaled_coord<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d]);
+        sane = sane && x0 > PosT(-1.0e9) && x0 < PosT(1.0e9);
+    }
+    if (!sane) {
+        atomicAdd(err, 1);
+        ValT* u = up + p * lda + c0;
+#pragma unroll
+        for (int c = 0; c < LDA; ++c) u[c] = (ValT)__builtin_nan("");
+        return;
+    }
     ValT a[3][KS];
     int nb[3] = {0, 0, 0};
 #pragma unroll

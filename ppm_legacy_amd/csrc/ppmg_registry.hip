@@ -268,6 +268,18 @@ int ppmg_rmsh_create_part_reg(const void* field_up, int lda, int topo_id, int me
 int ppmg_interp_p2m(const void* xp, int np, const void* up, int lda, int topo_id, int mesh_id, int kernel,
                     const int* ghostsize, void* field_up, int prec, int mem) {
     clear_error();
+    // ppm_interp_p2m.f:338-367: with ppm_debug > 0 the initialisation and Np
+    // checks come first, and Np == 0 returns before field_up is touched (no
+    // zeroing, no staging, no copy back).
+    if (debug_level() > 0) {
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            if (!g_initialized) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_NOINIT, "Please call ppm_init first!");
+        }
+        if (np < 0) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "particles must be specified");
+        if (np == 0) return 0;
+        if (kernel < 1 || kernel > 4) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "wrong kernel definition");
+    }
     Topo t;
     if (lookup(topo_id, mesh_id, &t)) return current_info();
     if (!ghostsize) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "ghostsize is NULL");
@@ -380,6 +392,8 @@ int ppmg_rmsh_remesh_reg(const void* xp, int np, const void* up, int lda, int to
     clear_error();
     Topo t;
     if (lookup(topo_id, mesh_id, &t)) return current_info();
+    if (!ghostsize) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "ghostsize is NULL");
+    if (np < 0 || lda < 1) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "Np must be >= 0 and lda >= 1");
     int dim = t.mesh.dim;
     const int ks = kernel == PPMG_KERNEL_MP4 ? 2 : 1;
     int g[3] = {0, 0, 0};

@@ -349,6 +349,75 @@ def test_create_part_band_and_positions(oracle):
     assert np.array_equal(xp, np.stack([kx, ky, kz], 1) * (1.0 / N))
 
 
+def test_create_part_weighted_norm_closed_form(oracle):
+    """With cutoff_weights the band test is sqrt(sum_c (u_c w_c)^2)
+    (ppm_rmsh_create_part.f:466-472).  Nodes are built so the weighted norm is
+    known by hand and a plausible mistake changes the selection:
+      A (0.4, 0, 0)       weighted 0.8 in band, unweighted 0.4 out;
+      B (1.5, 0, 0)       weighted 3.0 out, unweighted 1.5 in;
+      C (0.3, 1.6, 7)     weighted (0.6, 0.8, 0): 1.0 in (3-4-5), unweighted 7.2 out;
+      D (0.3, -1.6, 0)    weighted 1.0 in; |sum| before squaring 0.2 would be out;
+      E (0, 0, 10)        weight 0: out; a dropped weight would leave 10, out too,
+      F (0, 0, 1)         weight 0: out; a dropped weight leaves 1.0, in;
+      G (0, 3.9, 0)       weighted 1.95 in; squaring the weight twice (0.975) in,
+      H (0, 4.2, 0)       weighted 2.1 out; weight squared (1.05) would be in."""
+    N = 5
+    g = [1, 1, 1]
+    t = oracle.topo(3, [N + 1] * 3)
+    f = np.zeros(oracle.field_shape(t, 3, g))
+    cw = np.array([2.0, 0.5, 0.0])
+    nodes = {  # (i, j, k) real node (1-based as in the reference) -> strengths
+        (1, 1, 1): (0.4, 0, 0), (2, 1, 1): (1.5, 0, 0), (3, 2, 1): (0.3, 1.6, 7.0),
+        (4, 4, 2): (0.3, -1.6, 0), (5, 1, 3): (0, 0, 10.0), (1, 5, 4): (0, 0, 1.0),
+        (2, 3, 5): (0, 3.9, 0), (3, 3, 5): (0, 4.2, 0)}
+    for (i, j, k), v in nodes.items():
+        f[k - 1 + g[2], j - 1 + g[1], i - 1 + g[0]] = v      # stored box starts at node 1-g
+    cut = [0.5, 2.0]
+    info, xp, up = oracle.create_part(t, f, g, cut, cutoff_weights=cw)
+    assert info == 0
+    chosen = [(1, 1, 1), (3, 2, 1), (4, 4, 2), (2, 3, 5)]        # A, C, D, G in node order (x fastest)
+    assert np.array_equal(up, np.array([nodes[n] for n in chosen]))
+    assert np.array_equal(xp, np.array([[i - 1, j - 1, k - 1] for i, j, k in chosen]) * (1.0 / N))
+    # unweighted: B (1.5), D (|(0.3, -1.6, 0)| = 1.63) and F (1.0) are in band
+    info, _, up = oracle.create_part(t, f, g, cut)
+    assert np.array_equal(up, np.array([nodes[(2, 1, 1)], nodes[(4, 4, 2)], nodes[(1, 5, 4)]]))
+
+
+def test_bsp2_position_gradient_fd(oracle):
+    """BSP2 is piecewise linear: with the particle's cell fixed, every node
+    value is linear in its position and L = 0.5 sum w m^2 is quadratic, so
+    central differences are exact up to rounding.  Particles keep at least
+    0.1 h from every node plane so no step crosses a kink."""
+    N = 8
+    t = oracle.topo(3, [N + 1] * 3)
+    rng = np.random.default_rng(71)
+    cell = rng.integers(0, N, size=(30, 3))
+    xp = (cell + rng.uniform(0.1, 0.9, size=(30, 3))) / N
+    up = rng.normal(size=(30, 2))
+    wn = rng.normal(size=(N, N, N, 2))
+
+    def loss(x):
+        _, f = oracle.p2m(t, BSP2, x, up, [2, 2, 2])
+        m = oracle.unique_nodes(f, [2, 2, 2], 3)
+        return 0.5 * (wn * m * m).sum(), m
+
+    _, m = loss(xp)
+    info, gu, gx = oracle.p2m_adjoint(t, BSP2, xp, up, wn * m)
+    assert info == 0
+    step = 1e-6 / N
+    scale = np.abs(gx).max()
+    for p in range(30):
+        for d in range(3):
+            xa = xp.copy(); xa[p, d] += step
+            xb = xp.copy(); xb[p, d] -= step
+            fd = (loss(xa)[0] - loss(xb)[0]) / (2 * step)
+            assert abs(fd - gx[p, d]) <= 1e-7 * scale
+    # a sign or axis mistake in the derivative stencil would also break the
+    # dot product with a displacement: dL ~ <gx, dx> for a small random dx
+    dx = rng.normal(size=xp.shape) * step
+    assert np.isclose(loss(xp + dx)[0] - loss(xp - dx)[0], 2 * (gx * dx).sum(), rtol=1e-6)
+
+
 # ------------------------------------------------------------ impose bc ----
 def test_impose_part_bc(oracle):
     t = oracle.topo(3, [9, 9, 9], [-1.0, 0.0, 0.0], [1.0, 1.0, 2.0])
