@@ -111,6 +111,24 @@ int ppmg_plan_destroy(ppmg_plan* plan);
 int ppmg_p2m(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void* xp, int lda,
              const void* up, const ppmg_field* field, ppmg_plan* plan, void* stream);
 
+/* Allocation-free P2M.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and results as ppmg_p2m, without a plan: the binning scratch comes from one
+ * fixed device array of PPMG_NOALLOC_SCRATCH_BYTES per GPU (held until the
+ * process ends), and the call makes no device allocation or free.  The 3D
+ * M'4 tiled path needs about 17 bytes per particle plus 8 per cell (2M
+ * particles on a 128^3 mesh fit); the general path (BSP2, 2D, other layouts)
+ * adds its sorted weights and strengths.  Inputs that cannot fit are rejected
+ * before any launch (PPMG_ERR_ARGUMENT), a shortfall found later fails with
+ * PPMG_ERR_ALLOC before the field is written.  Calls on one GPU share the
+ * array and a call returns before its kernels finish, so calls that overlap
+ * on one GPU race: issue them on one stream, or synchronise between them.
+ * There is no forward record for ppmg_p2m_adjoint. */
+#define PPMG_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ppmg_p2m_noalloc(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void* xp, int lda,
+                     const void* up, const ppmg_field* field, void* stream);
+
 /* M2P: replaces ppm_interp_m2p_{s,d}{s,v}_{2d,3d} (ppm_interp_m2p.f:126-161,
  * generic ppm_module_interp_m2p.f:46-59) preceded by the caller's ghost_get
  * (ppm_map_field_ghost.f:627-669): periodic images are read in-kernel with

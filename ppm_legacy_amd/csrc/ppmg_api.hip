@@ -41,10 +41,28 @@ static bool injected_alloc_failure(size_t n) {
     return n > (size_t)std::strtoull(e, nullptr, 10);
 }
 
+static thread_local Arena* g_arena = nullptr;
+void bind_arena(Arena* a) { g_arena = a; }
+
 int DevBuf::ensure(size_t n) {
     if (n <= bytes && p) return 0;
     release();
     size_t want = n < 256 ? 256 : n;
+    if (Arena* a = g_arena) {
+        want = (want + 255) & ~(size_t)255;
+        if (want > a->cap - a->used) {
+            set_error(PPMG_ERROR_FATAL, PPMG_ERR_ALLOC,
+                      "allocation-free call: its scratch needs more than the fixed " + std::to_string(a->cap) +
+                          " bytes (" + std::to_string(a->used) + " used, " + std::to_string(want) +
+                          " more requested)");
+            return PPMG_ERROR_FATAL;
+        }
+        p = a->base + a->used;
+        a->used += want;
+        bytes = want;
+        borrowed = true;
+        return 0;
+    }
     hipError_t e = injected_alloc_failure(want) ? hipErrorOutOfMemory : hipMalloc(&p, want);
     if (e != hipSuccess) {
         p = nullptr;
@@ -56,9 +74,10 @@ int DevBuf::ensure(size_t n) {
     return 0;
 }
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    borrowed = false;
 }
 
 static int fail(int severity, int code, const std::string& msg) {

@@ -254,10 +254,22 @@ int debug_level();
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;   // carved from a bound Arena: never freed
     int ensure(size_t n);
     void release();
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
+
+// Fixed device memory of an allocation-free call.  While an arena is bound on
+// the calling thread, DevBuf::ensure carves buffers from it in bump order
+// (256-byte aligned) instead of calling hipMalloc, and a request past its end
+// fails with PPMG_ERR_ALLOC.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+};
+void bind_arena(Arena* a);   // nullptr unbinds
 
 }  // namespace ppmg
 
