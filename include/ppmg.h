@@ -218,6 +218,9 @@ int ppmg_plan_timings(ppmg_plan* plan, double* ms, int n);
  * the plan: key[np] (cell of each particle, x fastest: i + nbin_x*(j + nbin_y*k),
  * nbin = Nm-1 on periodic axes, Nm otherwise), perm[np] (particle indices in
  * stable cell order, 0-based) and start[ncells+1] (cell heads, 0-based).
+ * Input already in cell order is binned without writing keys or the identity
+ * permutation; this call then computes them, reading the positions that
+ * binning was given, which must still hold them.
  * The ppm_util_rank3d analogue of these is ppmg_util_rank (1-based). */
 int ppmg_plan_sort_view(const ppmg_plan* plan, const int** key, const int** perm, const int** start,
                         int64_t* np, int64_t* ncells);

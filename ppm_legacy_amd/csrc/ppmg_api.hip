@@ -289,6 +289,12 @@ int ppmg_plan_sort_view(const ppmg_plan* plan, const int** key, const int** perm
         if (materialize_perm(w, w->bin_stream)) return g_status.info;
         PPMG_HIP(hipStreamSynchronize(w->bin_stream));
     }
+    if (key && !plan->keys_written) {
+        // input binned in cell order left the keys unwritten: compute them now
+        ppmg_plan* w = const_cast<ppmg_plan*>(plan);
+        if (materialize_keys(w, w->bin_stream)) return g_status.info;
+        PPMG_HIP(hipStreamSynchronize(w->bin_stream));
+    }
     if (key) *key = plan->key.as<int>();
     if (perm) *perm = plan->perm.as<int>();
     if (start) *start = plan->start.as<int>();

@@ -161,6 +161,26 @@ __device__ __forceinline__ float row_shl(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + K, 0xF, 0xF, true));
 }
 
+// Exclusive prefix sum of v over the 64 lanes of a wavefront (lane = this
+// lane's index).
+__device__ __forceinline__ int wave_excl_prefix(int v, int lane) {
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x - v;
+}
+
+// Orders one wavefront's LDS writes before its later LDS reads by other lanes
+// of the same wavefront (a wavefront's LDS instructions execute in order; this
+// keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // ---------------------------------------------------------------- weights --
 // M'4 on the nodes INT(x0)-1 .. INT(x0)+2 (0-based) in the 3D unrolled form
 // of ppm_interp_p2m.f:824-853 (same polynomials as ppm_interp_m2p.f).
@@ -282,6 +302,10 @@ struct ppmg_plan {
     bool perm_identity = false;  // the permutation of the last binning is the identity ...
     bool perm_written = true;    // ... and has (or has not yet) been written into `perm`
     hipStream_t bin_stream = nullptr;   // stream of the last binning
+    const void* bin_xp = nullptr;       // positions, precision and mesh of the last binning
+    int bin_prec = 0;
+    ppmg::DevMesh bin_mesh{};
+    bool keys_written = true;    // false: the last binning (cell-ordered input) wrote no keys
     int dim = 0, kernel = 0, prec = 0;
     ppmg::DevMesh mesh{};
     bool have_forward = false;

@@ -571,7 +571,7 @@ static int adjoint_typed(ppmg_plan* plan, const DevMesh& m, int kernel, int64_t 
     const int KS = kernel == PPMG_KERNEL_MP4 ? 4 : 2;
     if (np <= 0) return 0;
     unsigned gr = nblk(np, 256);
-    if (materialize_perm(plan, s)) return PPMG_ERROR_FATAL;
+    if (materialize_perm(plan, s) || materialize_keys(plan, s)) return PPMG_ERROR_FATAL;
     const int* perm = plan->perm.as<int>();
     const int* key = plan->key.as<int>();
     for (int c0 = 0; c0 < lda; c0 += 4) {

@@ -25,6 +25,10 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
 // materialize_perm() first.
 inline const int* tile_perm(const ppmg_plan* plan) { return plan->perm_identity ? nullptr : plan->perm.as<int>(); }
 int materialize_perm(ppmg_plan* plan, hipStream_t s);
+// Input binned in cell order leaves the keys unwritten too; readers of
+// plan->key call materialize_keys() first (it reads the positions of that
+// binning, which must still hold them).
+int materialize_keys(ppmg_plan* plan, hipStream_t s);
 int rank_particles(ppmg_plan* plan, const RankGeom& g, int prec, int64_t np, const void* xp,
                    int* lpdx, int* lhbx, int64_t* icount, hipStream_t s);
 // xp (np rows of dim) reordered by the 1-based ranking lpdx1; npbx (may be

@@ -173,6 +173,89 @@ int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, int64_t tota
     return 0;
 }
 
+// Byte-wide cell counts of the binning.  In every block of 256 cells, cell c
+// sits in word c mod 64, byte (c / 64) mod 4: the 64 consecutive cells a
+// wavefront of lattice-ordered particles hits fall in 64 different words, so
+// the atomic counts of neighbouring lanes do not contend for one word.  The
+// array is padded to whole blocks and zeroed, so reads need no bounds.
+__device__ __forceinline__ int64_t count_word(int64_t c) { return ((c >> 8) << 6) + (c & 63); }
+__device__ __forceinline__ unsigned count_shift(int64_t c) { return (unsigned)((c >> 6) & 3) * 8u; }
+
+// Counts of the 16 cells base .. base+15 (base a multiple of 16): 16
+// consecutive words of one block, one byte of each.
+__device__ __forceinline__ void load_counts16(const uint32_t* __restrict__ words, int64_t base, int (&c)[16]) {
+    const uint4* q = reinterpret_cast<const uint4*>(words + count_word(base));
+    const unsigned sh = count_shift(base);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint4 v = q[k];
+        c[4 * k] = (int)((v.x >> sh) & 255u);
+        c[4 * k + 1] = (int)((v.y >> sh) & 255u);
+        c[4 * k + 2] = (int)((v.z >> sh) & 255u);
+        c[4 * k + 3] = (int)((v.w >> sh) & 255u);
+    }
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce_u8(const uint32_t* __restrict__ in,
+                                                               int64_t* __restrict__ block_sums) {
+    __shared__ int64_t sh[SCAN_BLOCK / 64 + 1];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int c[SCAN_ITEMS];
+    load_counts16(in, base, c);
+    int sum = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) sum += c[i];
+    int64_t excl;
+    const int64_t tot = block_exclusive_scan(sum, sh, excl);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_down_u8(const uint32_t* __restrict__ in, int64_t n,
+                                                             const int64_t* __restrict__ offs, int* __restrict__ out,
+                                                             int64_t total_out) {
+    __shared__ int64_t sh[SCAN_BLOCK / 64 + 1];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int c[SCAN_ITEMS];
+    load_counts16(in, base, c);
+    int sum = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) sum += c[i];
+    int64_t excl;
+    block_exclusive_scan(sum, sh, excl);
+    int run = (int)(offs[blockIdx.x] + excl);
+    int o[SCAN_ITEMS];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        o[i] = run;
+        run += c[i];
+    }
+    if (base + SCAN_ITEMS <= n) {
+        int4* q = reinterpret_cast<int4*>(out + base);
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS / 4; ++k) q[k] = make_int4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i)
+            if (base + i < n) out[base + i] = o[i];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_BLOCK - 1) out[n] = (int)total_out;
+}
+
+// out (16-byte aligned, n + 1 entries) = exclusive prefix of the byte counts
+// of cells 0..n-1 (laid out as count_word / count_shift, padded to whole
+// 256-cell blocks), then the total.
+static int exclusive_scan_u8(const uint32_t* in, int64_t n, int* out, DevBuf& tmp, int64_t total, hipStream_t s) {
+    int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb < 1) nb = 1;
+    if (tmp.ensure(sizeof(int64_t) * (size_t)nb)) return PPMG_ERROR_FATAL;
+    int64_t* sums = tmp.as<int64_t>();
+    hipLaunchKernelGGL(k_scan_reduce_u8, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SUMS_THREADS), 0, s, sums, nb);
+    hipLaunchKernelGGL(k_scan_down_u8, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, s, in, n, sums, out, total);
+    PPMG_HIP(hipGetLastError());
+    return 0;
+}
+
 // ------------------------------------------------------------ binning ---
 // Cell key of one particle with the index arithmetic of ppm_interp_p2m.f:
 // 800-822 (x0 = (x - min)*dxi, INT truncation) and the sub test of :535-561;
@@ -224,18 +307,18 @@ __device__ __forceinline__ void set_heads(int* __restrict__ start, int lo, int h
 // from the previous particle's position, so descents across wavefront seams
 // are caught here too.  The flag is read before it is written, so the many
 // wavefronts of unsorted input do not queue on one word.
-//   SLOT (unsorted input expected): take each particle's arrival slot in its
-//     cell by an atomic count.
+//   SLOT (unsorted input expected): write the keys and take each particle's
+//     arrival slot in its cell by an atomic count (byte-wide: slots 0..254).
 //   !SLOT (input expected in cell order): the first particle of each run of
 //     equal keys writes its index as the head of its cell and of the empty
 //     cells before it (back to its predecessor's cell); the last particle
-//     closes the array with np.  No counts, no scan, and the permutation is
-//     the identity.  Valid only if no descent is found (else the caller counts
-//     from the keys).
+//     closes the array with np.  No counts, no scan, no keys written, and the
+//     permutation is the identity.  Valid only if no descent is found (else
+//     the caller bins again on the SLOT path).
 template <int DIM, typename PosT, bool SLOT>
 __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64_t np, DevMesh m, int64_t ncells,
-                                              int* __restrict__ key, int* __restrict__ slot,
-                                              int* __restrict__ count, int* __restrict__ start,
+                                              int* __restrict__ key, uint8_t* __restrict__ slot,
+                                              uint32_t* __restrict__ count, int* __restrict__ start,
                                               int* __restrict__ err, int4* __restrict__ gaps, int gapcap,
                                               int* __restrict__ multi, int* __restrict__ multi_n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -274,10 +357,16 @@ __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64
         if (live) {
             key[p] = k;
             if (k >= 0) {
-                sl = atomicAdd(&count[k], 1);
-                slot[p] = sl;
+                // byte-wide counts (count_word / count_shift)
+                const unsigned sh = count_shift(k);
+                sl = (int)((atomicAdd(&count[count_word(k)], 1u << sh) >> sh) & 255u);
+                slot[p] = (uint8_t)sl;
             }
         }
+        // a cell's 256th arrival carries into the next cell's byte: err[3]
+        // sends the caller to the full-width count
+        const uint64_t ov = __ballot(sl == 255);
+        if (ov && lane == 0) err[3] = 1;
         __syncthreads();
         if (sl == 1) listed[atomicAdd(&nlisted, 1)] = k;
         __syncthreads();
@@ -286,9 +375,7 @@ __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64
         if (threadIdx.x == 0) multi_n[blockIdx.x] = n;
         return;
     }
-    if (!live) return;
-    key[p] = k;
-    if (k < 0 || desc) return;
+    if (!live || k < 0 || desc) return;
     if (k != left) set_heads(start, left + 1, k, (int)p, err, gaps, gapcap);
     if (p == np - 1) set_heads(start, k + 1, (int)ncells, (int)np, err, gaps, gapcap);
 }
@@ -301,7 +388,8 @@ __global__ void __launch_bounds__(256) k_fill_gaps(const int4* __restrict__ gaps
     }
 }
 
-// Unsorted input: arrival slot of every particle in its cell.
+// Full-width arrival slots of every particle in its cell (a cell with more
+// particles than a byte counts).
 __global__ void __launch_bounds__(256) k_count(const int* __restrict__ key, int64_t np, int* __restrict__ slot,
                                                int* __restrict__ count) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -353,9 +441,11 @@ __global__ void __launch_bounds__(256) k_rank_keys(const PosT* __restrict__ xp, 
     slot[p] = atomicAdd(&count[k], 1);
 }
 
-// Four particles per thread: keys and slots as 128-bit loads (plan buffers
-// are 256-byte aligned), then the four head loads in flight together.
-__global__ void __launch_bounds__(256) k_scatter(const int* __restrict__ key, const int* __restrict__ slot,
+// Four particles per thread: keys and slots as vector loads (plan buffers
+// are 256-byte aligned), then the four head loads in flight together.  SlotT:
+// uint8_t (byte-wide counting) or int (full width).
+template <typename SlotT>
+__global__ void __launch_bounds__(256) k_scatter(const int* __restrict__ key, const SlotT* __restrict__ slot,
                                                  const int* __restrict__ start, int64_t np,
                                                  int* __restrict__ perm) {
     const int64_t p0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
@@ -363,14 +453,20 @@ __global__ void __launch_bounds__(256) k_scatter(const int* __restrict__ key, co
     int k[4], sl[4];
     if (p0 + 4 <= np) {
         const int4 kv = *reinterpret_cast<const int4*>(key + p0);
-        const int4 sv = *reinterpret_cast<const int4*>(slot + p0);
         k[0] = kv.x; k[1] = kv.y; k[2] = kv.z; k[3] = kv.w;
-        sl[0] = sv.x; sl[1] = sv.y; sl[2] = sv.z; sl[3] = sv.w;
+        if constexpr (sizeof(SlotT) == 1) {
+            const uint32_t sv = *reinterpret_cast<const uint32_t*>(slot + p0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sl[i] = (int)((sv >> (8 * i)) & 255u);
+        } else {
+            const int4 sv = *reinterpret_cast<const int4*>(slot + p0);
+            sl[0] = sv.x; sl[1] = sv.y; sl[2] = sv.z; sl[3] = sv.w;
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             k[i] = p0 + i < np ? key[p0 + i] : -1;
-            sl[i] = p0 + i < np ? slot[p0 + i] : 0;
+            sl[i] = p0 + i < np ? (int)slot[p0 + i] : 0;
         }
     }
     int h[4];
@@ -453,14 +549,28 @@ int fill_zero(void* p, size_t bytes, hipStream_t s) {
     return 0;
 }
 
+// Keys of a binning that took the cell-order path (which writes none).
+template <int DIM, typename PosT>
+__global__ void __launch_bounds__(256) k_keys_only(const PosT* __restrict__ xp, int64_t np, DevMesh m,
+                                                   int* __restrict__ key) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    PosT xv[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) xv[d] = xp[p * DIM + d];
+    key[p] = cell_key<DIM, PosT>(xv, m);
+}
+
 int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp,
                   hipStream_t s, int64_t* nbad) {
-    int64_t ncells = (int64_t)m.nbin[0] * m.nbin[1] * m.nbin[2];
+    const int64_t ncells = (int64_t)m.nbin[0] * m.nbin[1] * m.nbin[2];
     const int64_t gapcap = np / 16 + 4096;
-    if (plan->key.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
-        plan->slot.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
-        plan->perm.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
-        plan->count.ensure(sizeof(int) * (size_t)ncells) ||
+    const size_t npb = (size_t)(np > 0 ? np : 1);
+    // byte-wide counts (count_word) come in whole blocks of 256 cells, and
+    // the scan reads whole tiles of SCAN_TILE cells
+    const size_t count_alloc = ((size_t)ncells + SCAN_TILE - 1) / SCAN_TILE * SCAN_TILE;
+    if (plan->key.ensure(sizeof(int) * npb) || plan->slot.ensure(npb + 16) ||
+        plan->perm.ensure(sizeof(int) * npb) || plan->count.ensure(count_alloc) ||
         plan->start.ensure(sizeof(int) * (size_t)(ncells + 1)) ||
         plan->gaps.ensure(sizeof(int4) * (size_t)gapcap) ||
         plan->multi.ensure(sizeof(int) * (size_t)nblk(np, 256) * 256) ||
@@ -470,6 +580,10 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     plan->bin_stream = s;
     plan->np = np;
     plan->ncells = ncells;
+    plan->bin_xp = xp;
+    plan->bin_prec = prec;
+    plan->bin_mesh = m;
+    plan->keys_written = true;
     *nbad = 0;
     if (np == 0) {
         plan->perm_identity = true;
@@ -481,10 +595,10 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     // the same kind of particles each call (advected particles, or particles
     // fresh from create_part) predicts right and does each step once.
     const bool predict_sorted = plan->sorted_input;
-    if (!predict_sorted && fill_zero(plan->count.p, sizeof(int) * (size_t)ncells, s)) return PPMG_ERROR_FATAL;
+    if (!predict_sorted && fill_zero(plan->count.p, count_alloc, s)) return PPMG_ERROR_FATAL;
     int* key = plan->key.as<int>();
-    int* slot = plan->slot.as<int>();
-    int* count = plan->count.as<int>();
+    uint8_t* slot8 = plan->slot.as<uint8_t>();
+    uint32_t* count8 = plan->count.as<uint32_t>();
     int* start = plan->start.as<int>();
     int* err = plan->err.as<int>();
     int4* gaps = plan->gaps.as<int4>();
@@ -493,7 +607,7 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     {
         const unsigned g = nblk(np, 256);
 #define KEYS(D, T, S) hipLaunchKernelGGL((k_keys<D, T, S>), dim3(g), dim3(256), 0, s, (const T*)xp, np, m, ncells, key, \
-                                         slot, count, start, err, gaps, (int)gapcap, multi, multi_n)
+                                         slot8, count8, start, err, gaps, (int)gapcap, multi, multi_n)
         const bool single = prec == PPMG_PREC_F32;
         if (m.dim == 3) {
             if (single) { if (predict_sorted) KEYS(3, float, false); else KEYS(3, float, true); }
@@ -505,42 +619,81 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
 #undef KEYS
         PPMG_HIP(hipGetLastError());
     }
-    int herr[3] = {0, 0, 0};
-    PPMG_HIP(hipMemcpyAsync(herr, plan->err.p, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
+    int herr[4] = {0, 0, 0, 0};
+    PPMG_HIP(hipMemcpyAsync(herr, plan->err.p, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
     PPMG_HIP(hipStreamSynchronize(s));
     *nbad = herr[0];
     if (herr[0]) return 0;
     const bool sorted = herr[1] == 0;
-    if (predict_sorted && sorted && herr[2] <= gapcap) {
-        // the heads are written but for the queued runs of empty cells
-        if (herr[2] > 0)
-            hipLaunchKernelGGL(k_fill_gaps, dim3((unsigned)(herr[2] < 4096 ? herr[2] : 4096)), dim3(256), 0, s, gaps,
-                               herr[2], start);
-    } else {
-        if (predict_sorted) {
-            // wrong prediction (or too many long gaps): count from the keys
-            if (fill_zero(plan->count.p, sizeof(int) * (size_t)ncells, s)) return PPMG_ERROR_FATAL;
-            hipLaunchKernelGGL(k_count, dim3(nblk(np, 256)), dim3(256), 0, s, key, np, slot, count);
+    if (predict_sorted) {
+        if (sorted && herr[2] <= gapcap) {
+            // the heads are written but for the queued runs of empty cells
+            if (herr[2] > 0)
+                hipLaunchKernelGGL(k_fill_gaps, dim3((unsigned)(herr[2] < 4096 ? herr[2] : 4096)), dim3(256), 0, s,
+                                   gaps, herr[2], start);
+            PPMG_HIP(hipGetLastError());
+            // input in cell order: identity permutation and keys, both written only on demand
+            plan->perm_identity = true;
+            plan->perm_written = false;
+            plan->keys_written = false;
+            return 0;
         }
-        if (exclusive_scan(count, ncells, start, plan->scan_tmp, np, s)) return PPMG_ERROR_FATAL;
-        if (!sorted) {
-            hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 1024)), dim3(256), 0, s, key, slot, start, np,
+        // wrong prediction (or more long gaps than the queue holds): bin
+        // again on the counting path
+        plan->sorted_input = false;
+        return bin_particles(plan, m, prec, np, xp, s, nbad);
+    }
+    if (!herr[3]) {
+        if (exclusive_scan_u8(count8, ncells, start, plan->scan_tmp, np, s)) return PPMG_ERROR_FATAL;
+        if (!sorted)
+            hipLaunchKernelGGL(k_scatter<uint8_t>, dim3(nblk(np, 1024)), dim3(256), 0, s, key, slot8, start, np,
                                plan->perm.as<int>());
-            // the key pass listed the cells with two or more particles, unless a
-            // wrong prediction sent it down the cell-order path
-            if (predict_sorted)
-                hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, start, ncells,
-                                   plan->perm.as<int>());
-            else
-                hipLaunchKernelGGL(k_stabilize_listed, dim3(nblk(nblk(np, 256), STAB_GROUP)), dim3(64), 0, s, start,
-                                   multi, multi_n, (int64_t)nblk(np, 256), plan->perm.as<int>());
-        }
+    } else {
+        // a cell holds more particles than a byte counts: count again at full width
+        if (plan->slot.ensure(sizeof(int) * npb) || plan->count.ensure(sizeof(int) * (size_t)ncells))
+            return PPMG_ERROR_FATAL;
+        int* slot32 = plan->slot.as<int>();
+        int* count32 = plan->count.as<int>();
+        if (fill_zero(count32, sizeof(int) * (size_t)ncells, s)) return PPMG_ERROR_FATAL;
+        hipLaunchKernelGGL(k_count, dim3(nblk(np, 256)), dim3(256), 0, s, key, np, slot32, count32);
+        if (exclusive_scan(count32, ncells, start, plan->scan_tmp, np, s)) return PPMG_ERROR_FATAL;
+        if (!sorted)
+            hipLaunchKernelGGL(k_scatter<int>, dim3(nblk(np, 1024)), dim3(256), 0, s, key, slot32, start, np,
+                               plan->perm.as<int>());
+    }
+    if (!sorted) {
+        // the key pass listed the cells whose second particle it saw; after a
+        // byte overflow that list may miss cells, so every cell is sorted
+        if (herr[3])
+            hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, start, ncells,
+                               plan->perm.as<int>());
+        else
+            hipLaunchKernelGGL(k_stabilize_listed, dim3(nblk(nblk(np, 256), STAB_GROUP)), dim3(64), 0, s, start,
+                               multi, multi_n, (int64_t)nblk(np, 256), plan->perm.as<int>());
     }
     PPMG_HIP(hipGetLastError());
     // input in cell order: the permutation is the identity, written only on demand
     plan->perm_identity = sorted;
     plan->perm_written = !sorted;
     plan->sorted_input = sorted;
+    return 0;
+}
+
+int materialize_keys(ppmg_plan* plan, hipStream_t s) {
+    if (!plan->keys_written && plan->np > 0) {
+        const unsigned g = nblk(plan->np, 256);
+        const DevMesh& m = plan->bin_mesh;
+        int* key = plan->key.as<int>();
+        if (plan->bin_prec == PPMG_PREC_F32) {
+            if (m.dim == 3) hipLaunchKernelGGL((k_keys_only<3, float>), dim3(g), dim3(256), 0, s, (const float*)plan->bin_xp, plan->np, m, key);
+            else hipLaunchKernelGGL((k_keys_only<2, float>), dim3(g), dim3(256), 0, s, (const float*)plan->bin_xp, plan->np, m, key);
+        } else {
+            if (m.dim == 3) hipLaunchKernelGGL((k_keys_only<3, double>), dim3(g), dim3(256), 0, s, (const double*)plan->bin_xp, plan->np, m, key);
+            else hipLaunchKernelGGL((k_keys_only<2, double>), dim3(g), dim3(256), 0, s, (const double*)plan->bin_xp, plan->np, m, key);
+        }
+        PPMG_HIP(hipGetLastError());
+    }
+    plan->keys_written = true;
     return 0;
 }
 
@@ -623,7 +776,7 @@ int rank_particles(ppmg_plan* plan, const RankGeom& g, int prec, int64_t np, con
     int64_t inside = np - stats[0];
     if (exclusive_scan(plan->count.as<int>(), ncells, lhbx, plan->scan_tmp, inside, s)) return PPMG_ERROR_FATAL;
     if (np > 0) {
-        hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 1024)), dim3(256), 0, s, plan->key.as<int>(),
+        hipLaunchKernelGGL(k_scatter<int>, dim3(nblk(np, 1024)), dim3(256), 0, s, plan->key.as<int>(),
                            plan->slot.as<int>(), lhbx, np, lpdx);
         hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, lhbx, ncells, lpdx);
         hipLaunchKernelGGL(k_to_one_based, dim3(nblk(inside, 256)), dim3(256), 0, s, lpdx, inside);
