@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--slab", action="store_true",
                     help="only time one GPU's share of an 8-way z split of the 512^3 cycle: a 512x512x64-cell "
                          "slab with open z faces in the reference's stored box (ghost size 2)")
+    ap.add_argument("--config", choices=["c5", "c4"], default="c5",
+                    help="c5 (default): the advected lattice; c4: 2^26 particles in 4 Gaussian blobs + uniform, "
+                         "P2M then M2P of the same particles")
+    ap.add_argument("--no-box", action="store_true", help="skip the stored-box (reference layout) leg")
     return ap.parse_args()
 
 
@@ -108,22 +112,43 @@ def make_inputs(core, wl, N, lda, seed, torch):
     return mesh, x, w.contiguous(), xfresh, dt
 
 
-def run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan):
-    core.p2m(mesh, x, up, field, [0, 0, 0], 2, prec, p_plan)
-    core.m2p(mesh, xfresh, field, [0, 0, 0], out, 2, prec, m_plan)
+def clustered_inputs(core, wl, N, n, lda, torch):
+    """2^26 particles at 512^3: 80% from 4 Gaussian blobs (sigma 0.05, centres
+    from the workload's Philox streams), 20% uniform, wrapped into the unit
+    cube, in generation order (not cell order); 3 strengths ~ N(0,1)."""
+    import numpy as np
+    centres = torch.from_numpy(np.stack([wl.uniforms(4, wl.STREAM_BLOB + d, 4)[0] for d in range(3)], 1)).cuda()
+    g = core.gen_normals(n, 3, 44)
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    uni = torch.rand((n, 3), dtype=torch.float64, device="cuda", generator=gen)
+    idx = torch.arange(n, device="cuda")
+    x = torch.where((idx < int(0.8 * n))[:, None], centres[idx % 4] + 0.05 * g, uni)
+    del g, uni, idx
+    x = x - torch.floor(x)
+    x = torch.where(x >= 1.0, x - 1.0, x).contiguous()
+    return core.Mesh(3, N + 1), x, core.gen_normals(n, lda, 45)
 
 
-def time_form(core, torch, mesh, x, up, xfresh, prec, steps, warmup):
+def run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan, lo=(0, 0, 0)):
+    core.p2m(mesh, x, up, field, list(lo), 2, prec, p_plan)
+    core.m2p(mesh, xfresh, field, list(lo), out, 2, prec, m_plan)
+
+
+def time_form(core, torch, mesh, x, up, xfresh, prec, steps, warmup, ghost=0):
+    """FP64 / FP32c cycle on the compact layout (ghost 0) or on the reference's
+    stored box 1-g..Nm+g (ghost g)."""
     N = mesh.cells[0]
     lda = up.shape[1]
     vdt = core.VAL_DTYPE[prec]
-    field = torch.empty(mesh.compact_shape(lda), dtype=vdt, device="cuda")
+    lo = (-ghost,) * 3
+    shape = mesh.compact_shape(lda) if ghost == 0 else mesh.padded_shape(lda, [ghost] * 3)
+    field = torch.empty(shape, dtype=vdt, device="cuda")
     out = torch.empty((xfresh.shape[0], lda), dtype=vdt, device="cuda")
     p_plan, m_plan = core.Plan(), core.Plan()
     p_plan.set_timing(True)
     m_plan.set_timing(True)
     for _ in range(warmup):
-        run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan)
+        run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan, lo)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -131,7 +156,7 @@ def time_form(core, torch, mesh, x, up, xfresh, prec, steps, warmup):
     e0.record()
     t0 = time.perf_counter()
     for _ in range(steps):
-        run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan)
+        run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan, lo)
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -141,7 +166,7 @@ def time_form(core, torch, mesh, x, up, xfresh, prec, steps, warmup):
     # which the timed cycles above do not do.
     phases = {"bin": [], "weights": [], "deposit": [], "m2p_bin": [], "m2p": []}
     for _ in range(steps):
-        run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan)
+        run_cycle(core, mesh, x, up, field, xfresh, out, prec, p_plan, m_plan, lo)
         tp = p_plan.timings()
         tm = m_plan.timings()
         phases["bin"].append(tp[0])
@@ -154,11 +179,11 @@ def time_form(core, torch, mesh, x, up, xfresh, prec, steps, warmup):
     # backward of the last P2M (strength and position gradients of <g, field>),
     # reusing its binning: reported beside the cycle, not part of it
     g = torch.randn(field.shape, dtype=torch.float64, device="cuda").to(vdt)
-    core.p2m_adjoint(mesh, x, up, g, [0, 0, 0], p_plan, 2, prec)
+    core.p2m_adjoint(mesh, x, up, g, list(lo), p_plan, 2, prec)
     torch.cuda.synchronize()
     e0.record()
     for _ in range(3):
-        core.p2m_adjoint(mesh, x, up, g, [0, 0, 0], p_plan, 2, prec)
+        core.p2m_adjoint(mesh, x, up, g, list(lo), p_plan, 2, prec)
     e1.record()
     torch.cuda.synchronize()
     avg["adjoint"] = e0.elapsed_time(e1) / 3
@@ -166,8 +191,44 @@ def time_form(core, torch, mesh, x, up, xfresh, prec, steps, warmup):
     return ms, wall, avg, field, out
 
 
+def clustered_run(core, wl, torch, args):
+    """The clustered workload: P2M of 2^26 blob particles at 512^3, then M2P of
+    the field at the same particles, FP64 and FP32c; cell occupancy from the
+    binning."""
+    N, lda = args.n, args.lda
+    n = (1 << 26) if N == 512 else N ** 3 // 2
+    mesh, x, w = clustered_inputs(core, wl, N, n, lda, torch)
+    plan = core.Plan()
+    field = torch.empty(mesh.compact_shape(lda), dtype=torch.float64, device="cuda")
+    core.p2m(mesh, x, w, field, [0, 0, 0], 2, core.PREC_F64, plan)
+    _, _, start = plan.sort_view()
+    counts = torch.diff(start.long())
+    occ = {"max": int(counts.max().item()), "empty_cell_frac": (counts == 0).double().mean().item(),
+           "p99_over_occupied": float(torch.quantile(counts[counts > 0][:1 << 24].double(), 0.99).item())}
+    del field, start, counts, plan
+    forms = {}
+    for name, prec in (("f64", core.PREC_F64), ("f32c", core.PREC_F32C)):
+        up = w if prec == core.PREC_F64 else w.float()
+        ms, _, ph, f, o = time_form(core, torch, mesh, x, up, x, prec, args.steps, args.warmup)
+        forms[name] = {"ms_per_step": ms, "particles_per_s": n / (ms * 1e-3), "phases_ms": ph,
+                       "deposit_ns_per_particle": ph["deposit"] * 1e6 / n}
+        del f, o
+        print(f"# c4 {name} cycle {ms:.3f} ms  phases {ph}", file=sys.stderr, flush=True)
+    b64_pass = n * (24 + 8 * lda) + N ** 3 * 8 * lda
+    return {"metric": "particles/s 3D M'4 P2M+M2P, clustered particles @512^3",
+            "value": forms["f64"]["particles_per_s"], "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": forms["f64"]["ms_per_step"], "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic (80% in 4 Gaussian blobs, sigma 0.05; 20% uniform)",
+            "config": {"workload": f"clustered P2M+M2P, periodic {N}^3 cells, {n} particles, lda={lda}",
+                       "cells": N, "particles": n, "lda": lda},
+            "cell_occupancy": occ,
+            "cycle_hbm_frac": 2 * b64_pass / (forms["f64"]["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "forms": forms}
+
+
 def cpu_baseline(oracle_mod, core, wl, torch, n, lda, seed):
-    """Oracle P2M + M2P (OpenMP, all host cores of this process) on an n^3 sample."""
+    """Oracle P2M + M2P (OpenMP over the host cores of this process, at most
+    16: one GPU's share of the host's CPUs) on an n^3 sample."""
     ncores = len(os.sched_getaffinity(0))
     nthreads = min(ncores, 16)
     mesh, x, w, xfresh, _ = make_inputs(core, wl, n, lda, seed, torch)
@@ -189,6 +250,8 @@ def cpu_baseline(oracle_mod, core, wl, torch, n, lda, seed):
     best = sorted(times)[len(times) // 2]
     npart = n ** 3
     return {"value": npart / best, "unit": "particles/s", "cores": nthreads, "kind": "port",
+            "affinity_cores": ncores,
+            "threads_note": "threads = min(affinity cores, 16): one GPU's share of the host is 16 CPUs",
             "sample": f"{n}^3 cells, {npart} particles, lda={lda}, one P2M+M2P remesh cycle (FP64), "
                       f"median of {reps} cycles, {best:.3f} s/cycle"}
 
@@ -230,6 +293,9 @@ def main():
     if args.slab:
         print(json.dumps(slab_cycle(core, torch, args.steps, args.warmup, args.lda, args.seed)))
         return
+    if args.config == "c4":
+        print(json.dumps(clustered_run(core, wl, torch, args)))
+        return
     N, lda = args.n, args.lda
     np_ = N ** 3
     mesh, x, w64, xfresh, dt = make_inputs(core, wl, N, lda, args.seed, torch)
@@ -266,10 +332,27 @@ def main():
         del f32, o32, w32, df, do
         print(f"# fp32c cycle {ms32:.3f} ms  phases {ph32}", file=sys.stderr, flush=True)
 
+    # the same FP64 cycle on the reference's stored box 1-g..Nm+g (g = 2: the
+    # layout the Fortran drop-in passes): node N and the ghost layers are
+    # written too, and the adjoint reads its cotangent there
+    box = None
+    if not args.no_box:
+        msb, _, phb, fb, _ob = time_form(core, torch, mesh, x, w64, xfresh, core.PREC_F64, args.steps, args.warmup,
+                                         ghost=2)
+        uniq = fb[2:2 + N, 2:2 + N, 2:2 + N]
+        box = {"ms_per_step": msb, "particles_per_s": np_ / (msb * 1e-3), "ghost": 2,
+               "unique_nodes_max_rel_diff_vs_compact": (uniq - f64).abs().max().item() / f64.abs().max().item(),
+               "phases_ms": phb}
+        del fb, _ob, uniq
+        print(f"# fp64 stored-box cycle {msb:.3f} ms  phases {phb}", file=sys.stderr, flush=True)
+
     # dominant kernel of the FP64 cycle, priced with the algorithmic bytes of its pass
     dom = max(("deposit", "m2p"), key=lambda k: ph64[k])
     dom_kernel = "k_p3_tile" if dom == "deposit" else "k_m3_tile"
     achieved = b64_pass / (ph64[dom] * 1e-3) / 1e9
+    # adjoint (strength and position gradients in one launch): read x and u,
+    # write dL/du and dL/dx, read the cotangent once
+    b_adj = np_ * (24 + 8 * lda + 24 + 8 * lda) + nn * 8 * lda
     traffic, traffic_src = measured_traffic(dom_kernel, N, lda)
     cpu = None
     if not args.no_cpu:
@@ -301,7 +384,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": b64_pass, "kernel_ms": ph64[dom], "traffic_source": traffic_src},
+        "adjoint_roofline": {"kernel": "k_a3_tile", "ms": ph64["adjoint"], "algorithmic_bytes": b_adj,
+                             "achieved": b_adj / (ph64["adjoint"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": b_adj / (ph64["adjoint"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "fp32c": res or None,
+        "stored_box": box,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

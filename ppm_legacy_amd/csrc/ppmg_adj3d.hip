@@ -33,10 +33,56 @@ constexpr int A3_RY = A3_TY + 3;
 constexpr int A3_PL = A3_RY * 16;
 constexpr int A3_THREADS = 16 * A3_TY;
 
-template <int LDA, typename PosT, typename ValT, typename AccT>
+// Cotangent of stencil node u (unwrapped, periodic axes) when the forward
+// wrote a stored box: the forward's value at a stored node real on every axis
+// is the folded sum of the unique node wrap(u), copied to node N of a boxed
+// axis, and a ghost node holds the raw, unwrapped sum.  So the particle
+// collects g over the stored copies of wrap(u) (node 0 and its copy N on each
+// boxed axis) and, when u itself is a stored ghost node, g at u.  Summed in
+// FP64 (component c of the stored box).
+template <typename ValT>
+__device__ __forceinline__ double box_cotangent(const DevMesh& m, const FieldGeom& fg, const ValT* __restrict__ g,
+                                                int c, const int (&u)[3]) {
+    int s0[3], s1[3];
+    bool real = true, stored = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int N = m.N[d], hi = fg.lo[d] + fg.ext[d] - 1;
+        int w = u[d] % N;
+        if (w < 0) w += N;
+        s0[d] = w;
+        s1[d] = w == 0 && hi >= N ? N : -1;
+        if (u[d] < 0 || u[d] > N) real = false;
+        if (u[d] < fg.lo[d] || u[d] > hi) stored = false;
+    }
+    auto at = [&](int x, int y, int z) {
+        return (double)g[(((int64_t)(z - fg.lo[2]) * fg.ext[1] + (y - fg.lo[1])) * fg.ext[0] + (x - fg.lo[0])) *
+                             fg.lda + c];
+    };
+    double acc = 0.0;
+    for (int a = 0; a < 2; ++a) {
+        const int x = a ? s1[0] : s0[0];
+        if (x < 0) continue;
+        for (int b = 0; b < 2; ++b) {
+            const int y = b ? s1[1] : s0[1];
+            if (y < 0) continue;
+            for (int e = 0; e < 2; ++e) {
+                const int z = e ? s1[2] : s0[2];
+                if (z >= 0) acc += at(x, y, z);
+            }
+        }
+    }
+    if (!real && stored) acc += at(u[0], u[1], u[2]);
+    return acc;
+}
+
+// BOXED: the cotangent is on a stored box (some axis holds 0..N with its
+// ghosts, the reference's field_up(1-g:Nm+g)); the ring fetch reads it
+// through box_cotangent.  Otherwise it is the compact layout.
+template <int LDA, typename PosT, typename ValT, typename AccT, bool BOXED>
 __global__ void __launch_bounds__(A3_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
 k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
-          const ValT* __restrict__ up, const ValT* __restrict__ g, ValT* __restrict__ grad_u,
+          const ValT* __restrict__ up, const ValT* __restrict__ g, FieldGeom fg, ValT* __restrict__ grad_u,
           ValT* __restrict__ grad_x, int lda, int c0, int tz, int accumulate_x) {
     __shared__ ValT ring[4 * LDA * A3_PL];   // [plane][c][ry][nx]
     const int tid = threadIdx.x;
@@ -51,29 +97,41 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const int64_t plane = (int64_t)Nx * Ny;
 
     constexpr int NLD = (A3_PL * LDA + A3_THREADS - 1) / A3_THREADS;
-    ValT stage[NLD];
+    // (BOXED reads each plane straight into the ring between the barriers)
+    ValT stage[BOXED ? 1 : NLD];
     // Byte offset within a field plane of each element this thread stages
     // (the same in every plane; a plane fits in 2^31 bytes, checked at launch).
-    unsigned soff[NLD];
+    unsigned soff[BOXED ? 1 : NLD];
+    if constexpr (!BOXED) {
 #pragma unroll
-    for (int j = 0; j < NLD; ++j) {
-        const int i = tid + j * A3_THREADS;
-        const int c = i % LDA;
-        const int node = i / LDA;
-        const int nx = node & 15, ry = node >> 4;
-        int gx = (tx0 - 1 + nx) % Nx;
-        if (gx < 0) gx += Nx;
-        int gy = (ty0 - 1 + ry) % Ny;
-        if (gy < 0) gy += Ny;
-        soff[j] = (unsigned)(((gy * Nx + gx) * lda + c0 + c) * (int)sizeof(ValT));
+        for (int j = 0; j < NLD; ++j) {
+            const int i = tid + j * A3_THREADS;
+            const int c = i % LDA;
+            const int node = i / LDA;
+            const int nx = node & 15, ry = node >> 4;
+            int gx = (tx0 - 1 + nx) % Nx;
+            if (gx < 0) gx += Nx;
+            int gy = (ty0 - 1 + ry) % Ny;
+            if (gy < 0) gy += Ny;
+            soff[j] = (unsigned)(((gy * Nx + gx) * lda + c0 + c) * (int)sizeof(ValT));
+        }
     }
     auto fetch_plane = [&](int n) {
         int nz = n % Nz;
         if (nz < 0) nz += Nz;
         const char* base = (const char*)(g + (int64_t)nz * plane * lda);
 #pragma unroll
-        for (int j = 0; j < NLD; ++j) {
+        for (int j = 0; j < (BOXED ? 0 : NLD); ++j) {
             if (tid + j * A3_THREADS < A3_PL * LDA) stage[j] = *(const ValT*)(base + soff[j]);
+        }
+    };
+    // BOXED: plane n of the ring read through box_cotangent straight into LDS
+    auto fetch_store_boxed = [&](int n) {
+        ValT* dst = ring + (n & 3) * (LDA * A3_PL);
+        for (int i = tid; i < A3_PL * LDA; i += A3_THREADS) {
+            const int node = i / LDA;
+            const int u[3] = {tx0 - 1 + (node & 15), ty0 - 1 + (node >> 4), n};
+            dst[(i % LDA) * A3_PL + node] = (ValT)box_cotangent<ValT>(m, fg, g, c0 + i % LDA, u);
         }
     };
     auto store_plane = [&](int n) {
@@ -85,8 +143,12 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         }
     };
     for (int n = z0 - 1; n <= z0 + 2; ++n) {
-        fetch_plane(n);
-        store_plane(n);
+        if constexpr (BOXED) {
+            fetch_store_boxed(n);
+        } else {
+            fetch_plane(n);
+            store_plane(n);
+        }
     }
 
     const bool plane_lane = ex >= 1 && ex <= nxo && row < nyo;
@@ -112,7 +174,7 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         asm volatile("" ::"v"(first), "v"(cnt) : "memory");
         heads(cz + 1, first_n, cnt_n);
         const bool more = cz + 1 < z1;
-        if (more) fetch_plane(cz + 3);
+        if (!BOXED && more) fetch_plane(cz + 3);
         for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
             const bool has = cnt > r;
             int p = 0;
@@ -212,21 +274,38 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             }
         }
         __syncthreads();   // every row is done with plane cz-1: its slot takes plane cz+3
-        if (more) store_plane(cz + 3);
+        if (more) {
+            if constexpr (BOXED) fetch_store_boxed(cz + 3);
+            else store_plane(cz + 3);
+        }
         __syncthreads();
     }
 }
 
-template <int LL, typename PosT, typename ValT, typename AccT>
-static void launch_a3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const ValT* g, ValT* gu,
-                      ValT* gx, int lda, int c0, int acc, hipStream_t s) {
+template <int LL, typename PosT, typename ValT, typename AccT, bool BOXED>
+static void launch_a3_form(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const ValT* g,
+                           const FieldGeom& fg, ValT* gu, ValT* gx, int lda, int c0, int acc, hipStream_t s) {
     static std::atomic<int> slots{0};
-    if (slots.load() == 0) slots.store(resident_workgroups(k_a3_tile<LL, PosT, ValT, AccT>, A3_THREADS));
+    if (slots.load() == 0) slots.store(resident_workgroups(k_a3_tile<LL, PosT, ValT, AccT, BOXED>, A3_THREADS));
     const int tx = (m.N[0] + A3_TX - 1) / A3_TX, ty = (m.N[1] + A3_TY - 1) / A3_TY;
     const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 1);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
-    hipLaunchKernelGGL((k_a3_tile<LL, PosT, ValT, AccT>), grid, dim3(A3_THREADS), 0, s, m, plan->start.as<int>(),
-                       tile_perm(plan), xp, up, g, gu, gx, lda, c0, tz, acc);
+    hipLaunchKernelGGL((k_a3_tile<LL, PosT, ValT, AccT, BOXED>), grid, dim3(A3_THREADS), 0, s, m,
+                       plan->start.as<int>(), tile_perm(plan), xp, up, g, fg, gu, gx, lda, c0, tz, acc);
+}
+
+// The compact layout of every axis, or a stored box on some axis.
+static bool a3_compact(const DevMesh& m, const FieldGeom& fg) {
+    for (int d = 0; d < 3; ++d)
+        if (fg.lo[d] != 0 || fg.ext[d] != m.N[d]) return false;
+    return true;
+}
+
+template <int LL, typename PosT, typename ValT, typename AccT>
+static void launch_a3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const ValT* g,
+                      const FieldGeom& fg, ValT* gu, ValT* gx, int lda, int c0, int acc, hipStream_t s) {
+    if (a3_compact(m, fg)) launch_a3_form<LL, PosT, ValT, AccT, false>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
+    else launch_a3_form<LL, PosT, ValT, AccT, true>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
 }
 
 // One launch per four components (the position gradient accumulates over
@@ -234,42 +313,49 @@ static void launch_a3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const V
 // after another and the stencil x-offsets one per pass, which keeps the
 // kernel at 3 wavefronts per SIMD.
 template <typename PosT, typename ValT>
-static int adjoint3d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, int lda, const ValT* up, const ValT* g,
-                           ValT* gu, ValT* gx, hipStream_t s) {
+static int adjoint3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT* xp, int lda, const ValT* up,
+                           const ValT* g, const FieldGeom& fg, ValT* gu, ValT* gx, hipStream_t s) {
+    (void)np;
     for (int c0 = 0; c0 < lda; c0 += 4) {
         const int L = lda - c0 < 4 ? lda - c0 : 4;
         const int acc = c0 > 0 ? 1 : 0;
-        if (L == 1) launch_a3<1, PosT, ValT, double>(plan, m, xp, up, g, gu, gx, lda, c0, acc, s);
-        else if (L == 2) launch_a3<2, PosT, ValT, double>(plan, m, xp, up, g, gu, gx, lda, c0, acc, s);
-        else if (L == 3) launch_a3<3, PosT, ValT, double>(plan, m, xp, up, g, gu, gx, lda, c0, acc, s);
-        else launch_a3<4, PosT, ValT, double>(plan, m, xp, up, g, gu, gx, lda, c0, acc, s);
+#define AL(LL) launch_a3<LL, PosT, ValT, double>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s)
+        if (L == 1) AL(1);
+        else if (L == 2) AL(2);
+        else if (L == 3) AL(3);
+        else AL(4);
+#undef AL
         PPMG_HIP(hipGetLastError());
     }
     return 0;
 }
 
+// Periodic axes, each in the compact layout or a stored box holding the real
+// range 0..N (the reference's 1-g..Nm+g box is lo = -g).
 bool adjoint3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
     if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return false;
     for (int d = 0; d < 3; ++d) {
         if (!m.periodic[d] || m.N[d] < 4) return false;
-        if (fg.lo[d] != 0 || fg.ext[d] != m.N[d]) return false;
+        const bool compact = fg.lo[d] == 0 && fg.ext[d] == m.N[d];
+        const bool boxed = fg.lo[d] <= 0 && fg.lo[d] + fg.ext[d] - 1 >= m.N[d];
+        if (!compact && !boxed) return false;
     }
-    // the ring fetch addresses a field plane with 32-bit byte offsets
-    return (int64_t)m.N[0] * m.N[1] * fg.lda * 8 < ((int64_t)1 << 31);
+    // the compact ring fetch addresses a field plane with 32-bit byte offsets
+    return !a3_compact(m, fg) || (int64_t)m.N[0] * m.N[1] * fg.lda * 8 < ((int64_t)1 << 31);
 }
 
 // Uses the forward's binning in the plan (every particle is inside the sub:
 // the forward P2M would have failed otherwise).
 int adjoint3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
-                   const void* g, void* gu, void* gx, hipStream_t s) {
+                   const FieldGeom& fg, const void* g, void* gu, void* gx, hipStream_t s) {
     if (np <= 0) return 0;
     if (prec == PPMG_PREC_F64)
-        return adjoint3d_typed<double, double>(plan, m, (const double*)xp, lda, (const double*)up, (const double*)g,
-                                               (double*)gu, (double*)gx, s);
+        return adjoint3d_typed<double, double>(plan, m, np, (const double*)xp, lda, (const double*)up,
+                                               (const double*)g, fg, (double*)gu, (double*)gx, s);
     if (prec == PPMG_PREC_F32C)
-        return adjoint3d_typed<double, float>(plan, m, (const double*)xp, lda, (const float*)up, (const float*)g,
-                                              (float*)gu, (float*)gx, s);
-    return adjoint3d_typed<float, float>(plan, m, (const float*)xp, lda, (const float*)up, (const float*)g,
+        return adjoint3d_typed<double, float>(plan, m, np, (const double*)xp, lda, (const float*)up,
+                                              (const float*)g, fg, (float*)gu, (float*)gx, s);
+    return adjoint3d_typed<float, float>(plan, m, np, (const float*)xp, lda, (const float*)up, (const float*)g, fg,
                                          (float*)gu, (float*)gx, s);
 }
 

@@ -333,7 +333,7 @@ int ppmg_p2m(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     const char* gen = std::getenv("PPMG_GENERAL");
     const bool force_general = gen && gen[0] && gen[0] != '0';
     if (!force_general && p2m3d_fast_applicable(m, kernel, prec, fg)) {
-        if (p2m3d_fast(plan, m, prec, np, xp, fg, up, field->data, s)) return g_status.info;
+        if (p2m3d_fast(plan, m, prec, np, xp, fg, up, field->data, nullptr, s)) return g_status.info;
     } else if (p2m_general(plan, m, kernel, prec, np, xp, lda, up, fg, field->data, nullptr, s)) {
         return g_status.info;
     }
@@ -415,7 +415,7 @@ int ppmg_p2m_adjoint(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, co
     const char* gen = std::getenv("PPMG_GENERAL");
     const bool force_general = gen && gen[0] && gen[0] != '0';
     if (!force_general && adjoint3d_fast_applicable(m, kernel, fg)) {
-        if (adjoint3d_fast(plan, m, prec, np, xp, lda, up, g->data, grad_up, grad_xp, as_stream(stream)))
+        if (adjoint3d_fast(plan, m, prec, np, xp, lda, up, fg, g->data, grad_up, grad_xp, as_stream(stream)))
             return g_status.info;
     } else if (adjoint_general(plan, m, kernel, prec, np, xp, lda, up, fg, g->data, grad_up, grad_xp,
                                as_stream(stream))) {
@@ -529,7 +529,15 @@ int ppmg_rmsh_remesh(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, co
         return fail(PPMG_ERROR_FATAL, PPMG_ERR_PART_UNASS,
                     "MAJOR PROBLEM: " + std::to_string(nbad) + " particles lie in no sub");
     const void* tabs[3] = {wx1, wx2, wx3};
-    if (p2m_general(plan, m, kernel, prec, np, xp, lda, up, fg, field->data, tabs, s)) return g_status.info;
+    // 3D M'4 on the tiled deposit with the caller's weights (PPMG_GENERAL=1
+    // forces the general path, for cross-checks)
+    const char* gen = std::getenv("PPMG_GENERAL");
+    const bool force_general = gen && gen[0] && gen[0] != '0';
+    if (!force_general && p2m3d_fast_applicable(m, kernel, prec, fg)) {
+        if (p2m3d_fast(plan, m, prec, np, xp, fg, up, field->data, tabs, s)) return g_status.info;
+    } else if (p2m_general(plan, m, kernel, prec, np, xp, lda, up, fg, field->data, tabs, s)) {
+        return g_status.info;
+    }
     return 0;
 }
 

@@ -295,7 +295,8 @@ void bind_arena(Arena* a);   // nullptr unbinds
 
 // Opaque plan: scratch of the binning and the forward record for the adjoint.
 struct ppmg_plan {
-    ppmg::DevBuf key, slot, count, start, perm, wsorted, usorted, scan_tmp, err, misc, misc2, gaps, multi, multi_n, ghost;
+    ppmg::DevBuf key, slot, count, start, perm, wsorted, usorted, scan_tmp, err, misc, misc2, gaps, multi,
+        multi_n, ghost;
     int64_t np = 0;
     int64_t ncells = 0;
     bool sorted_input = false;   // last binning found the input already in cell order

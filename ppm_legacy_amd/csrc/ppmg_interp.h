@@ -21,8 +21,10 @@ int create_part_general(ppmg_plan* plan, const DevMesh& m, int prec, const Field
 // 3D M'4 tiled deposit (ppmg_p2m3d.hip): per axis (periodic or open) the
 // compact layout of a periodic axis or a stored box holding the real range 0..N.
 bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg);
+// wtab: NULL (weights from the positions), or the remesh weight tables
+// (four per particle and axis, ppm_rmsh_comp_weights' layout).
 int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
-               const void* up, void* field, hipStream_t s);
+               const void* up, void* field, const void* const* wtab, hipStream_t s);
 
 // 3D M'4 tiled gather (ppmg_m2p3d.hip), same layouts as the deposit: bins the
 // particles into the plan first; returns 1 without touching up when a particle
@@ -31,11 +33,11 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
 int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
                const void* field, void* up, hipStream_t s);
 
-// Tiled 3D M'4 adjoint (ppmg_adj3d.hip), on the forward's binning; compact
-// layout only.
+// Tiled 3D M'4 adjoint (ppmg_adj3d.hip), on the forward's binning; periodic
+// axes, each in the compact layout or a stored box holding 0..N.
 bool adjoint3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
 int adjoint3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
-                   const void* g, void* gu, void* gx, hipStream_t s);
+                   const FieldGeom& fg, const void* g, void* gu, void* gx, hipStream_t s);
 
 // Ghost mapping of a padded field (ppmg_ghost.hip).  A block is the part of
 // one periodic image of the real nodes that the stored box holds: src is its

@@ -323,9 +323,10 @@ static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const F
 }
 
 template <typename PosT, typename ValT, typename AccT>
-static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, const ValT* field,
-                       ValT* up, hipStream_t s) {
+static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT* xp, const FieldGeom& fg,
+                       const ValT* field, ValT* up, hipStream_t s) {
     const int lda = fg.lda;
+    (void)np;
     for (int c0 = 0; c0 < lda; c0 += 4) {
         int L = lda - c0 < 4 ? lda - c0 : 4;
 #define TL(LL) launch_m3<LL, PosT, ValT, AccT>(plan, m, xp, fg, c0, field, up, s)
@@ -355,12 +356,12 @@ int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const vo
     mark(plan, s);
     if (np <= 0) return 0;
     if (prec == PPMG_PREC_F64)
-        return m2p3d_typed<double, double, double>(plan, m, (const double*)xp, fg, (const double*)field,
+        return m2p3d_typed<double, double, double>(plan, m, np, (const double*)xp, fg, (const double*)field,
                                                    (double*)up, s);
     if (prec == PPMG_PREC_F32C)
-        return m2p3d_typed<double, float, double>(plan, m, (const double*)xp, fg, (const float*)field, (float*)up,
-                                                  s);
-    return m2p3d_typed<float, float, double>(plan, m, (const float*)xp, fg, (const float*)field, (float*)up, s);
+        return m2p3d_typed<double, float, double>(plan, m, np, (const double*)xp, fg, (const float*)field,
+                                                  (float*)up, s);
+    return m2p3d_typed<float, float, double>(plan, m, np, (const float*)xp, fg, (const float*)field, (float*)up, s);
 }
 
 }  // namespace ppmg

@@ -88,11 +88,22 @@ __device__ __forceinline__ void shift_in(const ValT (&wx)[4], const ValT (&wz)[4
 // every node receives its additions in an order fixed by the program and the
 // binning, as with one wavefront.  A taller tile (more wavefronts) re-reads
 // fewer halo cells: the window is 16 x (TY+3) cells for 13 x TY nodes.
-template <int LDA, typename PosT, typename ValT, int TY, int NW, bool OPEN>
+//
+// TAB: the 1D weights come from the caller's tables (ppm_rmsh_remesh's
+// wx1/wx2/wx3, four per particle and axis) instead of the positions.
+template <typename ValT> struct WTab {
+    const ValT* w[3];
+};
+
+template <int LDA, typename PosT, typename ValT, int TY, int NW, bool OPEN, bool TAB>
 __global__ void __launch_bounds__(64 * NW)
 k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
-          const ValT* __restrict__ up, FieldGeom fg, int c0, int tz, ValT* __restrict__ field) {
+          const ValT* __restrict__ up, FieldGeom fg, int c0, int tz, ValT* __restrict__ field, WTab<ValT> wt) {
     const int lda = fg.lda;
+    // per-particle data besides the strengths: the position, or (TAB) the
+    // twelve 1D weights
+    constexpr int NPD = TAB ? 12 : 3;
+    using PD = std::conditional_t<TAB, ValT, PosT>;
     using A = LdsAcc<ValT>;
     using slot_t = typename A::slot_t;
     constexpr int NT = 64 * NW;
@@ -158,16 +169,23 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             n = start[key + 1];   // end, subtracted where used: the load stays in flight
         }
     };
-    // position and strengths of particle p (an index into the caller's
-    // arrays) when has, zeros otherwise
-    auto load = [&](int p, bool has, PosT (&x)[3], ValT (&u)[LDA]) {
+    // position (or weights) and strengths of particle p (an index into the
+    // caller's arrays) when has, zeros otherwise
+    auto load = [&](int p, bool has, PD (&x)[NPD], ValT (&u)[LDA]) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) x[d] = PosT(0);
+        for (int d = 0; d < NPD; ++d) x[d] = PD(0);
 #pragma unroll
         for (int c = 0; c < LDA; ++c) u[c] = ValT(0);
         if (has) {
+            if constexpr (TAB) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) x[d] = xp[(int64_t)p * 3 + d];
+                for (int d = 0; d < 3; ++d)
+#pragma unroll
+                    for (int o = 0; o < 4; ++o) x[4 * d + o] = wt.w[d][(int64_t)p * 4 + o];
+            } else {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) x[d] = xp[(int64_t)p * 3 + d];
+            }
 #pragma unroll
             for (int c = 0; c < LDA; ++c) u[c] = up[(int64_t)p * lda + c0 + c];
         }
@@ -177,6 +195,23 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         int ci;
 #pragma unroll
         for (int d = 0; d < 3; ++d) fr[d] = (ValT)cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
+    };
+    // the 1D weights of a particle
+    auto weights = [&](const PD (&x)[NPD], ValT (&wx)[4], ValT (&wy)[4], ValT (&wz)[4]) {
+        if constexpr (TAB) {
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                wx[o] = x[o];
+                wy[o] = x[4 + o];
+                wz[o] = x[8 + o];
+            }
+        } else {
+            ValT fr[3];
+            offsets(x, fr);
+            mp4_unrolled<ValT>(fr[0], wx);
+            mp4_unrolled<ValT>(fr[1], wy);
+            mp4_unrolled<ValT>(fr[2], wz);
+        }
     };
     // The permutation entry of a step's first particle is looked up one step
     // ahead, and its position and strengths at the end of the step before.
@@ -188,7 +223,7 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     int pn = lookup(f1, n1);
     // Round 0's particle of the next step is loaded at the end of a step, while
     // the wavefront waits at the phase barrier; it is not live during the rounds.
-    PosT xn[3];
+    PD xn[NPD];
     ValT un[LDA];
     load(pn, n1 > f1, xn, un);
 
@@ -208,12 +243,9 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             const int ly0 = ey4[j] - 3;              // node row of stencil offset oy = 0
             // one round: every lane deposits one particle of its cell (zero
             // strength where it has none)
-            auto round = [&](const PosT (&x)[3], const ValT (&u)[LDA]) {
-                ValT wx[4], wy[4], wz[4], fr[3];
-                offsets(x, fr);
-                mp4_unrolled<ValT>(fr[0], wx);
-                mp4_unrolled<ValT>(fr[1], wy);
-                mp4_unrolled<ValT>(fr[2], wz);
+            auto round = [&](const PD (&x)[NPD], const ValT (&u)[LDA]) {
+                ValT wx[4], wy[4], wz[4];
+                weights(x, wx, wy, wz);
                 // Lane ex finishes node ex-3, which takes x-offset k from the particle
                 // of lane ex-k: fetch that particle's wx_k * u, wy and wz by DPP
                 // (k = 1..3), then form the 16 (y,z) sums locally.
@@ -256,7 +288,7 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                 uint64_t act = __ballot(cnt > r);
                 if (act == 0ull) break;
                 if (__popcll(act) > P3_SERIAL) {
-                    PosT x[3];
+                    PD x[NPD];
                     ValT u[LDA];
                     const int p = cnt > r ? (perm ? perm[first + r] : first + r) : 0;
                     load(p, cnt > r, x, u);
@@ -275,11 +307,18 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                     const int nx = (src & 15) - 3 + ox, ny = qrow + (src >> 4) - 3 + oy;
                     const bool own = nx >= 0 && nx < nxo && ny >= 0 && ny < nyo;
                     for (int rr = r; rr < sc; ++rr) {
-                        PosT x[3];
-                        ValT u[LDA], fr[3];
+                        PD x[NPD];
+                        ValT u[LDA], wyz, wxo;
                         load(perm ? perm[sf + rr] : sf + rr, true, x, u);
-                        offsets(x, fr);
-                        const ValT wyz = mp4_at<ValT>(fr[1], oy) * mp4_at<ValT>(fr[2], oz), wxo = mp4_at<ValT>(fr[0], ox);
+                        if constexpr (TAB) {
+                            wyz = x[4 + oy] * x[8 + oz];
+                            wxo = x[ox];
+                        } else {
+                            ValT fr[3];
+                            offsets(x, fr);
+                            wyz = mp4_at<ValT>(fr[1], oy) * mp4_at<ValT>(fr[2], oz);
+                            wxo = mp4_at<ValT>(fr[0], ox);
+                        }
                         if (own) {
 #pragma unroll
                             for (int c = 0; c < LDA; ++c) A::add(pl + c * P3_PL + nx + P3_TT * ny, wyz * (wxo * u[c]));
@@ -355,17 +394,17 @@ bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGe
 // 13 x TY nodes and NW wavefronts per tile.  The z chunk is chosen from the
 // device's resident workgroup count (queried once per instantiation): each
 // chunk marches 3 extra cell planes.
-template <int LL, typename PosT, typename ValT, int TY, int NW, bool OPEN>
-static void launch_p3_shape(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const FieldGeom& fg,
-                            int c0, ValT* field, hipStream_t s) {
+template <int LL, typename PosT, typename ValT, int TY, int NW, bool OPEN, bool TAB>
+static void launch_p3_shape(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const int* perm,
+                            const FieldGeom& fg, int c0, ValT* field, const WTab<ValT>& wt, hipStream_t s) {
     static std::atomic<int> slots{0};
-    if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT, TY, NW, OPEN>, 64 * NW));
+    if (slots.load() == 0) slots.store(resident_workgroups(k_p3_tile<LL, PosT, ValT, TY, NW, OPEN, TAB>, 64 * NW));
     const int Rx = m.N[0] + !m.periodic[0], Ry = m.N[1] + !m.periodic[1], Rz = m.N[2] + !m.periodic[2];
     const int tx = (Rx + P3_TT - 1) / P3_TT, ty = (Ry + TY - 1) / TY;
     const int tz = pick_z_chunk((int64_t)tx * ty, Rz, slots.load(), 3);
     dim3 grid((unsigned)((int64_t)tx * ty * ((Rz + tz - 1) / tz)));
-    hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT, TY, NW, OPEN>), grid, dim3(64 * NW), 0, s, m, plan->start.as<int>(),
-                       tile_perm(plan), xp, up, fg, c0, tz, field);
+    hipLaunchKernelGGL((k_p3_tile<LL, PosT, ValT, TY, NW, OPEN, TAB>), grid, dim3(64 * NW), 0, s, m,
+                       plan->start.as<int>(), perm, xp, up, fg, c0, tz, field, wt);
 }
 
 // Tile height: PPMG_P2M_ROWS (9, 13, 21 or 29 node rows: 1, 2, 3 or 4
@@ -379,24 +418,31 @@ static int p3_rows() {
     return r == 9 || r == 13 || r == 21 || r == 29 ? r : P3_TY;
 }
 
-template <int LL, typename PosT, typename ValT, bool OPEN>
-static void launch_p3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const FieldGeom& fg,
-                           int c0, ValT* field, hipStream_t s) {
-    switch (p3_rows()) {
-        case 9: launch_p3_shape<LL, PosT, ValT, 9, 1, OPEN>(plan, m, xp, up, fg, c0, field, s); break;
-        case 13: launch_p3_shape<LL, PosT, ValT, 13, 2, OPEN>(plan, m, xp, up, fg, c0, field, s); break;
-        case 29: launch_p3_shape<LL, PosT, ValT, 29, 4, OPEN>(plan, m, xp, up, fg, c0, field, s); break;
-        default: launch_p3_shape<LL, PosT, ValT, 21, 3, OPEN>(plan, m, xp, up, fg, c0, field, s); break;
+template <int LL, typename PosT, typename ValT, bool OPEN, bool TAB>
+static void launch_p3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const int* perm,
+                           const FieldGeom& fg, int c0, ValT* field, const WTab<ValT>& wt, hipStream_t s) {
+    if constexpr (TAB) {
+        // the remesh path takes the default tile only
+        launch_p3_shape<LL, PosT, ValT, 21, 3, OPEN, TAB>(plan, m, xp, up, perm, fg, c0, field, wt, s);
+    } else {
+#define SHAPE(TY, NW) launch_p3_shape<LL, PosT, ValT, TY, NW, OPEN, TAB>(plan, m, xp, up, perm, fg, c0, field, wt, s)
+        switch (p3_rows()) {
+            case 9: SHAPE(9, 1); break;
+            case 13: SHAPE(13, 2); break;
+            case 29: SHAPE(29, 4); break;
+            default: SHAPE(21, 3); break;
+        }
+#undef SHAPE
     }
 }
 
-template <int LL, typename PosT, typename ValT>
-static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const FieldGeom& fg, int c0,
-                      ValT* field, hipStream_t s) {
+template <int LL, typename PosT, typename ValT, bool TAB>
+static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const int* perm,
+                      const FieldGeom& fg, int c0, ValT* field, const WTab<ValT>& wt, hipStream_t s) {
     if (m.periodic[0] && m.periodic[1] && m.periodic[2])
-        launch_p3_open<LL, PosT, ValT, false>(plan, m, xp, up, fg, c0, field, s);
+        launch_p3_open<LL, PosT, ValT, false, TAB>(plan, m, xp, up, perm, fg, c0, field, wt, s);
     else
-        launch_p3_open<LL, PosT, ValT, true>(plan, m, xp, up, fg, c0, field, s);
+        launch_p3_open<LL, PosT, ValT, true, TAB>(plan, m, xp, up, perm, fg, c0, field, wt, s);
 }
 
 // Stored nodes besides the unique ones, for axes whose stored box holds the
@@ -444,12 +490,13 @@ __global__ void __launch_bounds__(256) k_p3_dup(FieldGeom fg, int3 n, int3 r, in
 // which is what ppm_map_field_ghost_put leaves in ghost nodes (the general
 // path's rule for nodes that are not real).  Enumerated as the planes outside
 // 0..N in z, then the rows outside in y (z real), then the nodes outside in x
-// (y, z real).  Weights and products in the value precision, sums in FP64.
+// (y, z real).  Weights and products in the value precision, sums in FP64;
+// with weight tables (wt.w[0] set) the weights are the tables'.
 template <typename PosT, typename ValT, int LL>
 __global__ void __launch_bounds__(256) k_p3_shell(DevMesh m, FieldGeom fg, int3 r, const int* __restrict__ start,
                                                   const int* __restrict__ perm, const PosT* __restrict__ xp,
                                                   const ValT* __restrict__ up, int c0, ValT* __restrict__ field,
-                                                  int64_t nshell) {
+                                                  int64_t nshell, WTab<ValT> wt) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nshell) return;
     const int e0 = fg.ext[0], e1 = fg.ext[1], e2 = fg.ext[2];
@@ -489,13 +536,18 @@ __global__ void __launch_bounds__(256) k_p3_shell(DevMesh m, FieldGeom fg, int3 
                 const int b = start[key], e = start[key + 1];
                 for (int q = b; q < e; ++q) {
                     const int p = perm ? perm[q] : q;
-                    ValT fr[3];
+                    ValT w;
+                    if (wt.w[0]) {
+                        w = (wt.w[0][(int64_t)p * 4 + ox] * wt.w[1][(int64_t)p * 4 + oy]) * wt.w[2][(int64_t)p * 4 + oz];
+                    } else {
+                        ValT fr[3];
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        int ci;
-                        fr[d] = (ValT)cell_offset<PosT>(xp[(int64_t)p * 3 + d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
+                        for (int d = 0; d < 3; ++d) {
+                            int ci;
+                            fr[d] = (ValT)cell_offset<PosT>(xp[(int64_t)p * 3 + d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
+                        }
+                        w = (mp4_at<ValT>(fr[0], ox) * mp4_at<ValT>(fr[1], oy)) * mp4_at<ValT>(fr[2], oz);
                     }
-                    const ValT w = (mp4_at<ValT>(fr[0], ox) * mp4_at<ValT>(fr[1], oy)) * mp4_at<ValT>(fr[2], oz);
 #pragma unroll
                     for (int c = 0; c < LL; ++c) acc[c] += (double)(w * up[(int64_t)p * fg.lda + c0 + c]);
                 }
@@ -509,13 +561,21 @@ __global__ void __launch_bounds__(256) k_p3_shell(DevMesh m, FieldGeom fg, int3 
 
 template <typename PosT, typename ValT>
 static int p2m3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT* xp, const FieldGeom& fg,
-                       const ValT* up, ValT* field, hipStream_t s) {
-    (void)np;
+                       const ValT* up, ValT* field, const ValT* const* wtab, hipStream_t s) {
     mark(plan, s);
     const int lda = fg.lda;
+    WTab<ValT> wt{{nullptr, nullptr, nullptr}};
+    if (wtab)
+        for (int d = 0; d < 3; ++d) wt.w[d] = wtab[d];
+    (void)np;
+    const int* pk = tile_perm(plan);
     for (int c0 = 0; c0 < lda; c0 += 4) {
         int L = lda - c0 < 4 ? lda - c0 : 4;
-#define TL(LL) launch_p3<LL, PosT, ValT>(plan, m, xp, up, fg, c0, field, s)
+#define TL(LL)                                                                            \
+    do {                                                                                  \
+        if (wtab) launch_p3<LL, PosT, ValT, true>(plan, m, xp, up, pk, fg, c0, field, wt, s);   \
+        else launch_p3<LL, PosT, ValT, false>(plan, m, xp, up, pk, fg, c0, field, wt, s);       \
+    } while (0)
         if (L == 1) TL(1);
         else if (L == 2) TL(2);
         else if (L == 3) TL(3);
@@ -544,7 +604,7 @@ static int p2m3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT
         for (int c0 = 0; c0 < lda; c0 += 4) {
             const int L = lda - c0 < 4 ? lda - c0 : 4;
 #define SH(LL) hipLaunchKernelGGL((k_p3_shell<PosT, ValT, LL>), dim3(g), dim3(256), 0, s, m, fg, r3, \
-                                  plan->start.as<int>(), tile_perm(plan), xp, up, c0, field, nshell)
+                                  plan->start.as<int>(), tile_perm(plan), xp, up, c0, field, nshell, wt)
             if (L == 1) SH(1);
             else if (L == 2) SH(2);
             else if (L == 3) SH(3);
@@ -558,12 +618,15 @@ static int p2m3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT
 }
 
 int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
-               const void* up, void* field, hipStream_t s) {
+               const void* up, void* field, const void* const* wtab, hipStream_t s) {
     if (prec == PPMG_PREC_F64)
-        return p2m3d_typed<double, double>(plan, m, np, (const double*)xp, fg, (const double*)up, (double*)field, s);
+        return p2m3d_typed<double, double>(plan, m, np, (const double*)xp, fg, (const double*)up, (double*)field,
+                                           (const double* const*)wtab, s);
     if (prec == PPMG_PREC_F32C)
-        return p2m3d_typed<double, float>(plan, m, np, (const double*)xp, fg, (const float*)up, (float*)field, s);
-    return p2m3d_typed<float, float>(plan, m, np, (const float*)xp, fg, (const float*)up, (float*)field, s);
+        return p2m3d_typed<double, float>(plan, m, np, (const double*)xp, fg, (const float*)up, (float*)field,
+                                          (const float* const*)wtab, s);
+    return p2m3d_typed<float, float>(plan, m, np, (const float*)xp, fg, (const float*)up, (float*)field,
+                                     (const float* const*)wtab, s);
 }
 
 }  // namespace ppmg

@@ -832,6 +832,49 @@ def test_comp_weights_and_remesh(core, oracle, dim, kernel):
     assert _rel(field.cpu().numpy(), fref) < 1e-12
 
 
+@pytest.mark.parametrize("prec", [F64, F32C])
+@pytest.mark.parametrize("layout", ["padded", "compact"])
+def test_remesh_tiled_deposits_the_weight_tables(core, oracle, monkeypatch, prec, layout):
+    """3D M'4 ppm_rmsh_remesh (ppm_rmsh_remesh.f:618-802) runs on the tiled
+    deposit with the caller's weight tables.  The tables here are M'4's
+    perturbed by up to 20% per entry, so a kernel that recomputed weights
+    from the positions would fail.  Clustered particles put hundreds in some
+    cells (the tile kernel's later rounds and its one-at-a-time leftovers).
+    Checked against the oracle's remesh with the same tables on every stored
+    node (reference box with ghosts, or the compact unique nodes) and, in
+    FP64, against the general path."""
+    N = 20
+    rng = np.random.default_rng(88 + prec)
+    xp = np.concatenate([rng.normal(0.5, 0.03, size=(2500, 3)) % 1.0, rng.uniform(0, 1, size=(1500, 3))])
+    up = rng.normal(size=(len(xp), 3))
+    t = oracle.topo(3, [N + 1] * 3)
+    info, wref = oracle.comp_weights(t, MP4, xp)
+    assert info == 0
+    wt = [w * (1.0 + 0.2 * rng.uniform(-1, 1, size=w.shape)) for w in wref]
+    g = [2, 2, 2]
+    info, fref = oracle.remesh(t, MP4, xp, up, g, wt)
+    assert info == 0
+    mesh = core.Mesh(3, N + 1)
+    vdt = torch.float64 if prec == F64 else torch.float32
+    x = _dev(xp, torch.float64)
+    u = _dev(up, vdt)
+    w = [_dev(a, vdt) for a in wt]
+    if layout == "padded":
+        shape, lo, ref = mesh.padded_shape(3, g), [-2] * 3, fref
+    else:
+        shape, lo, ref = mesh.compact_shape(3), [0] * 3, oracle.unique_nodes(fref, g, 3)
+    field = torch.empty(shape, dtype=vdt, device="cuda")
+    core.remesh(mesh, x, u, w, field, lo, MP4, prec)
+    tol = 1e-12 if prec == F64 else 1e-6
+    assert _rel(field.double().cpu().numpy(), ref) < tol
+    if prec == F64:
+        monkeypatch.setenv("PPMG_GENERAL", "1")
+        gen = torch.empty_like(field)
+        core.remesh(mesh, x, u, w, gen, lo, MP4, prec)
+        monkeypatch.delenv("PPMG_GENERAL")
+        assert _rel(field.cpu().numpy(), gen.cpu().numpy()) < 1e-12
+
+
 def test_create_part(core, oracle):
     N = 10
     g = [1, 1, 1]
@@ -973,6 +1016,42 @@ def test_adjoint_stored_boxes(core, oracle, bc, kernel, prec):
                 xs.append(float(np.sum(fp * gc)))
             fd = (xs[0] - xs[1]) / (2 * h)
             assert abs(gx[p, d] - fd) <= tol_fd * scale, (p, d, gx[p, d], fd)
+
+
+@pytest.mark.parametrize("prec", [F64, F32C])
+@pytest.mark.parametrize("layout", ["box_g2", "mixed", "compact"])
+def test_adjoint_tiled_stored_box_vs_general(core, monkeypatch, prec, layout):
+    """The tiled adjoint reads a cotangent on a stored box of periodic axes
+    (the reference's 1-g..Nm+g box, or compact on some axes and boxed with
+    other ghost widths on others) through the copies of each unique node
+    and the raw ghost nodes.  Clustered particles (crowded cells, particles
+    on the faces' stencils) with lda 3: strength and position gradients
+    against the general per-particle adjoint, which collects g node by node
+    from every stored node the forward writes (1e-12 in FP64, 1e-6 in
+    FP32c)."""
+    N = 24
+    rng = np.random.default_rng(140 + prec)
+    xp = np.concatenate([rng.normal(0.02, 0.03, size=(2000, 3)) % 1.0, rng.uniform(0, 1, size=(2000, 3))])
+    up = rng.normal(size=(len(xp), 3))
+    vdt = _val_dtype(prec)
+    mesh = core.Mesh(3, N + 1)
+    if layout == "box_g2":
+        lo, ext = [-2, -2, -2], [N + 5] * 3
+    elif layout == "mixed":
+        lo, ext = [0, -1, -3], [N, N + 3, N + 7]
+    else:
+        lo, ext = [0, 0, 0], [N] * 3
+    x, u = _dev(xp, torch.float64), _dev(up, vdt)
+    field = torch.empty((ext[2], ext[1], ext[0], 3), dtype=vdt, device="cuda")
+    plan = core.p2m(mesh, x, u, field, lo, MP4, prec)
+    g = _dev(rng.normal(size=(ext[2], ext[1], ext[0], 3)), vdt)
+    gu, gx = core.p2m_adjoint(mesh, x, u, g, lo, plan, MP4, prec)
+    monkeypatch.setenv("PPMG_GENERAL", "1")
+    ru, rx = core.p2m_adjoint(mesh, x, u, g, lo, plan, MP4, prec)
+    monkeypatch.delenv("PPMG_GENERAL")
+    tol = 1e-12 if prec == F64 else 1e-6
+    assert _rel(gu.double().cpu().numpy(), ru.double().cpu().numpy()) < tol
+    assert _rel(gx.double().cpu().numpy(), rx.double().cpu().numpy()) < tol
 
 
 def test_impose_part_bc(core, oracle):
