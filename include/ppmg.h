@@ -56,7 +56,9 @@ extern "C" {
 /* precision forms */
 #define PPMG_PREC_F64  0   /* double xp, double values: the reference's *_d* specifics */
 #define PPMG_PREC_F32C 1   /* double xp, cell index and offset in FP64, float weights and
-                              values, Kahan-compensated per-node / per-particle sums */
+                              values in memory; the tiled 3D M'4 kernels multiply in FP32
+                              and sum each node / particle in FP64, the general paths sum
+                              in Kahan-compensated FP32 */
 #define PPMG_PREC_F32  2   /* float xp and values: the reference's *_s* specifics
                               (sums compensated as in F32C) */
 

@@ -92,11 +92,12 @@ def build_fortran(verbose=True):
     if os.path.exists(lib):
         os.remove(lib)
     subprocess.run(["ar", "rcs", lib, *objs], check=True)
-    drv = os.path.join(fdir, "ppmg_fortran_check.f90")
+    # the test client of the drop-in (tests/fortran), linked against it
+    drv = os.path.join(ROOT, "tests", "fortran", "ppmg_fortran_check.f90")
     if os.path.exists(drv):
-        exe = os.path.join(HERE, "ppmg_fortran_check")
+        exe = os.path.join(ROOT, "tests", "fortran", "ppmg_fortran_check")
         cmd = [flang, "-O2", "-I", out, drv, lib, "-L", HERE, "-lppmg",
-               "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
+               "-Wl,-rpath,$ORIGIN/../../ppm_legacy_amd", "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"flang link failed:\n{r.stdout}\n{r.stderr}")

@@ -2,8 +2,8 @@
 module and routine names (ppm_init, ppm_mktopo, ppm_interp_p2m/m2p,
 ppm_rmsh_comp_weights/remesh/create_part, ppm_util_rank3d,
 ppm_impose_part_bc) runs on the GPU library and is checked against the
-oracle.  The program is ppm_legacy_amd/fortran/ppmg_fortran_check.f90, built
-by __graft_entry__.build() with flang."""
+oracle.  The program is tests/fortran/ppmg_fortran_check.f90, built by
+__graft_entry__.build() with flang against the drop-in modules."""
 import os
 import subprocess
 
@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "ppm_legacy_amd", "ppmg_fortran_check")
+EXE = os.path.join(ROOT, "tests", "fortran", "ppmg_fortran_check")
 BSP2, MP4 = 1, 2
 
 

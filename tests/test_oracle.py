@@ -8,6 +8,8 @@ is chosen so that a plausible slip (dropped term, wrong sign, shifted index,
 transposed operand) fails it; test_planted_mistakes_are_caught shows that on
 purpose-broken copies of the oracle's output.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -539,3 +541,25 @@ def test_divfree_field_is_divergence_free():
     assert np.max(np.abs(div)) < 1e-6 * scale
     k, A, phi = modes
     assert np.all(np.linalg.norm(k, axis=1) >= 1) and np.all(np.linalg.norm(k, axis=1) <= 8)
+
+
+# --------------------------------------------------------- sanitizers -----
+def test_oracle_under_address_and_undefined_sanitizers():
+    """Every oracle routine (2D/3D, both kernels, FP64/FP32, periodic and open
+    axes, ghost widths 2-3, empty input, the OpenMP variants) built with
+    AddressSanitizer and UndefinedBehaviorSanitizer (oracle/Makefile,
+    `make sanitize`): no invalid access or undefined operation, and the
+    closed-form checks of the driver hold (mass conservation, M2P of a
+    constant)."""
+    import shutil
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc is not installed")
+    subprocess.run(["make", "-s", "-C", os.path.join(root, "oracle"), "sanitize"], check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1:verify_asan_link_order=0",
+               UBSAN_OPTIONS="halt_on_error=1", OMP_NUM_THREADS="2")
+    r = subprocess.run([os.path.join(root, "oracle", "_san", "sanitize_check")], capture_output=True, text=True,
+                       env=env, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "all checks passed" in r.stdout
