@@ -139,6 +139,19 @@ int ppmg_p2m_noalloc(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, co
 int ppmg_m2p(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void* xp, int lda,
              const ppmg_field* field, void* up, ppmg_plan* plan, void* stream);
 
+/* The binning step of a coming ppmg_m2p, done ahead (no reference
+ * counterpart: ppm_interp_m2p bins inside the call, ppm_interp_m2p.f:494-770).
+ * It lets the M2P particles be binned on one stream while other work, such as
+ * the P2M that produces the field, runs on another.  The next ppmg_m2p on
+ * this plan with the same mesh, kernel, precision, np and xp pointer uses
+ * this binning instead of binning again (once); xp must not change in
+ * between, and that ppmg_m2p must be ordered after this call (same stream,
+ * or an event).  Any other binning on the plan discards it.  Only the 3D M'4
+ * tiled gather reads a binning; for other kernels and dimensions this call
+ * does nothing. */
+int ppmg_m2p_prebin(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void* xp, ppmg_plan* plan,
+                    void* stream);
+
 /* Adjoint of ppmg_p2m (no reference counterpart).  g = dL/dm on the stored
  * nodes of any layout ppmg_p2m writes (compact, or a box with ghosts, on
  * periodic and non-periodic axes): each particle collects g from every stored

@@ -53,7 +53,7 @@ class PpmgError(RuntimeError):
 
 # every exported entry point declared in include/ppmg.h
 EXPORTS = [
-    "ppmg_plan_create", "ppmg_plan_destroy", "ppmg_p2m", "ppmg_p2m_noalloc", "ppmg_m2p", "ppmg_p2m_adjoint", "ppmg_util_rank",
+    "ppmg_plan_create", "ppmg_plan_destroy", "ppmg_p2m", "ppmg_p2m_noalloc", "ppmg_m2p", "ppmg_m2p_prebin", "ppmg_p2m_adjoint", "ppmg_util_rank",
     "ppmg_impose_part_bc", "ppmg_rmsh_comp_weights", "ppmg_rmsh_remesh", "ppmg_rmsh_create_part",
     "ppmg_plan_set_timing", "ppmg_plan_timings", "ppmg_plan_sort_view", "ppmg_last_error", "ppmg_set_debug", "ppmg_device_sync", "ppmg_init",
     "ppmg_finalize", "ppmg_set_precision_mode", "ppmg_mktopo_field", "ppmg_topo_destroy",

@@ -178,6 +178,16 @@ def m2p(mesh, xp, field, lo, up, kernel=KERNEL_MP4, prec=PREC_F64, plan=None):
     return plan
 
 
+def m2p_prebin(mesh, xp, plan, kernel=KERNEL_MP4, prec=PREC_F64):
+    """Bin the particles of the next m2p on `plan` now (on the current
+    stream), so that binning can overlap other work; xp must not change
+    before that m2p."""
+    _check_tensor(xp, POS_DTYPE[prec], "xp", mesh.dim)
+    check(lib().ppmg_m2p_prebin(mesh.ref(), kernel, prec, ctypes.c_int64(xp.shape[0]), _ptr(xp), plan.handle,
+                                _stream()))
+    return plan
+
+
 def p2m_adjoint(mesh, xp, up, g, lo, plan, kernel=KERNEL_MP4, prec=PREC_F64, want_u=True, want_x=True):
     """Gradients of a loss of the P2M field: returns (dL/du, dL/dx)."""
     _check_tensor(g, VAL_DTYPE[prec], "g")

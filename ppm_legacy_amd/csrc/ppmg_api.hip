@@ -363,12 +363,16 @@ int ppmg_m2p(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     if (make_field(field, m, &fg)) return g_status.info;
     if (fg.lda != lda) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "field lda differs from lda");
     hipStream_t s = as_stream(stream);
+    // a binning ppmg_m2p_prebin left for exactly these particles is used once
+    const bool prebinned = plan->prebinned && plan->prebin_xp == xp && plan->prebin_np == np &&
+                           plan->prebin_prec == prec && same_mesh(plan->prebin_mesh, m);
+    plan->prebinned = false;
     if (plan->err.ensure(sizeof(int) * 4) || fill_zero(plan->err.p, sizeof(int) * 4, s)) return g_status.info;
     mark(plan, s, true);
     const char* gen = std::getenv("PPMG_GENERAL");
     const bool force_general = gen && gen[0] && gen[0] != '0';
     if (!force_general && m2p3d_fast_applicable(m, kernel, fg)) {
-        int rc = m2p3d_fast(plan, m, prec, np, xp, fg, field->data, up, s);
+        int rc = m2p3d_fast(plan, m, prec, np, xp, fg, field->data, up, prebinned, s);
         if (rc < 0) return g_status.info;
         if (rc == 0) {
             mark(plan, s);
@@ -386,6 +390,33 @@ int ppmg_m2p(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     if (bad)
         return fail(PPMG_ERROR_ERROR, PPMG_ERR_FIELD_BOUNDS,
                     std::to_string(bad) + " particles read nodes outside the stored field");
+    return 0;
+}
+
+int ppmg_m2p_prebin(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void* xp, ppmg_plan* plan,
+                    void* stream) {
+    clear_error();
+    if (check_plan(plan)) return PPMG_ERROR_ERROR;
+    plan->prebinned = false;
+    if (kernel != PPMG_KERNEL_MP4 && kernel != PPMG_KERNEL_BSP2)
+        return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "Only Mp4 and Bsp2 are available.");
+    if (np < 0) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "particles must be specified");
+    if (np > 0 && !xp) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "xp is NULL");
+    if (np >= ((int64_t)1 << 31)) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "more than 2^31-1 particles");
+    if (np == 0) return 0;
+    DevMesh m;
+    if (make_mesh(mesh, prec, Spacing::Single, &m)) return g_status.info;
+    // only the 3D M'4 tiled gather reads a binning
+    if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return 0;
+    int64_t nbad = 0;
+    plan->have_forward = false;
+    if (bin_particles(plan, m, prec, np, xp, as_stream(stream), &nbad)) return g_status.info;
+    plan->prebinned = true;
+    plan->prebin_xp = xp;
+    plan->prebin_np = np;
+    plan->prebin_nbad = nbad;
+    plan->prebin_prec = prec;
+    plan->prebin_mesh = m;
     return 0;
 }
 

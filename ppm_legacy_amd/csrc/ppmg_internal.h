@@ -307,6 +307,13 @@ struct ppmg_plan {
     int bin_prec = 0;
     ppmg::DevMesh bin_mesh{};
     bool keys_written = true;    // false: the last binning (cell-ordered input) wrote no keys
+    // ppmg_m2p_prebin: the binning of a coming ppmg_m2p (its positions,
+    // count, precision and mesh, and the particles it found outside the sub)
+    bool prebinned = false;
+    const void* prebin_xp = nullptr;
+    int64_t prebin_np = 0, prebin_nbad = 0;
+    int prebin_prec = 0;
+    ppmg::DevMesh prebin_mesh{};
     int dim = 0, kernel = 0, prec = 0;
     ppmg::DevMesh mesh{};
     bool have_forward = false;

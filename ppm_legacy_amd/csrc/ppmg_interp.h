@@ -27,11 +27,12 @@ int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const vo
                const void* up, void* field, const void* const* wtab, hipStream_t s);
 
 // 3D M'4 tiled gather (ppmg_m2p3d.hip), same layouts as the deposit: bins the
-// particles into the plan first; returns 1 without touching up when a particle
-// lies outside the sub.
+// particles into the plan first (unless prebinned: the plan already holds
+// their binning); returns 1 without touching up when a particle lies outside
+// the sub.
 bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
 int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
-               const void* field, void* up, hipStream_t s);
+               const void* field, void* up, bool prebinned, hipStream_t s);
 
 // Tiled 3D M'4 adjoint (ppmg_adj3d.hip), on the forward's binning; periodic
 // axes, each in the compact layout or a stored box holding 0..N.

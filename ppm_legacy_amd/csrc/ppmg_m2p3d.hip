@@ -349,9 +349,9 @@ static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT
 // it saves bytes (field in HBM and LDS, output) and sums its 64 products in
 // FP64 registers -- more accurate than compensated FP32 at no extra cost.
 int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
-               const void* field, void* up, hipStream_t s) {
-    int64_t nbad = 0;
-    if (bin_particles(plan, m, prec, np, xp, s, &nbad)) return PPMG_ERROR_FATAL;
+               const void* field, void* up, bool prebinned, hipStream_t s) {
+    int64_t nbad = prebinned ? plan->prebin_nbad : 0;
+    if (!prebinned && bin_particles(plan, m, prec, np, xp, s, &nbad)) return PPMG_ERROR_FATAL;
     if (nbad > 0) return 1;
     mark(plan, s);
     if (np <= 0) return 0;

@@ -578,6 +578,7 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
         plan->err.ensure(sizeof(int) * 4))
         return PPMG_ERROR_FATAL;
     plan->bin_stream = s;
+    plan->prebinned = false;   // a binning left by ppmg_m2p_prebin is overwritten
     plan->np = np;
     plan->ncells = ncells;
     plan->bin_xp = xp;
@@ -744,6 +745,7 @@ int sort_rows(int dim, int prec, int64_t np, void* xp, const int* lpdx1, const i
 int rank_particles(ppmg_plan* plan, const RankGeom& g, int prec, int64_t np, const void* xp,
                    int* lpdx, int* lhbx, int64_t* icount, hipStream_t s) {
     int64_t ncells = (int64_t)g.ntot[0] * g.ntot[1] * g.ntot[2];
+    plan->prebinned = false;   // the plan's scratch is reused below
     if (plan->key.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
         plan->slot.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
         plan->count.ensure(sizeof(int) * (size_t)ncells) || plan->err.ensure(sizeof(int) * 4))

@@ -832,6 +832,54 @@ def test_comp_weights_and_remesh(core, oracle, dim, kernel):
     assert _rel(field.cpu().numpy(), fref) < 1e-12
 
 
+@pytest.mark.parametrize("order", ["lattice", "shuffled"])
+def test_m2p_prebin_on_a_second_stream(core, order):
+    """ppmg_m2p_prebin bins the particles of the next M2P on a side stream
+    while a P2M runs on the main one; the M2P that follows (ordered after
+    both) must equal an M2P that bins itself, bitwise.  A prebinning is used
+    only by the M2P of the same positions: another position array, or another
+    binning on the plan in between, makes the M2P bin again."""
+    N = 40
+    xq = wl.jittered_lattice(N, seed=31)
+    if order == "shuffled":
+        xq = xq[np.random.default_rng(5).permutation(len(xq))]
+    xs = wl.jittered_lattice(N, seed=32)
+    us = wl.strengths(33, len(xs), 3)
+    mesh = core.Mesh(3, N + 1)
+    x, u, q = _dev(xs, torch.float64), _dev(us, torch.float64), _dev(xq, torch.float64)
+    field = torch.empty(mesh.compact_shape(3), dtype=torch.float64, device="cuda")
+    core.p2m(mesh, x, u, field, [0] * 3, MP4, F64)
+    ref = torch.empty((len(xq), 3), dtype=torch.float64, device="cuda")
+    core.m2p(mesh, q, field, [0] * 3, ref, MP4, F64)
+    torch.cuda.synchronize()
+    main, side = torch.cuda.current_stream(), torch.cuda.Stream()
+    plan = core.Plan()
+    pp = core.Plan()
+    field2 = torch.empty_like(field)
+    for rep in range(3):
+        out = torch.full_like(ref, 7.0)
+        main.wait_stream(side)
+        core.p2m(mesh, x, u, field2, [0] * 3, MP4, F64, pp)
+        with torch.cuda.stream(side):
+            core.m2p_prebin(mesh, q, plan, MP4, F64)
+            side.wait_stream(main)
+            core.m2p(mesh, q, field2, [0] * 3, out, MP4, F64, plan)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), rep
+    # prebinned for another array of the same particles: the M2P bins again
+    q2 = q.clone()
+    core.m2p_prebin(mesh, q, plan, MP4, F64)
+    out = torch.empty_like(ref)
+    core.m2p(mesh, q2, field, [0] * 3, out, MP4, F64, plan)
+    assert torch.equal(out, ref)
+    # a P2M binning on the plan discards the prebinning
+    core.m2p_prebin(mesh, q, plan, MP4, F64)
+    core.p2m(mesh, x, u, field2, [0] * 3, MP4, F64, plan)
+    core.m2p(mesh, q, field, [0] * 3, out, MP4, F64, plan)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+
+
 @pytest.mark.parametrize("prec", [F64, F32C])
 @pytest.mark.parametrize("layout", ["padded", "compact"])
 def test_remesh_tiled_deposits_the_weight_tables(core, oracle, monkeypatch, prec, layout):
