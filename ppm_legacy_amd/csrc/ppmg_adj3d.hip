@@ -190,12 +190,12 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                 const AccT f0 = (AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci);
                 const AccT f1 = (AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci);
                 const AccT f2 = (AccT)cell_offset<PosT>(x[2], (PosT)m.minv[2], (PosT)m.dxi[2], ci);
-                mp4_unrolled<AccT>(f0, wx);
-                mp4_unrolled<AccT>(f1, wy);
-                mp4_unrolled<AccT>(f2, wz);
-                mp4_unrolled_deriv<AccT>(f0, dx);
-                mp4_unrolled_deriv<AccT>(f1, dy);
-                mp4_unrolled_deriv<AccT>(f2, dz);
+                mp4_factored<AccT>(f0, wx);
+                mp4_factored<AccT>(f1, wy);
+                mp4_factored<AccT>(f2, wz);
+                mp4_factored_deriv<AccT>(f0, dx);
+                mp4_factored_deriv<AccT>(f1, dy);
+                mp4_factored_deriv<AccT>(f2, dz);
 #pragma unroll
                 for (int o = 0; o < 4; ++o) {
                     dx[o] *= (AccT)m.dxi[0];

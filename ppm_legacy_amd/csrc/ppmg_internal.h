@@ -196,6 +196,31 @@ __device__ __forceinline__ void mp4_unrolled(T frac, T a[4]) {
     a[3] = T(2) + (T(4) + (T(2.5) + T(0.5) * x13) * x13) * x13;
 }
 
+// The same four weights in factored form, 11 operations instead of 16: with
+// t = frac and s = 1 - t, W(1+t) = -t s^2/2, W(t) = 1 + t^2 (1.5 t - 2.5),
+// W(s) = 1 + s^2 (1.5 s - 2.5) and W(2-t) = -t^2 s/2.  Equal to mp4_unrolled
+// up to rounding (a few ulps); the tiled kernels use it.
+template <typename T>
+__device__ __forceinline__ void mp4_factored(T t, T a[4]) {
+    const T s = T(1) - t, t2 = t * t, s2 = s * s;
+    a[0] = (T(-0.5) * t) * s2;
+    a[1] = T(1) + t2 * (T(1.5) * t - T(2.5));
+    a[2] = T(1) + s2 * (T(1.5) * s - T(2.5));
+    a[3] = (T(-0.5) * s) * t2;
+}
+
+// d/dt of mp4_factored's weights, factored the same way (equal to
+// mp4_unrolled_deriv up to rounding): s (1.5 t - 0.5), t (4.5 t - 5),
+// s (5 - 4.5 s), t (1.5 t - 1).
+template <typename T>
+__device__ __forceinline__ void mp4_factored_deriv(T t, T d[4]) {
+    const T s = T(1) - t;
+    d[0] = s * (T(1.5) * t - T(0.5));
+    d[1] = t * (T(4.5) * t - T(5));
+    d[2] = s * (T(5) - T(4.5) * s);
+    d[3] = t * (T(1.5) * t - T(1));
+}
+
 // Weight o (0..3) of mp4_unrolled alone, without branches.  With
 // t = (frac + 1) - o the four unrolled polynomials are the inner form
 // 1 + (-2.5 + 1.5|t|) t^2 (o = 1, 2) and the outer form

@@ -201,9 +201,9 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             AccT wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, wz[4] = {0, 0, 0, 0};
             if (has) {
                 int ci;
-                mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci), wx);
-                mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci), wy);
-                mp4_unrolled<AccT>((AccT)cell_offset<PosT>(x[2], (PosT)m.minv[2], (PosT)m.dxi[2], ci), wz);
+                mp4_factored<AccT>((AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci), wx);
+                mp4_factored<AccT>((AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci), wy);
+                mp4_factored<AccT>((AccT)cell_offset<PosT>(x[2], (PosT)m.minv[2], (PosT)m.dxi[2], ci), wz);
             }
             // (y,z) weights of the particles of lanes ex+1 (k=0), ex (1), ex-1 (2), ex-2 (3)
             AccT a[4][4], b[4][4];

@@ -208,9 +208,9 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         } else {
             ValT fr[3];
             offsets(x, fr);
-            mp4_unrolled<ValT>(fr[0], wx);
-            mp4_unrolled<ValT>(fr[1], wy);
-            mp4_unrolled<ValT>(fr[2], wz);
+            mp4_factored<ValT>(fr[0], wx);
+            mp4_factored<ValT>(fr[1], wy);
+            mp4_factored<ValT>(fr[2], wz);
         }
     };
     // The permutation entry of a step's first particle is looked up one step
