@@ -146,9 +146,8 @@ int ppmg_m2p(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
  * this plan with the same mesh, kernel, precision, np and xp pointer uses
  * this binning instead of binning again (once); xp must not change in
  * between, and that ppmg_m2p must be ordered after this call (same stream,
- * or an event).  Any other binning on the plan discards it.  Only the 3D M'4
- * tiled gather reads a binning; for other kernels and dimensions this call
- * does nothing. */
+ * or an event).  Any other binning on the plan discards it.  Only the 3D tiled
+ * gather (M'4 and BSP2) reads a binning; in 2D this call does nothing. */
 int ppmg_m2p_prebin(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void* xp, ppmg_plan* plan,
                     void* stream);
 

@@ -333,7 +333,9 @@ int ppmg_p2m(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     const char* gen = std::getenv("PPMG_GENERAL");
     const bool force_general = gen && gen[0] && gen[0] != '0';
     if (!force_general && p2m3d_fast_applicable(m, kernel, prec, fg)) {
-        if (p2m3d_fast(plan, m, prec, np, xp, fg, up, field->data, nullptr, s)) return g_status.info;
+        if (p2m3d_fast(plan, m, kernel, prec, np, xp, fg, up, field->data, nullptr, s)) return g_status.info;
+    } else if (!force_general && p2m2d_fast_applicable(m, kernel, fg)) {
+        if (p2m2d_fast(plan, m, kernel, prec, xp, fg, up, field->data, s)) return g_status.info;
     } else if (p2m_general(plan, m, kernel, prec, np, xp, lda, up, fg, field->data, nullptr, s)) {
         return g_status.info;
     }
@@ -372,7 +374,7 @@ int ppmg_m2p(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     const char* gen = std::getenv("PPMG_GENERAL");
     const bool force_general = gen && gen[0] && gen[0] != '0';
     if (!force_general && m2p3d_fast_applicable(m, kernel, fg)) {
-        int rc = m2p3d_fast(plan, m, prec, np, xp, fg, field->data, up, prebinned, s);
+        int rc = m2p3d_fast(plan, m, kernel, prec, np, xp, fg, field->data, up, prebinned, s);
         if (rc < 0) return g_status.info;
         if (rc == 0) {
             mark(plan, s);
@@ -406,8 +408,8 @@ int ppmg_m2p_prebin(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, con
     if (np == 0) return 0;
     DevMesh m;
     if (make_mesh(mesh, prec, Spacing::Single, &m)) return g_status.info;
-    // only the 3D M'4 tiled gather reads a binning
-    if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return 0;
+    // only the 3D tiled gather (M'4 and BSP2 bin alike) reads a binning
+    if (m.dim != 3) return 0;
     int64_t nbad = 0;
     plan->have_forward = false;
     if (bin_particles(plan, m, prec, np, xp, as_stream(stream), &nbad)) return g_status.info;
@@ -564,8 +566,8 @@ int ppmg_rmsh_remesh(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, co
     // forces the general path, for cross-checks)
     const char* gen = std::getenv("PPMG_GENERAL");
     const bool force_general = gen && gen[0] && gen[0] != '0';
-    if (!force_general && p2m3d_fast_applicable(m, kernel, prec, fg)) {
-        if (p2m3d_fast(plan, m, prec, np, xp, fg, up, field->data, tabs, s)) return g_status.info;
+    if (!force_general && kernel == PPMG_KERNEL_MP4 && p2m3d_fast_applicable(m, kernel, prec, fg)) {
+        if (p2m3d_fast(plan, m, kernel, prec, np, xp, fg, up, field->data, tabs, s)) return g_status.info;
     } else if (p2m_general(plan, m, kernel, prec, np, xp, lda, up, fg, field->data, tabs, s)) {
         return g_status.info;
     }

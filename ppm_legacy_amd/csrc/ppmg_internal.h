@@ -209,6 +209,18 @@ __device__ __forceinline__ void mp4_factored(T t, T a[4]) {
     a[3] = (T(-0.5) * s) * t2;
 }
 
+// BSP2 weights in the reference's form (ppm_interp_p2m.f:3077-3104):
+// 1 - t and 1 + (t - 1); bsp2_at is weight o alone.
+template <typename T>
+__device__ __forceinline__ void bsp2_weights(T t, T a[2]) {
+    a[0] = T(1) - t;
+    a[1] = T(1) + (t - T(1));
+}
+template <typename T>
+__device__ __forceinline__ T bsp2_at(T t, int o) {
+    return o == 0 ? T(1) - t : T(1) + (t - T(1));
+}
+
 // d/dt of mp4_factored's weights, factored the same way (equal to
 // mp4_unrolled_deriv up to rounding): s (1.5 t - 0.5), t (4.5 t - 5),
 // s (5 - 4.5 s), t (1.5 t - 1).
@@ -231,6 +243,25 @@ __device__ __forceinline__ T mp4_at(T frac, int o) {
     const T in = T(1) + (T(-2.5) + T(1.5) * s) * (s * s);
     const T out = T(2) + (T(-4) + (T(2.5) - T(0.5) * s) * s) * s;
     return (o == 1 || o == 2) ? in : out;
+}
+
+// Stencil geometry of the tiled kernels per axis: KS nodes per particle (4:
+// M'4, 2: BSP2); a particle of cell c reaches nodes c - B .. c - B + KS - 1.
+template <int KS> struct Stencil {
+    static constexpr int H = KS - 1;       // halo cells a tile reads on the low side of an axis
+    static constexpr int B = KS == 4 ? 1 : 0;
+};
+
+// The KS 1D weights of a particle at fraction t of its cell, and weight o alone.
+template <int KS, typename T>
+__device__ __forceinline__ void tile_weights(T t, T (&a)[KS]) {
+    if constexpr (KS == 4) mp4_factored<T>(t, a);
+    else bsp2_weights<T>(t, a);
+}
+template <int KS, typename T>
+__device__ __forceinline__ T tile_weight_at(T t, int o) {
+    if constexpr (KS == 4) return mp4_at<T>(t, o);
+    else return bsp2_at<T>(t, o);
 }
 
 // M'4 by distance branch, the 2D form of ppm_interp_p2m.f:3620-3651.

@@ -18,21 +18,27 @@ int create_part_general(ppmg_plan* plan, const DevMesh& m, int prec, const Field
                         const double* cutoff, const double* cw, int resetpos, int64_t maxnp, int64_t* np_out,
                         void* xp, void* up, const FieldGeom* fw, const void* wfield, void* wp, hipStream_t s);
 
-// 3D M'4 tiled deposit (ppmg_p2m3d.hip): per axis (periodic or open) the
-// compact layout of a periodic axis or a stored box holding the real range 0..N.
+// 3D tiled deposit (ppmg_p2m3d.hip), M'4 or BSP2: per axis (periodic or open)
+// the compact layout of a periodic axis or a stored box holding the real range 0..N.
 bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg);
-// wtab: NULL (weights from the positions), or the remesh weight tables
-// (four per particle and axis, ppm_rmsh_comp_weights' layout).
-int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
-               const void* up, void* field, const void* const* wtab, hipStream_t s);
+// wtab: NULL (weights from the positions), or, for M'4 only, the remesh weight
+// tables (four per particle and axis, ppm_rmsh_comp_weights' layout).
+int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp,
+               const FieldGeom& fg, const void* up, void* field, const void* const* wtab, hipStream_t s);
 
-// 3D M'4 tiled gather (ppmg_m2p3d.hip), same layouts as the deposit: bins the
-// particles into the plan first (unless prebinned: the plan already holds
-// their binning); returns 1 without touching up when a particle lies outside
-// the sub.
+// 2D tiled deposit (ppmg_p2m2d.hip), M'4 or BSP2: per axis the compact layout
+// of a periodic axis or a stored box holding the real range 0..N.
+bool p2m2d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
+int p2m2d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, const void* xp, const FieldGeom& fg,
+               const void* up, void* field, hipStream_t s);
+
+// 3D tiled gather (ppmg_m2p3d.hip), M'4 or BSP2, same layouts as the deposit:
+// bins the particles into the plan first (unless prebinned: the plan already
+// holds their binning); returns 1 without touching up when a particle lies
+// outside the sub.
 bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
-int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
-               const void* field, void* up, bool prebinned, hipStream_t s);
+int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp,
+               const FieldGeom& fg, const void* field, void* up, bool prebinned, hipStream_t s);
 
 // Tiled 3D M'4 adjoint (ppmg_adj3d.hip), on the forward's binning; periodic
 // axes, each in the compact layout or a stored box holding 0..N.

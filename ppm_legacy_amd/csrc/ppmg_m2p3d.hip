@@ -36,11 +36,27 @@
 
 namespace ppmg {
 
-constexpr int M3_TX = 13;               // owned cells per tile along x (16 lanes = 13 + 3 node columns)
 constexpr int M3_TY = 16;               // owned cell rows per tile (one 16-lane row each)
-constexpr int M3_RY = M3_TY + 3;        // node rows in the ring
-constexpr int M3_PL = M3_RY * 16;       // node slots per ring plane and component
 constexpr int M3_THREADS = 16 * M3_TY;  // 256
+
+// Tile geometry for a stencil of KS nodes per axis (4: M'4, 2: BSP2): a
+// particle of cell c reads nodes c - B .. c - B + KS - 1.  Lane ex of a row
+// stands for node column tx0 - B + ex; the cells of lanes B .. B + TX - 1
+// carry particles.  The ring holds KS node planes of RY rows of 16 nodes.
+template <int KS> struct M3Geom {
+    static constexpr int B = KS == 4 ? 1 : 0;
+    static constexpr int TX = 17 - KS;            // owned cells per tile along x (13 or 15)
+    static constexpr int RY = M3_TY + KS - 1;     // node rows in the ring
+    static constexpr int PL = RY * 16;            // node slots per ring plane and component
+};
+
+// Value of lane ex + D of the same 16-lane row (D > 0: row_shl, D < 0: row_shr).
+template <int D, typename T>
+__device__ __forceinline__ T lane_from(T v) {
+    if constexpr (D > 0) return row_shl<D>(v);
+    else if constexpr (D < 0) return row_shr<-D>(v);
+    else return v;
+}
 
 // WPE: minimum wavefronts per SIMD the register allocation must allow (1: no
 // constraint).  FP64 needs 150 VGPRs at lda 3 and runs 3 wavefronts per SIMD;
@@ -73,11 +89,13 @@ __device__ __forceinline__ int raw_node(const DevMesh& m, const FieldGeom& fg, i
 
 // MIX: some open axis beside a periodic axis stored as a box, where whether a
 // node takes the periodic node map depends on all three of its coordinates.
-template <int LDA, typename PosT, typename ValT, typename AccT, int WPE, bool OPEN, bool MIX>
+template <int KS, int LDA, typename PosT, typename ValT, typename AccT, int WPE, bool OPEN, bool MIX>
 __global__ void __launch_bounds__(M3_THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
           const ValT* __restrict__ field, FieldGeom fg, ValT* __restrict__ up, int lda, int c0, int tz) {
-    __shared__ ValT ring[4 * LDA * M3_PL];   // [plane][c][ry][nx]
+    constexpr int B = M3Geom<KS>::B, M3_TX = M3Geom<KS>::TX, M3_PL = M3Geom<KS>::PL;
+    constexpr int RP = KS;                   // ring planes
+    __shared__ ValT ring[RP * LDA * M3_PL];  // [plane][c][ry][nx]
     const int tid = threadIdx.x;
     const int ex = tid & 15, row = tid >> 4;
     // tiles cover the cells: N per periodic axis, N+1 (0..N) per open one
@@ -105,10 +123,10 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         const int c = i % LDA;
         const int node = i / LDA;
         const int nx = node & 15, ry = node >> 4;
-        const int gx = stored_node<OPEN>(m, fg, 0, tx0 - 1 + nx) - fg.lo[0];
-        const int gy = stored_node<OPEN>(m, fg, 1, ty0 - 1 + ry) - fg.lo[1];
+        const int gx = stored_node<OPEN>(m, fg, 0, tx0 - B + nx) - fg.lo[0];
+        const int gy = stored_node<OPEN>(m, fg, 1, ty0 - B + ry) - fg.lo[1];
         soff[j] = (unsigned)(((gy * fg.ext[0] + gx) * lda + c0 + c) * (int)sizeof(ValT));
-        if (MIX && !(real_on_open_axis(m, 0, tx0 - 1 + nx) && real_on_open_axis(m, 1, ty0 - 1 + ry)))
+        if (MIX && !(real_on_open_axis(m, 0, tx0 - B + nx) && real_on_open_axis(m, 1, ty0 - B + ry)))
             soff[j] |= 0x80000000u;
     }
     // MIX: byte offset of element i of a plane read as stored
@@ -116,8 +134,8 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         const int c = i % LDA;
         const int node = i / LDA;
         const int nx = node & 15, ry = node >> 4;
-        const int gx = raw_node<OPEN>(m, fg, 0, tx0 - 1 + nx) - fg.lo[0];
-        const int gy = raw_node<OPEN>(m, fg, 1, ty0 - 1 + ry) - fg.lo[1];
+        const int gx = raw_node<OPEN>(m, fg, 0, tx0 - B + nx) - fg.lo[0];
+        const int gy = raw_node<OPEN>(m, fg, 1, ty0 - B + ry) - fg.lo[1];
         return (unsigned)(((gy * fg.ext[0] + gx) * lda + c0 + c) * (int)sizeof(ValT));
     };
     auto fetch_plane = [&](int n) {
@@ -144,22 +162,24 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         }
     };
     auto store_plane = [&](int n) {
-        ValT* dst = ring + (n & 3) * (LDA * M3_PL);
+        ValT* dst = ring + (n & (RP - 1)) * (LDA * M3_PL);
 #pragma unroll
         for (int j = 0; j < NLD; ++j) {
             const int i = tid + j * M3_THREADS;
             if (i < M3_PL * LDA) dst[(i % LDA) * M3_PL + i / LDA] = stage[j];
         }
     };
-    for (int n = z0 - 1; n <= z0 + 2; ++n) {
+    for (int n = z0 - B; n < z0 - B + KS; ++n) {
         fetch_plane(n);
         store_plane(n);
     }
 
-    const bool plane_lane = ex >= 1 && ex <= nxo && row < nyo;   // lanes that carry particles
-    const int cx = tx0 + ex - 1, cy = ty0 + row;
+    const bool plane_lane = ex >= B && ex - B < nxo && row < nyo;   // lanes that carry particles
+    const int cx = tx0 + ex - B, cy = ty0 + row;
     const int key0 = (int)((int64_t)cy * Nx + cx);
-    // Cell heads run one plane ahead of the computation.
+    // Cell heads run two planes ahead of the computation, the permutation
+    // entry of a plane's first particle one plane ahead, and its position is
+    // loaded at the end of the plane before.
     auto heads = [&](int z, int& f, int& n) {
         f = 0;
         n = 0;
@@ -169,23 +189,23 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             n = start[key + 1];   // end, subtracted where used: the load stays in flight
         }
     };
-    auto particle = [&](int f, int n, int& p, PosT (&x)[3]) {
-        p = 0;
+    auto lookup = [&](int f, int n) { return n > 0 ? (perm ? perm[f] : f) : 0; };
+    auto position = [&](int p, int n, PosT (&x)[3]) {
         x[0] = x[1] = x[2] = PosT(0);
         if (n > 0) {
-            p = perm ? perm[f] : f;
             x[0] = xp[(int64_t)p * 3];
             x[1] = xp[(int64_t)p * 3 + 1];
             x[2] = xp[(int64_t)p * 3 + 2];
         }
     };
-    int first_n, cnt_n;
+    int first_n, cnt_n, first_nn, cnt_nn;
     heads(z0, first_n, cnt_n);
+    heads(z0 + 1, first_nn, cnt_nn);
     // Round 0's particle of a plane is loaded at the end of the plane before,
     // while the workgroup refills the ring.
-    int pn;
+    int pn = lookup(first_n, cnt_n - first_n);
     PosT xn[3];
-    particle(first_n, cnt_n - first_n, pn, xn);
+    position(pn, cnt_n - first_n, xn);
     __syncthreads();
 
     for (int cz = z0; cz < z1; ++cz) {
@@ -193,26 +213,35 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         // take this plane's heads before the next loads are issued, so that
         // waiting for them does not wait for those loads too
         asm volatile("" ::"v"(first), "v"(cnt) : "memory");
-        heads(cz + 1, first_n, cnt_n);
+        first_n = first_nn;
+        cnt_n = cnt_nn;
+        const int pnext = lookup(first_n, cnt_n - first_n);   // next plane's first particle
+        heads(cz + 2, first_nn, cnt_nn);
         const bool more = cz + 1 < z1;
-        if (more) fetch_plane(cz + 3);
+        const int next = cz - B + KS;            // the plane that replaces plane cz - B
+        if (more) fetch_plane(next);
         // one round: the lanes of cells with a particle left interpolate it
         auto round = [&](int p, bool has, const PosT (&x)[3]) {
-            AccT wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, wz[4] = {0, 0, 0, 0};
+            AccT wx[KS], wy[KS], wz[KS];
+#pragma unroll
+            for (int o = 0; o < KS; ++o) wx[o] = wy[o] = wz[o] = AccT(0);
             if (has) {
                 int ci;
-                mp4_factored<AccT>((AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci), wx);
-                mp4_factored<AccT>((AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci), wy);
-                mp4_factored<AccT>((AccT)cell_offset<PosT>(x[2], (PosT)m.minv[2], (PosT)m.dxi[2], ci), wz);
+                tile_weights<KS, AccT>((AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci), wx);
+                tile_weights<KS, AccT>((AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci), wy);
+                tile_weights<KS, AccT>((AccT)cell_offset<PosT>(x[2], (PosT)m.minv[2], (PosT)m.dxi[2], ci), wz);
             }
-            // (y,z) weights of the particles of lanes ex+1 (k=0), ex (1), ex-1 (2), ex-2 (3)
-            AccT a[4][4], b[4][4];
+            // (y,z) weights of the particle whose stencil takes this lane's column
+            // as x offset k: that of lane ex + B - k (M'4: ex+1, ex, ex-1, ex-2)
+            AccT a[KS][KS], b[KS][KS];
 #pragma unroll
-            for (int o = 0; o < 4; ++o) {
-                a[0][o] = row_shl<1>(wy[o]); b[0][o] = row_shl<1>(wz[o]);
-                a[1][o] = wy[o];             b[1][o] = wz[o];
-                a[2][o] = row_shr<1>(wy[o]); b[2][o] = row_shr<1>(wz[o]);
-                a[3][o] = row_shr<2>(wy[o]); b[3][o] = row_shr<2>(wz[o]);
+            for (int o = 0; o < KS; ++o) {
+                a[0][o] = lane_from<B>(wy[o]);     b[0][o] = lane_from<B>(wz[o]);
+                a[1][o] = lane_from<B - 1>(wy[o]); b[1][o] = lane_from<B - 1>(wz[o]);
+                if constexpr (KS == 4) {
+                    a[2][o] = lane_from<B - 2>(wy[o]); b[2][o] = lane_from<B - 2>(wz[o]);
+                    a[3][o] = lane_from<B - 3>(wy[o]); b[3][o] = lane_from<B - 3>(wz[o]);
+                }
             }
             // per component: column partial sums Ck = sum_oz b_k[oz] * sum_oy a_k[oy] f(oz, oy)
             // (one 4-node y slice of the column live at a time), then the particle's sum;
@@ -220,24 +249,30 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             ValT res[LDA];
 #pragma unroll
             for (int c = 0; c < LDA; ++c) {
-                AccT C[4] = {AccT(0), AccT(0), AccT(0), AccT(0)};
+                AccT C[KS];
 #pragma unroll
-                for (int oz = 0; oz < 4; ++oz) {
-                    const ValT* f = ring + ((cz - 1 + oz) & 3) * (LDA * M3_PL) + c * M3_PL + row * 16 + ex;
-                    const AccT f0 = (AccT)f[0], f1 = (AccT)f[16], f2 = (AccT)f[32], f3 = (AccT)f[48];
+                for (int k = 0; k < KS; ++k) C[k] = AccT(0);
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        AccT t = a[k][0] * f0;
-                        t += a[k][1] * f1;
-                        t += a[k][2] * f2;
-                        t += a[k][3] * f3;
+                for (int oz = 0; oz < KS; ++oz) {
+                    const ValT* f = ring + ((cz - B + oz) & (RP - 1)) * (LDA * M3_PL) + c * M3_PL + row * 16 + ex;
+                    AccT fv[KS];
+#pragma unroll
+                    for (int oy = 0; oy < KS; ++oy) fv[oy] = (AccT)f[16 * oy];
+#pragma unroll
+                    for (int k = 0; k < KS; ++k) {
+                        AccT t = a[k][0] * fv[0];
+#pragma unroll
+                        for (int oy = 1; oy < KS; ++oy) t += a[k][oy] * fv[oy];
                         C[k] += b[k][oz] * t;
                     }
                 }
-                AccT u = wx[0] * row_shr<1>(C[0]);
-                u += wx[1] * C[1];
-                u += wx[2] * row_shl<1>(C[2]);
-                u += wx[3] * row_shl<2>(C[3]);
+                // x offset k of this lane's particle is the column of lane ex - B + k
+                AccT u = wx[0] * lane_from<-B>(C[0]);
+                u += wx[1] * lane_from<1 - B>(C[1]);
+                if constexpr (KS == 4) {
+                    u += wx[2] * lane_from<2 - B>(C[2]);
+                    u += wx[3] * lane_from<3 - B>(C[3]);
+                }
                 res[c] = (ValT)u;
                 // one component at a time: keeps the column reads of the next
                 // component after this one's sums (register pressure)
@@ -250,14 +285,15 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         };
         if (__ballot(cnt > 0) != 0ull) round(pn, cnt > 0, xn);
         for (int r = 1; __ballot(cnt > r) != 0ull; ++r) {
-            int p;
             PosT x[3];
-            particle(first + r, cnt - r, p, x);
+            const int p = lookup(first + r, cnt - r);
+            position(p, cnt - r, x);
             round(p, cnt > r, x);
         }
-        particle(first_n, cnt_n - first_n, pn, xn);   // next plane's round 0
+        pn = pnext;
+        position(pn, cnt_n - first_n, xn);   // next plane's round 0
         __syncthreads();   // every row is done with plane cz-1: its slot takes plane cz+3
-        if (more) store_plane(cz + 3);
+        if (more) store_plane(next);
         __syncthreads();
     }
 }
@@ -269,7 +305,9 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
 // there ghost_get maps a node only when its open coordinates are real nodes,
 // and the others are read as stored.
 bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
-    if (m.dim != 3 || kernel != PPMG_KERNEL_MP4) return false;
+    if (m.dim != 3 || (kernel != PPMG_KERNEL_MP4 && kernel != PPMG_KERNEL_BSP2)) return false;
+    // nodes a stencil reaches on an open axis (cells 0..N): -1..N+2 (M'4), 0..N+1 (BSP2)
+    const int below = kernel == PPMG_KERNEL_MP4 ? 1 : 0, above = kernel == PPMG_KERNEL_MP4 ? 2 : 1;
     bool open = false, swapped = false;
     for (int d = 0; d < 3; ++d) {
         open = open || !m.periodic[d];
@@ -278,7 +316,7 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
     // there a periodic box must hold every node a stencil reaches as stored
     if (open && swapped)
         for (int d = 0; d < 3; ++d)
-            if (fg.swap[d] && (fg.lo[d] > -1 || fg.lo[d] + fg.ext[d] - 1 < m.N[d] + 2)) return false;
+            if (fg.swap[d] && (fg.lo[d] > -below || fg.lo[d] + fg.ext[d] - 1 < m.N[d] + above)) return false;
     for (int d = 0; d < 3; ++d) {
         if (m.N[d] < 4) return false;
         const int hi = fg.lo[d] + fg.ext[d] - 1;
@@ -286,7 +324,7 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
             const bool compact = fg.lo[d] == 0 && fg.ext[d] == m.N[d];
             const bool boxed = fg.lo[d] <= 0 && hi >= m.N[d];
             if (!compact && !boxed) return false;
-        } else if (fg.lo[d] > -1 || hi < m.N[d] + 2) {
+        } else if (fg.lo[d] > -below || hi < m.N[d] + above) {
             return false;
         }
     }
@@ -296,13 +334,14 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
 
 // One launch of the gather for components c0 .. c0+LL-1; the z chunk comes
 // from the resident workgroup count (each chunk loads 3 extra node planes).
-template <int LL, typename PosT, typename ValT, typename AccT, bool OPEN, bool MIX>
+template <int KS, int LL, typename PosT, typename ValT, typename AccT, bool OPEN, bool MIX>
 static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
                            const ValT* field, ValT* up, hipStream_t s) {
-    auto kernel = k_m3_tile<LL, PosT, ValT, AccT, sizeof(ValT) == 4 ? 3 : 1, OPEN, MIX>;
+    constexpr int TX = M3Geom<KS>::TX;
+    auto kernel = k_m3_tile<KS, LL, PosT, ValT, AccT, sizeof(ValT) == 4 ? 3 : 1, OPEN, MIX>;
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(kernel, M3_THREADS));
-    const int tx = (m.nbin[0] + M3_TX - 1) / M3_TX, ty = (m.nbin[1] + M3_TY - 1) / M3_TY;
+    const int tx = (m.nbin[0] + TX - 1) / TX, ty = (m.nbin[1] + M3_TY - 1) / M3_TY;
     const int tz = pick_z_chunk((int64_t)tx * ty, m.nbin[2], slots.load(), 1);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.nbin[2] + tz - 1) / tz)));
     hipLaunchKernelGGL(kernel, grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
@@ -311,25 +350,25 @@ static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, co
 
 // OPEN: some axis is open (a fully periodic mesh compiles without the clamp);
 // MIX: and some periodic axis is stored as a box.
-template <int LL, typename PosT, typename ValT, typename AccT>
+template <int KS, int LL, typename PosT, typename ValT, typename AccT>
 static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
                       const ValT* field, ValT* up, hipStream_t s) {
     if (m.periodic[0] && m.periodic[1] && m.periodic[2])
-        launch_m3_open<LL, PosT, ValT, AccT, false, false>(plan, m, xp, fg, c0, field, up, s);
+        launch_m3_open<KS, LL, PosT, ValT, AccT, false, false>(plan, m, xp, fg, c0, field, up, s);
     else if (fg.swap[0] || fg.swap[1] || fg.swap[2])
-        launch_m3_open<LL, PosT, ValT, AccT, true, true>(plan, m, xp, fg, c0, field, up, s);
+        launch_m3_open<KS, LL, PosT, ValT, AccT, true, true>(plan, m, xp, fg, c0, field, up, s);
     else
-        launch_m3_open<LL, PosT, ValT, AccT, true, false>(plan, m, xp, fg, c0, field, up, s);
+        launch_m3_open<KS, LL, PosT, ValT, AccT, true, false>(plan, m, xp, fg, c0, field, up, s);
 }
 
-template <typename PosT, typename ValT, typename AccT>
+template <int KS, typename PosT, typename ValT, typename AccT>
 static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT* xp, const FieldGeom& fg,
                        const ValT* field, ValT* up, hipStream_t s) {
     const int lda = fg.lda;
     (void)np;
     for (int c0 = 0; c0 < lda; c0 += 4) {
         int L = lda - c0 < 4 ? lda - c0 : 4;
-#define TL(LL) launch_m3<LL, PosT, ValT, AccT>(plan, m, xp, fg, c0, field, up, s)
+#define TL(LL) launch_m3<KS, LL, PosT, ValT, AccT>(plan, m, xp, fg, c0, field, up, s)
         if (L == 1) TL(1);
         else if (L == 2) TL(2);
         else if (L == 3) TL(3);
@@ -348,20 +387,27 @@ static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT
 // issues at the FP32 rate, so the FP32-compensated form keeps FP32 only where
 // it saves bytes (field in HBM and LDS, output) and sums its 64 products in
 // FP64 registers -- more accurate than compensated FP32 at no extra cost.
-int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, const FieldGeom& fg,
-               const void* field, void* up, bool prebinned, hipStream_t s) {
+int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp,
+               const FieldGeom& fg, const void* field, void* up, bool prebinned, hipStream_t s) {
     int64_t nbad = prebinned ? plan->prebin_nbad : 0;
     if (!prebinned && bin_particles(plan, m, prec, np, xp, s, &nbad)) return PPMG_ERROR_FATAL;
     if (nbad > 0) return 1;
     mark(plan, s);
     if (np <= 0) return 0;
-    if (prec == PPMG_PREC_F64)
-        return m2p3d_typed<double, double, double>(plan, m, np, (const double*)xp, fg, (const double*)field,
-                                                   (double*)up, s);
-    if (prec == PPMG_PREC_F32C)
-        return m2p3d_typed<double, float, double>(plan, m, np, (const double*)xp, fg, (const float*)field,
-                                                  (float*)up, s);
-    return m2p3d_typed<float, float, double>(plan, m, np, (const float*)xp, fg, (const float*)field, (float*)up, s);
+#define TYPED(KS)                                                                                              \
+    do {                                                                                                       \
+        if (prec == PPMG_PREC_F64)                                                                             \
+            return m2p3d_typed<KS, double, double, double>(plan, m, np, (const double*)xp, fg,                 \
+                                                           (const double*)field, (double*)up, s);              \
+        if (prec == PPMG_PREC_F32C)                                                                            \
+            return m2p3d_typed<KS, double, float, double>(plan, m, np, (const double*)xp, fg,                  \
+                                                          (const float*)field, (float*)up, s);                 \
+        return m2p3d_typed<KS, float, float, double>(plan, m, np, (const float*)xp, fg, (const float*)field,   \
+                                                     (float*)up, s);                                           \
+    } while (0)
+    if (kernel == PPMG_KERNEL_BSP2) TYPED(2);
+    TYPED(4);
+#undef TYPED
 }
 
 }  // namespace ppmg

@@ -1,0 +1,173 @@
+// ppmg_tilebox.h -- the stored nodes of a field box besides the ones the
+// tiled deposits write (2D and 3D, M'4 and BSP2): node N of periodic axes
+// whose stored box holds the whole real range, and the ghost layers.  Shared
+// by ppmg_p2m2d.hip and ppmg_p2m3d.hip.
+#pragma once
+#include "ppmg_internal.h"
+#include "ppmg_sort.h"
+
+namespace ppmg {
+
+// Weight tables of ppm_rmsh_remesh (wx1/wx2/wx3, four per particle and axis),
+// used in place of the positions' weights (3D M'4 only).
+template <typename ValT> struct WTab {
+    const ValT* w[3];
+};
+
+// Stored nodes besides the unique ones, for axes whose stored box holds the
+// whole real range 0..N.  Per axis: R = stored real nodes (N compact, N+1
+// boxed), O = ext - R stored nodes outside 0..N.
+
+// Real nodes with index N on a boxed periodic axis (n.d = N; -1 on other
+// axes): the fold leaves node N equal to node 0, so they take the value the
+// tiles wrote there.  Enumerated as the face i = N, then j = N (i < N), then
+// k = N (i, j < N), over the real nodes (r per axis, b = 1 where node N is a
+// copy).
+template <typename ValT>
+__global__ void __launch_bounds__(256) k_box_dup(FieldGeom fg, int3 n, int3 r, int3 b, ValT* __restrict__ field,
+                                                int64_t ndup) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ndup) return;
+    const int64_t fx = b.x ? (int64_t)r.y * r.z : 0;
+    const int64_t fy = b.y ? (int64_t)(r.x - b.x) * r.z : 0;
+    int i, j, k;
+    if (t < fx) {
+        i = n.x;
+        j = (int)(t % r.y);
+        k = (int)(t / r.y);
+    } else if (t < fx + fy) {
+        const int64_t q = t - fx;
+        i = (int)(q % (r.x - b.x));
+        j = n.y;
+        k = (int)(q / (r.x - b.x));
+    } else {
+        const int64_t q = t - fx - fy;
+        i = (int)(q % (r.x - b.x));
+        j = (int)((q / (r.x - b.x)) % (r.y - b.y));
+        k = n.z;
+    }
+    auto at = [&](int x, int y, int z) {
+        return (((int64_t)(z - fg.lo[2]) * fg.ext[1] + (y - fg.lo[1])) * fg.ext[0] + (x - fg.lo[0])) * fg.lda;
+    };
+    const int64_t dst = at(i, j, k);
+    const int64_t src = at(i == n.x ? 0 : i, j == n.y ? 0 : j, k == n.z ? 0 : k);
+    for (int c = 0; c < fg.lda; ++c) field[dst + c] = field[src + c];
+}
+
+// Stored nodes outside 0..N on some axis (ghost layers): the raw sum of the
+// particles whose stencil reaches the node with no periodic wrap on any axis,
+// which is what ppm_map_field_ghost_put leaves in ghost nodes (the general
+// path's rule for nodes that are not real).  Enumerated as the planes outside
+// 0..N in z, then the rows outside in y (z real), then the nodes outside in x
+// (y, z real).  Weights and products in the value precision, sums in FP64;
+// with weight tables (wt.w[0] set) the weights are the tables'.  KS: stencil
+// width (4 M'4, 2 BSP2); DIM 2: one plane (no z stencil), positions of two
+// coordinates.
+template <int DIM, int KS, typename PosT, typename ValT, int LL>
+__global__ void __launch_bounds__(256) k_box_shell(DevMesh m, FieldGeom fg, int3 r, const int* __restrict__ start,
+                                                  const int* __restrict__ perm, const PosT* __restrict__ xp,
+                                                  const ValT* __restrict__ up, int c0, ValT* __restrict__ field,
+                                                  int64_t nshell, WTab<ValT> wt) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nshell) return;
+    const int e0 = fg.ext[0], e1 = fg.ext[1], e2 = fg.ext[2];
+    const int o0 = e0 - r.x, o1 = e1 - r.y, o2 = e2 - r.z;
+    // q-th stored node of axis d outside the real range: below 0, then above N
+    auto outer = [&](int d, int q) { return q < -fg.lo[d] ? fg.lo[d] + q : m.N[d] + 1 + (q + fg.lo[d]); };
+    const int64_t ra = (int64_t)o2 * e1 * e0, rb = (int64_t)r.z * o1 * e0;
+    int n[3];
+    if (t < ra) {
+        n[0] = fg.lo[0] + (int)(t % e0);
+        n[1] = fg.lo[1] + (int)((t / e0) % e1);
+        n[2] = outer(2, (int)(t / ((int64_t)e0 * e1)));
+    } else if (t < ra + rb) {
+        const int64_t q = t - ra;
+        n[0] = fg.lo[0] + (int)(q % e0);
+        n[1] = outer(1, (int)((q / e0) % o1));
+        n[2] = (int)(q / ((int64_t)e0 * o1));
+    } else {
+        const int64_t q = t - ra - rb;
+        n[0] = outer(0, (int)(q % o0));
+        n[1] = (int)((q / o0) % r.y);
+        n[2] = (int)(q / ((int64_t)o0 * r.y));
+    }
+    double acc[LL];
+#pragma unroll
+    for (int c = 0; c < LL; ++c) acc[c] = 0.0;
+    constexpr int B = Stencil<KS>::B;
+    constexpr int KZ = DIM == 3 ? KS : 1;
+    for (int oz = 0; oz < KZ; ++oz) {
+        const int cz = DIM == 3 ? n[2] + B - oz : 0;
+        if (cz < 0 || cz >= m.nbin[2]) continue;
+        for (int oy = 0; oy < KS; ++oy) {
+            const int cy = n[1] + B - oy;
+            if (cy < 0 || cy >= m.nbin[1]) continue;
+            for (int ox = 0; ox < KS; ++ox) {
+                const int cx = n[0] + B - ox;
+                if (cx < 0 || cx >= m.nbin[0]) continue;
+                const int key = cx + m.nbin[0] * (cy + m.nbin[1] * cz);
+                const int b = start[key], e = start[key + 1];
+                for (int q = b; q < e; ++q) {
+                    const int p = perm ? perm[q] : q;
+                    ValT w;
+                    if (DIM == 3 && KS == 4 && wt.w[0]) {
+                        w = (wt.w[0][(int64_t)p * 4 + ox] * wt.w[1][(int64_t)p * 4 + oy]) * wt.w[2][(int64_t)p * 4 + oz];
+                    } else {
+                        ValT fr[DIM];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) {
+                            int ci;
+                            fr[d] = (ValT)cell_offset<PosT>(xp[(int64_t)p * DIM + d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
+                        }
+                        w = tile_weight_at<KS, ValT>(fr[0], ox) * tile_weight_at<KS, ValT>(fr[1], oy);
+                        if constexpr (DIM == 3) w = w * tile_weight_at<KS, ValT>(fr[2], oz);
+                    }
+#pragma unroll
+                    for (int c = 0; c < LL; ++c) acc[c] += (double)(w * up[(int64_t)p * fg.lda + c0 + c]);
+                }
+            }
+        }
+    }
+    ValT* out = field + (((int64_t)(n[2] - fg.lo[2]) * e1 + (n[1] - fg.lo[1])) * e0 + (n[0] - fg.lo[0])) * fg.lda + c0;
+#pragma unroll
+    for (int c = 0; c < LL; ++c) out[c] = (ValT)acc[c];
+}
+
+// Fills the stored nodes of `field` that the tiles do not write (see above),
+// after the tile launches of a deposit.
+template <int DIM, int KS, typename PosT, typename ValT>
+int fill_stored_box(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const FieldGeom& fg,
+                    ValT* field, const WTab<ValT>& wt, hipStream_t s) {
+    const int lda = fg.lda;
+    int R[3], B[3], D[3];
+    for (int d = 0; d < 3; ++d) {
+        const bool compact = m.periodic[d] && fg.lo[d] == 0 && fg.ext[d] == m.N[d];
+        B[d] = d < DIM && m.periodic[d] && !compact;
+        R[d] = d >= DIM ? 1 : compact ? m.N[d] : m.N[d] + 1;
+        D[d] = B[d] ? m.N[d] : -1;
+    }
+    const int3 n3 = make_int3(D[0], D[1], D[2]), r3 = make_int3(R[0], R[1], R[2]),
+               b3 = make_int3(B[0], B[1], B[2]);
+    const int64_t ndup = (int64_t)R[0] * R[1] * R[2] - (int64_t)(R[0] - B[0]) * (R[1] - B[1]) * (R[2] - B[2]);
+    if (ndup > 0)
+        hipLaunchKernelGGL(k_box_dup<ValT>, dim3((unsigned)((ndup + 255) / 256)), dim3(256), 0, s, fg, n3, r3, b3,
+                           field, ndup);
+    const int64_t nshell = (int64_t)fg.ext[0] * fg.ext[1] * fg.ext[2] - (int64_t)R[0] * R[1] * R[2];
+    if (nshell > 0) {
+        const unsigned g = (unsigned)((nshell + 255) / 256);
+        for (int c0 = 0; c0 < lda; c0 += 4) {
+            const int L = lda - c0 < 4 ? lda - c0 : 4;
+#define SH(LL) hipLaunchKernelGGL((k_box_shell<DIM, KS, PosT, ValT, LL>), dim3(g), dim3(256), 0, s, m, fg, r3, \
+                                  plan->start.as<int>(), tile_perm(plan), xp, up, c0, field, nshell, wt)
+            if (L == 1) SH(1);
+            else if (L == 2) SH(2);
+            else if (L == 3) SH(3);
+            else SH(4);
+#undef SH
+        }
+    }
+    PPMG_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace ppmg

@@ -215,29 +215,30 @@ def test_p2m_compact_equals_padded_and_is_deterministic(core, monkeypatch):
     assert _rel(padded.cpu().numpy(), gen.cpu().numpy()) < 1e-13   # ghosts included
 
 
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
 @pytest.mark.parametrize("prec", [F64, F32C, F32])
 @pytest.mark.parametrize("lda", [1, 3, 4, 7])
-def test_p2m_tiled_fast_path(core, oracle, prec, lda, monkeypatch):
-    """Tiled 3D M'4 deposit (compact periodic layout) vs the oracle and vs the general path."""
-    N = 40                                  # 13-node tiles leave a ragged last tile
+def test_p2m_tiled_fast_path(core, oracle, kernel, prec, lda, monkeypatch):
+    """Tiled 3D deposit (M'4 and BSP2, compact periodic layout) vs the oracle and vs the general path."""
+    N = 40                                  # 13- and 15-node tiles leave a ragged last tile
     xp = wl.advect(wl.jittered_lattice(N, seed=8), wl.fourier_modes(8), N)[0]
     up = wl.strengths(8, len(xp), lda)
     t = oracle.topo(3, [N + 1] * 3)
     odt = np.float32 if prec == F32 else np.float64
-    info, ref = oracle.p2m(t, MP4, xp.astype(odt), up.astype(odt), [1, 1, 1], dtype=odt)
+    info, ref = oracle.p2m(t, kernel, xp.astype(odt), up.astype(odt), [1, 1, 1], dtype=odt)
     assert info == 0
     mesh = core.Mesh(3, N + 1)
     x, u = _dev(xp, _pos_dtype(prec)), _dev(up, _val_dtype(prec))
     fast = torch.empty(mesh.compact_shape(lda), dtype=_val_dtype(prec), device="cuda")
-    core.p2m(mesh, x, u, fast, [0] * 3, MP4, prec)
+    core.p2m(mesh, x, u, fast, [0] * 3, kernel, prec)
     assert _rel(fast.cpu().numpy(), oracle.unique_nodes(ref, [1, 1, 1], 3)) < TOL[prec]
     monkeypatch.setenv("PPMG_GENERAL", "1")
     gen = torch.empty_like(fast)
-    core.p2m(mesh, x, u, gen, [0] * 3, MP4, prec)
+    core.p2m(mesh, x, u, gen, [0] * 3, kernel, prec)
     monkeypatch.delenv("PPMG_GENERAL")
     assert _rel(fast.cpu().numpy(), gen.cpu().numpy()) < (1e-13 if prec == F64 else 1e-6)
     again = torch.empty_like(fast)
-    core.p2m(mesh, x, u, again, [0] * 3, MP4, prec)
+    core.p2m(mesh, x, u, again, [0] * 3, kernel, prec)
     assert torch.equal(fast, again)
 
 
@@ -292,10 +293,11 @@ def test_p2m_tile_heights_components(core, rows, lda, monkeypatch):
 GEN_TOL = {F64: 1e-13, F32C: 1e-6, F32: 1e-5}
 
 
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
 @pytest.mark.parametrize("prec", [F64, F32C, F32])
 @pytest.mark.parametrize("N", [5, 12, 40])
 @pytest.mark.parametrize("g", [0, 1, 2])
-def test_tiled_paths_padded_layout(core, N, g, prec, monkeypatch):
+def test_tiled_paths_padded_layout(core, N, g, prec, kernel, monkeypatch):
     """The tiled kernels on the reference's stored box (lo = -g, ext = N+1+2g)
     against the general paths: P2M on every stored node (node N equal to node
     0, ghost layers the raw unwrapped sums), and M2P of a random stored field
@@ -310,15 +312,15 @@ def test_tiled_paths_padded_layout(core, N, g, prec, monkeypatch):
     vdt = _val_dtype(prec)
     x, u = _dev(xp, _pos_dtype(prec)), _dev(up, vdt)
     fast = torch.full(mesh.padded_shape(lda, [g] * 3), 7.0, dtype=vdt, device="cuda")
-    core.p2m(mesh, x, u, fast, lo, MP4, prec)
+    core.p2m(mesh, x, u, fast, lo, kernel, prec)
     field = _dev(rng.normal(size=mesh.padded_shape(lda, [g] * 3)), vdt)
     out = torch.empty((len(xp), lda), dtype=vdt, device="cuda")
-    core.m2p(mesh, x, field, lo, out, MP4, prec)
+    core.m2p(mesh, x, field, lo, out, kernel, prec)
     monkeypatch.setenv("PPMG_GENERAL", "1")
     gen = torch.full_like(fast, 7.0)
-    core.p2m(mesh, x, u, gen, lo, MP4, prec)
+    core.p2m(mesh, x, u, gen, lo, kernel, prec)
     out_gen = torch.empty_like(out)
-    core.m2p(mesh, x, field, lo, out_gen, MP4, prec)
+    core.m2p(mesh, x, field, lo, out_gen, kernel, prec)
     monkeypatch.delenv("PPMG_GENERAL")
     assert _rel(fast.cpu().numpy(), gen.cpu().numpy()) < GEN_TOL[prec]
     s = slice(g, g + N)
@@ -326,11 +328,12 @@ def test_tiled_paths_padded_layout(core, N, g, prec, monkeypatch):
     assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < GEN_TOL[prec]
 
 
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
 @pytest.mark.parametrize("prec", [F64, F32C, F32])
 @pytest.mark.parametrize("bc", [[1, 1, 1, 1, 2, 2], [2, 2, 1, 1, 1, 1], [2, 2, 2, 2, 2, 2]],
                          ids=["open_z", "open_x", "all_open"])
 @pytest.mark.parametrize("N", [5, 12, 40])
-def test_tiled_paths_open_axes(core, oracle, bc, N, prec, monkeypatch):
+def test_tiled_paths_open_axes(core, oracle, bc, N, prec, kernel, monkeypatch):
     """Meshes with non-periodic axes on the tiled kernels: on an open axis the
     tiles make the N+1 real nodes 0..N with no wrap, ghost nodes keep their raw
     sums, and the gather reads node v itself.  Against the general paths on
@@ -357,28 +360,28 @@ def test_tiled_paths_open_axes(core, oracle, bc, N, prec, monkeypatch):
     for g in (0, 1, 2):
         lo = [-g] * 3
         fast = torch.full(mesh.padded_shape(lda, [g] * 3), 7.0, dtype=vdt, device="cuda")
-        core.p2m(mesh, x, u, fast, lo, MP4, prec)
+        core.p2m(mesh, x, u, fast, lo, kernel, prec)
         monkeypatch.setenv("PPMG_GENERAL", "1")
         gen = torch.full_like(fast, 7.0)
-        core.p2m(mesh, x, u, gen, lo, MP4, prec)
+        core.p2m(mesh, x, u, gen, lo, kernel, prec)
         monkeypatch.delenv("PPMG_GENERAL")
         assert _rel(fast.cpu().numpy(), gen.cpu().numpy()) < GEN_TOL[prec], g
         if g == 2:
-            info, ref = oracle.p2m(t, MP4, xp.astype(odt), up.astype(odt), [2, 2, 2], dtype=odt)
+            info, ref = oracle.p2m(t, kernel, xp.astype(odt), up.astype(odt), [2, 2, 2], dtype=odt)
             assert info == 0
             assert _rel(fast.cpu().numpy(), ref) < TOL[prec]
     fr = rng.normal(size=mesh.padded_shape(lda, [2] * 3))
     field = _dev(fr, vdt)
     out = torch.empty((npart, lda), dtype=vdt, device="cuda")
-    core.m2p(mesh, x, field, [-2] * 3, out, MP4, prec)
+    core.m2p(mesh, x, field, [-2] * 3, out, kernel, prec)
     monkeypatch.setenv("PPMG_GENERAL", "1")
     out_gen = torch.empty_like(out)
-    core.m2p(mesh, x, field, [-2] * 3, out_gen, MP4, prec)
+    core.m2p(mesh, x, field, [-2] * 3, out_gen, kernel, prec)
     monkeypatch.delenv("PPMG_GENERAL")
     assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < GEN_TOL[prec]
     # the oracle runs ghost_get first: beside an open axis it fills only the
     # nodes whose open coordinates are real nodes, the rest read as stored
-    info, qref = oracle.m2p(t, MP4, xp.astype(odt), fr.astype(odt), [2] * 3, dtype=odt)
+    info, qref = oracle.m2p(t, kernel, xp.astype(odt), fr.astype(odt), [2] * 3, dtype=odt)
     assert info == 0
     assert _rel(out.cpu().numpy(), qref) < TOL[prec]
     if any(b == 1 for b in bc):
@@ -387,16 +390,62 @@ def test_tiled_paths_open_axes(core, oracle, bc, N, prec, monkeypatch):
         lo = [0 if bc[2 * d] == 1 else -2 for d in range(3)]
         shape = tuple(reversed([N if bc[2 * d] == 1 else N + 5 for d in range(3)])) + (lda,)
         fm = _dev(rng.normal(size=shape), vdt)
-        core.m2p(mesh, x, fm, lo, out, MP4, prec)
+        core.m2p(mesh, x, fm, lo, out, kernel, prec)
         pm = torch.full(shape, 7.0, dtype=vdt, device="cuda")
-        core.p2m(mesh, x, u, pm, lo, MP4, prec)
+        core.p2m(mesh, x, u, pm, lo, kernel, prec)
         monkeypatch.setenv("PPMG_GENERAL", "1")
-        core.m2p(mesh, x, fm, lo, out_gen, MP4, prec)
+        core.m2p(mesh, x, fm, lo, out_gen, kernel, prec)
         pg = torch.full_like(pm, 7.0)
-        core.p2m(mesh, x, u, pg, lo, MP4, prec)
+        core.p2m(mesh, x, u, pg, lo, kernel, prec)
         monkeypatch.delenv("PPMG_GENERAL")
         assert _rel(out.cpu().numpy(), out_gen.cpu().numpy()) < GEN_TOL[prec]
         assert _rel(pm.cpu().numpy(), pg.cpu().numpy()) < GEN_TOL[prec]
+
+
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+@pytest.mark.parametrize("prec", [F64, F32C, F32])
+@pytest.mark.parametrize("bc", [[1, 1, 1, 1], [1, 1, 2, 2], [2, 2, 2, 2]], ids=["periodic", "open_y", "all_open"])
+@pytest.mark.parametrize("N", [5, 12, 40, 301])
+def test_p2m2d_tiled(core, oracle, bc, N, prec, kernel, monkeypatch):
+    """The 2D tiled deposit (13- or 15-node strips marching four cell rows a
+    step) on the compact layout and on the reference's stored box (ghost
+    sizes 0..2), periodic and open axes: every stored node against the
+    general path, the ghost-size-2 box against the oracle, and two launches
+    bitwise equal.  N = 40 leaves a ragged last strip; N = 301 splits the rows
+    into several chunks.  A tenth of the particles sit on a face of an open
+    axis."""
+    rng = np.random.default_rng(11 * N + sum(bc) + kernel)
+    lda = 3
+    npart = max(3 * N * N, 2000)
+    xp = rng.uniform(0.0, 1.0, size=(npart, 2))
+    for d in range(2):
+        if bc[2 * d] != 1:
+            xp[rng.uniform(size=npart) < 0.05, d] = 1.0
+            xp[rng.uniform(size=npart) < 0.05, d] = 0.0
+    up = rng.normal(size=(npart, lda))
+    vdt = _val_dtype(prec)
+    odt = np.float32 if prec == F32 else np.float64
+    mesh = core.Mesh(2, N + 1, bcdef=bc)
+    x, u = _dev(xp, _pos_dtype(prec)), _dev(up, vdt)
+    layouts = [(g, [-g] * 2, mesh.padded_shape(lda, [g] * 2)) for g in (0, 1, 2)]
+    if all(b == 1 for b in bc):
+        layouts.append(("compact", [0, 0], mesh.compact_shape(lda)))
+    for g, lo, shape in layouts:
+        fast = torch.full(shape, 7.0, dtype=vdt, device="cuda")
+        core.p2m(mesh, x, u, fast, lo, kernel, prec)
+        again = torch.full(shape, 7.0, dtype=vdt, device="cuda")
+        core.p2m(mesh, x, u, again, lo, kernel, prec)
+        assert torch.equal(fast, again), g
+        monkeypatch.setenv("PPMG_GENERAL", "1")
+        gen = torch.full_like(fast, 7.0)
+        core.p2m(mesh, x, u, gen, lo, kernel, prec)
+        monkeypatch.delenv("PPMG_GENERAL")
+        assert _rel(fast.cpu().numpy(), gen.cpu().numpy()) < GEN_TOL[prec], g
+        if g == 2:
+            t = oracle.topo(2, [N + 1] * 2, bcdef=bc)
+            info, ref = oracle.p2m(t, kernel, xp.astype(odt), up.astype(odt), [2, 2], dtype=odt)
+            assert info == 0
+            assert _rel(fast.cpu().numpy(), ref) < TOL[prec]
 
 
 @pytest.mark.parametrize("bc", [[1] * 6, [2] * 6, [1, 1, 1, 1, 2, 2]], ids=["periodic", "all_open", "open_z"])
@@ -486,8 +535,9 @@ def test_empty_inputs(core, oracle):
     assert xo.shape == (0, 3) and uo.shape == (0, 2)
 
 
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
 @pytest.mark.parametrize("prec", [F64, F32C])
-def test_tiled_paths_noncubic_box(core, oracle, prec, monkeypatch):
+def test_tiled_paths_noncubic_box(core, oracle, prec, kernel, monkeypatch):
     """The three tiled kernels on a non-cubic mesh over a non-unit box: 40 x 29
     x 18 cells on [0,2] x [-1,0.5] x [0.25,1] (a different cell count, spacing
     and origin per axis, ragged tiles in x and y, a z extent under one chunk).
@@ -502,9 +552,9 @@ def test_tiled_paths_noncubic_box(core, oracle, prec, monkeypatch):
     xp, xq = box(12000), box(5000)
     up = rng.normal(size=(len(xp), lda))
     t = oracle.topo(3, [n + 1 for n in Ns], mn, mx)
-    info, ref = oracle.p2m(t, MP4, xp, up, [1, 1, 1])
+    info, ref = oracle.p2m(t, kernel, xp, up, [1, 1, 1])
     assert info == 0
-    info, uref = oracle.m2p(t, MP4, xq, ref, [1, 1, 1])
+    info, uref = oracle.m2p(t, kernel, xq, ref, [1, 1, 1])
     assert info == 0
     mesh = core.Mesh(3, [n + 1 for n in Ns], mn, mx)
     vdt = _val_dtype(prec)
@@ -512,21 +562,21 @@ def test_tiled_paths_noncubic_box(core, oracle, prec, monkeypatch):
     x, u = _dev(xp, _pos_dtype(prec)), _dev(up, vdt)
     field = torch.empty(mesh.compact_shape(lda), dtype=vdt, device="cuda")
     assert field.shape == (18, 29, 40, lda)
-    plan = core.p2m(mesh, x, u, field, [0] * 3, MP4, prec)
+    plan = core.p2m(mesh, x, u, field, [0] * 3, kernel, prec)
     assert _rel(field.double().cpu().numpy(), oracle.unique_nodes(ref, [1, 1, 1], 3)) < tol
     out = torch.empty((len(xq), lda), dtype=vdt, device="cuda")
-    core.m2p(mesh, _dev(xq, _pos_dtype(prec)), field, [0] * 3, out, MP4, prec)
+    core.m2p(mesh, _dev(xq, _pos_dtype(prec)), field, [0] * 3, out, kernel, prec)
     assert _rel(out.double().cpu().numpy(), uref) < tol
     gcot = rng.normal(size=field.shape)
-    info, gu_ref, gx_ref = oracle.p2m_adjoint(t, MP4, xp, up, gcot)
+    info, gu_ref, gx_ref = oracle.p2m_adjoint(t, kernel, xp, up, gcot)
     assert info == 0
     gc = _dev(gcot, vdt)
-    gu, gx = core.p2m_adjoint(mesh, x, u, gc, [0] * 3, plan, MP4, prec)
+    gu, gx = core.p2m_adjoint(mesh, x, u, gc, [0] * 3, plan, kernel, prec)
     tol_adj = 1e-12 if prec == F64 else 2e-6
     assert _rel(gu.double().cpu().numpy(), gu_ref) < tol_adj
     assert _rel(gx.double().cpu().numpy(), gx_ref) < tol_adj
     monkeypatch.setenv("PPMG_GENERAL", "1")
-    gu2, gx2 = core.p2m_adjoint(mesh, x, u, gc, [0] * 3, plan, MP4, prec)
+    gu2, gx2 = core.p2m_adjoint(mesh, x, u, gc, [0] * 3, plan, kernel, prec)
     monkeypatch.delenv("PPMG_GENERAL")
     assert _rel(gu.double().cpu().numpy(), gu2.double().cpu().numpy()) < (1e-13 if prec == F64 else 1e-6)
     assert _rel(gx.double().cpu().numpy(), gx2.double().cpu().numpy()) < (1e-13 if prec == F64 else 1e-6)
@@ -670,10 +720,12 @@ def test_m2p_compact_layout(core):
     assert _rel(up.cpu().numpy(), bf.m2p_periodic(xp, uniq, N)) < 1e-12
 
 
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
 @pytest.mark.parametrize("prec", [F64, F32C, F32])
 @pytest.mark.parametrize("lda", [1, 3, 4, 7])
-def test_m2p_tiled_fast_path(core, oracle, prec, lda, monkeypatch):
-    """Tiled 3D M'4 gather (compact periodic layout, particles in random order) vs oracle and general path."""
+def test_m2p_tiled_fast_path(core, oracle, kernel, prec, lda, monkeypatch):
+    """Tiled 3D gather (M'4 and BSP2, compact periodic layout, particles in
+    random order) vs the brute-force sum and the general path."""
     N = 40
     rng = np.random.default_rng(31 + lda)
     uniq = rng.normal(size=(N, N, N, lda))
@@ -682,14 +734,14 @@ def test_m2p_tiled_fast_path(core, oracle, prec, lda, monkeypatch):
     x = _dev(xp, _pos_dtype(prec))
     f = _dev(uniq, _val_dtype(prec))
     fast = torch.empty((len(xp), lda), dtype=_val_dtype(prec), device="cuda")
-    core.m2p(mesh, x, f, [0] * 3, fast, MP4, prec)
+    core.m2p(mesh, x, f, [0] * 3, fast, kernel, prec)
     monkeypatch.setenv("PPMG_GENERAL", "1")
     gen = torch.empty_like(fast)
-    core.m2p(mesh, x, f, [0] * 3, gen, MP4, prec)
+    core.m2p(mesh, x, f, [0] * 3, gen, kernel, prec)
     monkeypatch.delenv("PPMG_GENERAL")
     tol = 1e-13 if prec == F64 else 1e-6
     assert _rel(fast.cpu().numpy(), gen.cpu().numpy()) < tol
-    ref = bf.m2p_periodic(xp.astype(np.float32).astype(np.float64) if prec == F32 else xp, uniq, N)
+    ref = bf.m2p_periodic(xp.astype(np.float32).astype(np.float64) if prec == F32 else xp, uniq, N, kernel)
     assert _rel(fast.cpu().numpy(), ref) < (1e-12 if prec == F64 else 1e-5)
 
 

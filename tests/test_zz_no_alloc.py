@@ -80,7 +80,7 @@ def test_noalloc_matches_p2m_twice(core, monkeypatch, kernel, prec):
     monkeypatch.delenv("PPMG_FAIL_ALLOC_ABOVE")
 
 
-def test_noalloc_rejects_what_does_not_fit(core):
+def test_noalloc_rejects_what_does_not_fit(core, monkeypatch):
     from ppm_legacy_amd import _lib
     # 6M particles need 72 MB for keys, slots and the permutation alone
     N = 64
@@ -94,16 +94,26 @@ def test_noalloc_rejects_what_does_not_fit(core):
     assert torch.all(field == 2.0)
     # the general path's sorted weights (48 B per particle for BSP2 in 3D,
     # FP64) do not fit beside the binning of 1.5M particles: ppm_err_alloc,
-    # field untouched
+    # field untouched (the tiled deposit, which keeps no sorted copies, would
+    # fit, so the general path is forced)
     n = 1_500_000
     mesh2 = core.Mesh(3, N + 1)
     x2 = torch.rand(n, 3, dtype=torch.float64, device="cuda")
     u2 = torch.ones(n, 3, dtype=torch.float64, device="cuda")
     f2 = torch.full(mesh2.padded_shape(3, [2, 2, 2]), 2.0, dtype=torch.float64, device="cuda")
+    monkeypatch.setenv("PPMG_GENERAL", "1")
     info, code = _noalloc(core, mesh2, x2, u2, f2, [-2, -2, -2], BSP2, 0)
+    monkeypatch.delenv("PPMG_GENERAL")
     assert info == -1 and code == _lib.ERR_ALLOC
     torch.cuda.synchronize()
     assert torch.all(f2 == 2.0)
+    # on the tiled deposit the same call fits and matches ppmg_p2m
+    ref2 = torch.empty_like(f2)
+    core.p2m(mesh2, x2, u2, ref2, [-2, -2, -2], BSP2, 0, core.Plan())
+    info, code = _noalloc(core, mesh2, x2, u2, f2, [-2, -2, -2], BSP2, 0)
+    assert info == 0, code
+    torch.cuda.synchronize()
+    assert torch.equal(f2, ref2)
     # and the entry point still works afterwards
     mesh3, x3, u3 = _case(core, 16, 16 ** 3, 1, 0, 21)
     ref = torch.empty(mesh3.compact_shape(1), dtype=torch.float64, device="cuda")
