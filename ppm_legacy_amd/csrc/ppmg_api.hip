@@ -373,8 +373,11 @@ int ppmg_m2p(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     mark(plan, s, true);
     const char* gen = std::getenv("PPMG_GENERAL");
     const bool force_general = gen && gen[0] && gen[0] != '0';
-    if (!force_general && m2p3d_fast_applicable(m, kernel, fg)) {
-        int rc = m2p3d_fast(plan, m, kernel, prec, np, xp, fg, field->data, up, prebinned, s);
+    const bool tiled3 = !force_general && m2p3d_fast_applicable(m, kernel, fg);
+    const bool tiled2 = !force_general && !tiled3 && m2p2d_fast_applicable(m, kernel, fg);
+    if (tiled3 || tiled2) {
+        int rc = tiled3 ? m2p3d_fast(plan, m, kernel, prec, np, xp, fg, field->data, up, prebinned, s)
+                        : m2p2d_fast(plan, m, kernel, prec, np, xp, fg, field->data, up, s);
         if (rc < 0) return g_status.info;
         if (rc == 0) {
             mark(plan, s);

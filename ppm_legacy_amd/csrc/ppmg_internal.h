@@ -161,6 +161,31 @@ __device__ __forceinline__ float row_shl(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + K, 0xF, 0xF, true));
 }
 
+// Value of lane ex + D of the same 16-lane row (D > 0: row_shl, D < 0: row_shr).
+template <int D, typename T>
+__device__ __forceinline__ T lane_from(T v) {
+    if constexpr (D > 0) return row_shl<D>(v);
+    else if constexpr (D < 0) return row_shr<-D>(v);
+    else return v;
+}
+
+// Stored node read for node v along axis d, as the general path reads it:
+// on a periodic axis the compact layout wraps, and a stored box holding node
+// 0 and node N reads with ghost_get's node map (ppm_map_field_ghost_get.f:
+// 257-280: node 0 and the ghosts below it take node v+N, node N and the ghosts
+// above it v-N); an open axis reads node v itself (beside a periodic box the
+// MIX form reads some nodes through raw_node instead).  Stencils reach -1..N+2;
+// the ring of a tile also loads columns beyond that which no stencil reads:
+// they wrap (periodic) or are clamped into the box (open).
+template <bool OPEN>
+__device__ __forceinline__ int stored_node(const DevMesh& m, const FieldGeom& fg, int d, int v) {
+    const int N = m.N[d];
+    if (OPEN && !m.periodic[d]) return min(max(v, fg.lo[d]), fg.lo[d] + fg.ext[d] - 1);
+    if (fg.swap[d] && v > -N && v < 2 * N) return v <= 0 ? v + N : (v >= N ? v - N : v);
+    v %= N;
+    return v < 0 ? v + N : v;
+}
+
 // Exclusive prefix sum of v over the 64 lanes of a wavefront (lane = this
 // lane's index).
 __device__ __forceinline__ int wave_excl_prefix(int v, int lane) {

@@ -40,6 +40,14 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
 int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp,
                const FieldGeom& fg, const void* field, void* up, bool prebinned, hipStream_t s);
 
+// 2D tiled gather (ppmg_m2p2d.hip), M'4 or BSP2: periodic axes compact or a
+// stored box holding 0..N, open axes a box holding every node a stencil
+// reaches (no periodic box beside an open axis).  Bins first; returns 1
+// without touching up when a particle lies outside the sub.
+bool m2p2d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
+int m2p2d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp,
+               const FieldGeom& fg, const void* field, void* up, hipStream_t s);
+
 // Tiled 3D M'4 adjoint (ppmg_adj3d.hip), on the forward's binning; periodic
 // axes, each in the compact layout or a stored box holding 0..N.
 bool adjoint3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);

@@ -50,35 +50,6 @@ template <int KS> struct M3Geom {
     static constexpr int PL = RY * 16;            // node slots per ring plane and component
 };
 
-// Value of lane ex + D of the same 16-lane row (D > 0: row_shl, D < 0: row_shr).
-template <int D, typename T>
-__device__ __forceinline__ T lane_from(T v) {
-    if constexpr (D > 0) return row_shl<D>(v);
-    else if constexpr (D < 0) return row_shr<-D>(v);
-    else return v;
-}
-
-// WPE: minimum wavefronts per SIMD the register allocation must allow (1: no
-// constraint).  FP64 needs 150 VGPRs at lda 3 and runs 3 wavefronts per SIMD;
-// the FP32 ring (FP32c and FP32 forms) is allowed 3 and comes out at 126-132
-// VGPRs (FP32c: 4 wavefronts), with no spills.
-// Stored node read for node v along axis d, as the general path reads it:
-// on a periodic axis the compact layout wraps, and a stored box holding node
-// 0 and node N reads with ghost_get's node map (ppm_map_field_ghost_get.f:
-// 257-280: node 0 and the ghosts below it take node v+N, node N and the ghosts
-// above it v-N); an open axis reads node v itself (beside a periodic box the
-// MIX form reads some nodes through raw_node instead).  Stencils reach -1..N+2;
-// the ring of a tile also loads columns beyond that which no stencil reads:
-// they wrap (periodic) or are clamped into the box (open).
-template <bool OPEN>
-__device__ __forceinline__ int stored_node(const DevMesh& m, const FieldGeom& fg, int d, int v) {
-    const int N = m.N[d];
-    if (OPEN && !m.periodic[d]) return min(max(v, fg.lo[d]), fg.lo[d] + fg.ext[d] - 1);
-    if (fg.swap[d] && v > -N && v < 2 * N) return v <= 0 ? v + N : (v >= N ? v - N : v);
-    v %= N;
-    return v < 0 ? v + N : v;
-}
-
 // Stored node read as stored: v itself on a stored box (clamped into it for
 // the ring columns no stencil reads); a compact periodic axis wraps.
 template <bool OPEN>
@@ -87,6 +58,10 @@ __device__ __forceinline__ int raw_node(const DevMesh& m, const FieldGeom& fg, i
     return stored_node<OPEN>(m, fg, d, v);
 }
 
+// WPE: minimum wavefronts per SIMD the register allocation must allow (1: no
+// constraint).  FP64 needs 150 VGPRs at lda 3 and runs 3 wavefronts per SIMD;
+// the FP32 ring (FP32c and FP32 forms) is allowed 3 and comes out at 126-132
+// VGPRs (FP32c: 4 wavefronts), with no spills.
 // MIX: some open axis beside a periodic axis stored as a box, where whether a
 // node takes the periodic node map depends on all three of its coordinates.
 template <int KS, int LDA, typename PosT, typename ValT, typename AccT, int WPE, bool OPEN, bool MIX>

@@ -448,6 +448,50 @@ def test_p2m2d_tiled(core, oracle, bc, N, prec, kernel, monkeypatch):
             assert _rel(fast.cpu().numpy(), ref) < TOL[prec]
 
 
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+@pytest.mark.parametrize("prec", [F64, F32C, F32])
+@pytest.mark.parametrize("bc", [[1, 1, 1, 1], [1, 1, 2, 2], [2, 2, 2, 2]], ids=["periodic", "open_y", "all_open"])
+@pytest.mark.parametrize("N", [5, 12, 40, 301])
+def test_m2p2d_tiled(core, oracle, bc, N, prec, kernel, monkeypatch):
+    """The 2D tiled gather on random stored fields whose ghosts, node N and
+    node 0 all differ: the reference's stored box (ghost size 2) against the
+    oracle (which runs ghost_get first) and the general path; on a periodic
+    mesh also ghost sizes 0 and 1 and the compact layout against the general
+    path; particles in random order, a tenth of them on faces of open axes.
+    Beside an open axis a periodic box takes the general path (same result)."""
+    rng = np.random.default_rng(13 * N + sum(bc) + kernel)
+    lda = 3
+    npart = max(3 * N * N, 2000)
+    xp = rng.uniform(0.0, 1.0, size=(npart, 2))
+    for d in range(2):
+        if bc[2 * d] != 1:
+            xp[rng.uniform(size=npart) < 0.05, d] = 1.0
+            xp[rng.uniform(size=npart) < 0.05, d] = 0.0
+    vdt = _val_dtype(prec)
+    odt = np.float32 if prec == F32 else np.float64
+    mesh = core.Mesh(2, N + 1, bcdef=bc)
+    x = _dev(xp, _pos_dtype(prec))
+    layouts = [(2, [-2, -2], mesh.padded_shape(lda, [2, 2]))]
+    if all(b == 1 for b in bc):
+        layouts += [(g, [-g] * 2, mesh.padded_shape(lda, [g] * 2)) for g in (0, 1)]
+        layouts.append(("compact", [0, 0], mesh.compact_shape(lda)))
+    for g, lo, shape in layouts:
+        fr = rng.normal(size=shape)
+        f = _dev(fr, vdt)
+        out = torch.full((npart, lda), 7.0, dtype=vdt, device="cuda")
+        core.m2p(mesh, x, f, lo, out, kernel, prec)
+        monkeypatch.setenv("PPMG_GENERAL", "1")
+        gen = torch.full_like(out, 7.0)
+        core.m2p(mesh, x, f, lo, gen, kernel, prec)
+        monkeypatch.delenv("PPMG_GENERAL")
+        assert _rel(out.cpu().numpy(), gen.cpu().numpy()) < GEN_TOL[prec], g
+        if g == 2:
+            t = oracle.topo(2, [N + 1] * 2, bcdef=bc)
+            info, ref = oracle.m2p(t, kernel, xp.astype(odt), fr.astype(odt), [2, 2], dtype=odt)
+            assert info == 0
+            assert _rel(out.cpu().numpy(), ref) < TOL[prec]
+
+
 @pytest.mark.parametrize("bc", [[1] * 6, [2] * 6, [1, 1, 1, 1, 2, 2]], ids=["periodic", "all_open", "open_z"])
 @pytest.mark.parametrize("N", [2, 3, 4])
 def test_smallest_meshes(core, oracle, bc, N, monkeypatch):
