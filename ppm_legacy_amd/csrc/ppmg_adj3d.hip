@@ -96,24 +96,24 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const int nxo = min(A3_TX, Nx - tx0), nyo = min(A3_TY, Ny - ty0);
     const int64_t plane = (int64_t)Nx * Ny;
 
-    constexpr int NLD = (A3_PL * LDA + A3_THREADS - 1) / A3_THREADS;
+    // A thread stages whole nodes (all LDA components, adjacent in the field):
+    // the ring stores of one component then write consecutive nodes.
+    constexpr int NN = (A3_PL + A3_THREADS - 1) / A3_THREADS;   // nodes per thread and plane
     // (BOXED reads each plane straight into the ring between the barriers)
-    ValT stage[BOXED ? 1 : NLD];
-    // Byte offset within a field plane of each element this thread stages
-    // (the same in every plane; a plane fits in 2^31 bytes, checked at launch).
-    unsigned soff[BOXED ? 1 : NLD];
+    ValT stage[BOXED ? 1 : NN][LDA];
+    // Byte offset within a field plane of each node this thread stages (the
+    // same in every plane; a plane fits in 2^31 bytes, checked at launch).
+    unsigned soff[BOXED ? 1 : NN];
     if constexpr (!BOXED) {
 #pragma unroll
-        for (int j = 0; j < NLD; ++j) {
-            const int i = tid + j * A3_THREADS;
-            const int c = i % LDA;
-            const int node = i / LDA;
+        for (int j = 0; j < NN; ++j) {
+            const int node = tid + j * A3_THREADS;
             const int nx = node & 15, ry = node >> 4;
             int gx = (tx0 - 1 + nx) % Nx;
             if (gx < 0) gx += Nx;
             int gy = (ty0 - 1 + ry) % Ny;
             if (gy < 0) gy += Ny;
-            soff[j] = (unsigned)(((gy * Nx + gx) * lda + c0 + c) * (int)sizeof(ValT));
+            soff[j] = (unsigned)(((gy * Nx + gx) * lda + c0) * (int)sizeof(ValT));
         }
     }
     auto fetch_plane = [&](int n) {
@@ -121,8 +121,12 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         if (nz < 0) nz += Nz;
         const char* base = (const char*)(g + (int64_t)nz * plane * lda);
 #pragma unroll
-        for (int j = 0; j < (BOXED ? 0 : NLD); ++j) {
-            if (tid + j * A3_THREADS < A3_PL * LDA) stage[j] = *(const ValT*)(base + soff[j]);
+        for (int j = 0; j < (BOXED ? 0 : NN); ++j) {
+            if (tid + j * A3_THREADS < A3_PL) {
+                const ValT* src = (const ValT*)(base + soff[j]);
+#pragma unroll
+                for (int c = 0; c < LDA; ++c) stage[j][c] = src[c];
+            }
         }
     };
     // BOXED: plane n of the ring read through box_cotangent straight into LDS
@@ -137,9 +141,12 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     auto store_plane = [&](int n) {
         ValT* dst = ring + (n & 3) * (LDA * A3_PL);
 #pragma unroll
-        for (int j = 0; j < NLD; ++j) {
-            const int i = tid + j * A3_THREADS;
-            if (i < A3_PL * LDA) dst[(i % LDA) * A3_PL + i / LDA] = stage[j];
+        for (int j = 0; j < (BOXED ? 0 : NN); ++j) {
+            const int node = tid + j * A3_THREADS;
+            if (node < A3_PL) {
+#pragma unroll
+                for (int c = 0; c < LDA; ++c) dst[c * A3_PL + node] = stage[j][c];
+            }
         }
     };
     for (int n = z0 - 1; n <= z0 + 2; ++n) {
@@ -154,6 +161,10 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const bool plane_lane = ex >= 1 && ex <= nxo && row < nyo;
     const int cx = tx0 + ex - 1, cy = ty0 + row;
     const int key0 = (int)((int64_t)cy * Nx + cx);
+    // Cell heads run two planes ahead of the computation, the permutation
+    // entry of a plane's first particle one plane ahead, and that particle's
+    // position and strengths are loaded at the end of the plane before (while
+    // the workgroup refills the ring; they are not live during the rounds).
     auto heads = [&](int z, int& f, int& n) {
         f = 0;
         n = 0;
@@ -163,8 +174,26 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             n = start[key + 1];   // end, subtracted where used: the load stays in flight
         }
     };
-    int first_n, cnt_n;
+    auto lookup = [&](int f, int n) { return n > 0 ? (perm ? perm[f] : f) : 0; };
+    auto particle = [&](int p, int n, PosT (&x)[3], ValT (&u)[LDA]) {
+        x[0] = x[1] = x[2] = PosT(0);
+#pragma unroll
+        for (int c = 0; c < LDA; ++c) u[c] = ValT(0);
+        if (n > 0) {
+            x[0] = xp[(int64_t)p * 3];
+            x[1] = xp[(int64_t)p * 3 + 1];
+            x[2] = xp[(int64_t)p * 3 + 2];
+#pragma unroll
+            for (int c = 0; c < LDA; ++c) u[c] = up[(int64_t)p * lda + c0 + c];
+        }
+    };
+    int first_n, cnt_n, first_nn, cnt_nn;
     heads(z0, first_n, cnt_n);
+    heads(z0 + 1, first_nn, cnt_nn);
+    int pn = lookup(first_n, cnt_n - first_n);
+    PosT xn[3];
+    ValT un[LDA];
+    particle(pn, cnt_n - first_n, xn, un);
     __syncthreads();
 
     for (int cz = z0; cz < z1; ++cz) {
@@ -172,20 +201,19 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         // take this plane's heads before the next loads are issued, so that
         // waiting for them does not wait for those loads too
         asm volatile("" ::"v"(first), "v"(cnt) : "memory");
-        heads(cz + 1, first_n, cnt_n);
+        first_n = first_nn;
+        cnt_n = cnt_nn;
+        const int pnext = lookup(first_n, cnt_n - first_n);   // next plane's first particle
+        heads(cz + 2, first_nn, cnt_nn);
         const bool more = cz + 1 < z1;
         if (!BOXED && more) fetch_plane(cz + 3);
-        for (int r = 0; __ballot(cnt > r) != 0ull; ++r) {
-            const bool has = cnt > r;
-            int p = 0;
+        auto round = [&](int p, bool has, const PosT (&x)[3], const ValT (&uv)[LDA]) {
             AccT wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, wz[4] = {0, 0, 0, 0};
             AccT dx[4] = {0, 0, 0, 0}, dy[4] = {0, 0, 0, 0}, dz[4] = {0, 0, 0, 0};
             AccT u[LDA];
 #pragma unroll
             for (int c = 0; c < LDA; ++c) u[c] = AccT(0);
             if (has) {
-                p = perm ? perm[first + r] : first + r;
-                const PosT* x = xp + (int64_t)p * 3;
                 int ci;
                 const AccT f0 = (AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci);
                 const AccT f1 = (AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci);
@@ -203,7 +231,7 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                     dz[o] *= (AccT)m.dxi[2];
                 }
 #pragma unroll
-                for (int c = 0; c < LDA; ++c) u[c] = (AccT)up[(int64_t)p * lda + c0 + c];
+                for (int c = 0; c < LDA; ++c) u[c] = (AccT)uv[c];
             }
             // One pass per stencil x-offset k: the y and z weights (and
             // derivatives) of the particle k-1 lanes to the right arrive by DPP,
@@ -272,7 +300,17 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                     o[2] = (ValT)gx2;
                 }
             }
+        };
+        if (__ballot(cnt > 0) != 0ull) round(pn, cnt > 0, xn, un);
+        for (int r = 1; __ballot(cnt > r) != 0ull; ++r) {
+            PosT x[3];
+            ValT u[LDA];
+            const int p = lookup(first + r, cnt - r);
+            particle(p, cnt - r, x, u);
+            round(p, cnt > r, x, u);
         }
+        pn = pnext;
+        particle(pn, cnt_n - first_n, xn, un);   // next plane's round 0
         __syncthreads();   // every row is done with plane cz-1: its slot takes plane cz+3
         if (more) {
             if constexpr (BOXED) fetch_store_boxed(cz + 3);
@@ -308,22 +346,21 @@ static void launch_a3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const V
     else launch_a3_form<LL, PosT, ValT, AccT, true>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
 }
 
-// One launch per four components (the position gradient accumulates over
-// launches when lda > 4); inside a launch the components are contracted one
-// after another and the stencil x-offsets one per pass, which keeps the
-// kernel at 3 wavefronts per SIMD.
+// One launch per three components (the position gradient accumulates over
+// launches when lda > 3: four components in one launch would spill); inside a
+// launch the components are contracted one after another and the stencil
+// x-offsets one per pass, which keeps the kernel at 3 wavefronts per SIMD.
 template <typename PosT, typename ValT>
 static int adjoint3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT* xp, int lda, const ValT* up,
                            const ValT* g, const FieldGeom& fg, ValT* gu, ValT* gx, hipStream_t s) {
     (void)np;
-    for (int c0 = 0; c0 < lda; c0 += 4) {
-        const int L = lda - c0 < 4 ? lda - c0 : 4;
+    for (int c0 = 0; c0 < lda; c0 += 3) {
+        const int L = lda - c0 < 3 ? lda - c0 : 3;
         const int acc = c0 > 0 ? 1 : 0;
 #define AL(LL) launch_a3<LL, PosT, ValT, double>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s)
         if (L == 1) AL(1);
         else if (L == 2) AL(2);
-        else if (L == 3) AL(3);
-        else AL(4);
+        else AL(3);
 #undef AL
         PPMG_HIP(hipGetLastError());
     }

@@ -58,10 +58,10 @@ __device__ __forceinline__ int raw_node(const DevMesh& m, const FieldGeom& fg, i
     return stored_node<OPEN>(m, fg, d, v);
 }
 
-// WPE: minimum wavefronts per SIMD the register allocation must allow (1: no
-// constraint).  FP64 needs 150 VGPRs at lda 3 and runs 3 wavefronts per SIMD;
-// the FP32 ring (FP32c and FP32 forms) is allowed 3 and comes out at 126-132
-// VGPRs (FP32c: 4 wavefronts), with no spills.
+// WPE: minimum wavefronts per SIMD the register allocation must allow: 4 (at
+// most 128 VGPRs) where that needs no spills, else 3 (M'4 with four
+// components or the MIX form).  At 512^3, lda 3, FP64 the gather takes 2.94 ms
+// with 4 wavefronts against 3.18 ms with 3 (136 VGPRs unconstrained).
 // MIX: some open axis beside a periodic axis stored as a box, where whether a
 // node takes the periodic node map depends on all three of its coordinates.
 template <int KS, int LDA, typename PosT, typename ValT, typename AccT, int WPE, bool OPEN, bool MIX>
@@ -84,64 +84,64 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const int64_t plane = (int64_t)Nx * Ny;
 
     // Ring planes move global -> registers -> LDS so that the global reads of
-    // plane cz+3 are in flight while plane cz is computed.
-    constexpr int NLD = (M3_PL * LDA + M3_THREADS - 1) / M3_THREADS;
-    ValT stage[NLD];
-    // Byte offset within a field plane of each element this thread stages
-    // (the same in every plane; a plane fits in 2^31 bytes, checked at launch).
-    // MIX: bit 31 marks an element whose x or y is not a real node of an open
+    // plane cz+3 are in flight while plane cz is computed.  A thread stages
+    // whole nodes (all LDA components, adjacent in the field): the stores of
+    // one component then write consecutive nodes, free of bank conflicts.
+    constexpr int NN = (M3_PL + M3_THREADS - 1) / M3_THREADS;   // nodes per thread and plane
+    ValT stage[NN][LDA];
+    // Byte offset within a field plane of each node this thread stages (the
+    // same in every plane; a plane fits in 2^31 bytes, checked at launch).
+    // MIX: bit 31 marks a node whose x or y is not a real node of an open
     // axis, which every plane reads as stored.
-    unsigned soff[NLD];
+    unsigned soff[NN];
 #pragma unroll
-    for (int j = 0; j < NLD; ++j) {
-        const int i = tid + j * M3_THREADS;
-        const int c = i % LDA;
-        const int node = i / LDA;
+    for (int j = 0; j < NN; ++j) {
+        const int node = tid + j * M3_THREADS;
         const int nx = node & 15, ry = node >> 4;
         const int gx = stored_node<OPEN>(m, fg, 0, tx0 - B + nx) - fg.lo[0];
         const int gy = stored_node<OPEN>(m, fg, 1, ty0 - B + ry) - fg.lo[1];
-        soff[j] = (unsigned)(((gy * fg.ext[0] + gx) * lda + c0 + c) * (int)sizeof(ValT));
+        soff[j] = (unsigned)(((gy * fg.ext[0] + gx) * lda + c0) * (int)sizeof(ValT));
         if (MIX && !(real_on_open_axis(m, 0, tx0 - B + nx) && real_on_open_axis(m, 1, ty0 - B + ry)))
             soff[j] |= 0x80000000u;
     }
-    // MIX: byte offset of element i of a plane read as stored
-    auto raw_off = [&](int i) -> unsigned {
-        const int c = i % LDA;
-        const int node = i / LDA;
+    // MIX: byte offset of node `node` of a plane read as stored
+    auto raw_off = [&](int node) -> unsigned {
         const int nx = node & 15, ry = node >> 4;
         const int gx = raw_node<OPEN>(m, fg, 0, tx0 - B + nx) - fg.lo[0];
         const int gy = raw_node<OPEN>(m, fg, 1, ty0 - B + ry) - fg.lo[1];
-        return (unsigned)(((gy * fg.ext[0] + gx) * lda + c0 + c) * (int)sizeof(ValT));
+        return (unsigned)(((gy * fg.ext[0] + gx) * lda + c0) * (int)sizeof(ValT));
     };
     auto fetch_plane = [&](int n) {
         const int nz = stored_node<OPEN>(m, fg, 2, n) - fg.lo[2];
         const char* base = (const char*)(field + (int64_t)nz * fg.ext[0] * fg.ext[1] * lda);
+        const char* braw = base;
+        bool rz = true;
         if (MIX) {
-            const bool rz = real_on_open_axis(m, 2, n);
+            rz = real_on_open_axis(m, 2, n);
             const int nzr = raw_node<OPEN>(m, fg, 2, n) - fg.lo[2];
-            const char* braw = (const char*)(field + (int64_t)nzr * fg.ext[0] * fg.ext[1] * lda);
-#pragma unroll
-            for (int j = 0; j < NLD; ++j) {
-                const int i = tid + j * M3_THREADS;
-                if (i < M3_PL * LDA) {
-                    if (rz && (soff[j] >> 31) == 0u) stage[j] = *(const ValT*)(base + soff[j]);
-                    else stage[j] = *(const ValT*)(braw + raw_off(i));
-                }
-            }
-            return;
+            braw = (const char*)(field + (int64_t)nzr * fg.ext[0] * fg.ext[1] * lda);
         }
-        // component fastest in the field (coalesced), component-major in the ring
 #pragma unroll
-        for (int j = 0; j < NLD; ++j) {
-            if (tid + j * M3_THREADS < M3_PL * LDA) stage[j] = *(const ValT*)(base + soff[j]);
+        for (int j = 0; j < NN; ++j) {
+            const int node = tid + j * M3_THREADS;
+            if (node < M3_PL) {
+                const ValT* src;
+                if (MIX && !(rz && (soff[j] >> 31) == 0u)) src = (const ValT*)(braw + raw_off(node));
+                else src = (const ValT*)(base + soff[j]);
+#pragma unroll
+                for (int c = 0; c < LDA; ++c) stage[j][c] = src[c];
+            }
         }
     };
     auto store_plane = [&](int n) {
         ValT* dst = ring + (n & (RP - 1)) * (LDA * M3_PL);
 #pragma unroll
-        for (int j = 0; j < NLD; ++j) {
-            const int i = tid + j * M3_THREADS;
-            if (i < M3_PL * LDA) dst[(i % LDA) * M3_PL + i / LDA] = stage[j];
+        for (int j = 0; j < NN; ++j) {
+            const int node = tid + j * M3_THREADS;
+            if (node < M3_PL) {
+#pragma unroll
+                for (int c = 0; c < LDA; ++c) dst[c * M3_PL + node] = stage[j][c];
+            }
         }
     };
     for (int n = z0 - B; n < z0 - B + KS; ++n) {
@@ -313,7 +313,8 @@ template <int KS, int LL, typename PosT, typename ValT, typename AccT, bool OPEN
 static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
                            const ValT* field, ValT* up, hipStream_t s) {
     constexpr int TX = M3Geom<KS>::TX;
-    auto kernel = k_m3_tile<KS, LL, PosT, ValT, AccT, sizeof(ValT) == 4 ? 3 : 1, OPEN, MIX>;
+    constexpr int WPE = KS == 4 && (LL == 4 || MIX) ? 3 : 4;
+    auto kernel = k_m3_tile<KS, LL, PosT, ValT, AccT, WPE, OPEN, MIX>;
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(kernel, M3_THREADS));
     const int tx = (m.nbin[0] + TX - 1) / TX, ty = (m.nbin[1] + M3_TY - 1) / M3_TY;
