@@ -47,7 +47,72 @@ def parse():
                     help="c5 (default): the advected lattice; c4: 2^26 particles in 4 Gaussian blobs + uniform, "
                          "P2M then M2P of the same particles")
     ap.add_argument("--no-box", action="store_true", help="skip the stored-box (reference layout) leg")
+    ap.add_argument("--paths", action="store_true",
+                    help="only time P2M and M2P per path family, tiled against the general path: 3D BSP2 on "
+                         "the 512^3 advected lattice, 2D M'4 and BSP2 on an 8192^2 jittered lattice")
     return ap.parse_args()
+
+
+def paths_run(core, wl, torch, args):
+    """P2M and M2P (FP64, lda 3, binning included) of the tiled kernels against
+    the general node-gather / per-particle paths (PPMG_GENERAL=1) on the same
+    inputs: 3D BSP2 on the advected 512^3 lattice (M'4 beside it), 2D M'4 and
+    BSP2 on an 8192^2 lattice jittered by +-0.3h and displaced by up to
+    +-0.2h.  Median of three calls after one warm-up, CUDA events around each
+    call on the current stream."""
+    lda = args.lda
+
+    def one(mesh, x, w, kernel, shape, general):
+        if general:
+            os.environ["PPMG_GENERAL"] = "1"
+        else:
+            os.environ.pop("PPMG_GENERAL", None)
+        field = torch.empty(shape, dtype=torch.float64, device="cuda")
+        out = torch.empty_like(w)
+        pp, pm = core.Plan(), core.Plan()
+        res = {"p2m": [], "m2p": []}
+        for it in range(4):
+            for name, call in (("p2m", lambda: core.p2m(mesh, x, w, field, [0] * mesh.dim, kernel, 0, pp)),
+                               ("m2p", lambda: core.m2p(mesh, x, field, [0] * mesh.dim, out, kernel, 0, pm))):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                call()
+                e1.record()
+                torch.cuda.synchronize()
+                if it > 0:
+                    res[name].append(e0.elapsed_time(e1))
+        os.environ.pop("PPMG_GENERAL", None)
+        return {k: sorted(v)[1] for k, v in res.items()}
+
+    out = {}
+    N = args.n
+    mesh, x, w, xf, _ = make_inputs(core, wl, N, lda, args.seed, torch)
+    del xf
+    for name, kernel in (("3d_mp4", 2), ("3d_bsp2", 1)):
+        out[name] = {"cells": f"{N}^3", "particles": N ** 3,
+                     "tiled_ms": one(mesh, x, w, kernel, mesh.compact_shape(lda), False),
+                     "general_ms": one(mesh, x, w, kernel, mesh.compact_shape(lda), True)}
+        print(f"# paths {name}: {out[name]}", file=sys.stderr, flush=True)
+    del mesh, x, w
+    N2 = 8192 if N == 512 else 16 * N
+    n2 = N2 * N2
+    mesh2 = core.Mesh(2, N2 + 1)
+    gen = torch.Generator(device="cuda").manual_seed(args.seed)
+    idx = torch.arange(n2, device="cuda")
+    cells = torch.stack([idx % N2, idx // N2], 1).double()
+    del idx
+    x2 = (cells + 0.5 + 0.6 * (torch.rand((n2, 2), generator=gen, device="cuda", dtype=torch.float64) - 0.5)) / N2
+    del cells
+    x2 += 0.4 / N2 * (torch.rand((n2, 2), generator=gen, device="cuda", dtype=torch.float64) - 0.5)
+    x2 = (x2 - torch.floor(x2)).contiguous()
+    w2 = torch.randn((n2, lda), generator=gen, device="cuda", dtype=torch.float64)
+    for name, kernel in (("2d_mp4", 2), ("2d_bsp2", 1)):
+        out[name] = {"cells": f"{N2}^2", "particles": n2,
+                     "tiled_ms": one(mesh2, x2, w2, kernel, mesh2.compact_shape(lda), False),
+                     "general_ms": one(mesh2, x2, w2, kernel, mesh2.compact_shape(lda), True)}
+        print(f"# paths {name}: {out[name]}", file=sys.stderr, flush=True)
+    return {"paths": out, "dtype": "f64", "lda": lda,
+            "note": "ms per call, binning included; tiled = the default routing, general = PPMG_GENERAL=1"}
 
 
 def slab_cycle(core, torch, steps, warmup, lda, seed):
@@ -313,6 +378,9 @@ def main():
         return
     if args.config == "c4":
         print(json.dumps(clustered_run(core, wl, torch, args)))
+        return
+    if args.paths:
+        print(json.dumps(paths_run(core, wl, torch, args)))
         return
     N, lda = args.n, args.lda
     np_ = N ** 3

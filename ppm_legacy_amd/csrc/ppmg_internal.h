@@ -186,18 +186,6 @@ __device__ __forceinline__ int stored_node(const DevMesh& m, const FieldGeom& fg
     return v < 0 ? v + N : v;
 }
 
-// Exclusive prefix sum of v over the 64 lanes of a wavefront (lane = this
-// lane's index).
-__device__ __forceinline__ int wave_excl_prefix(int v, int lane) {
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    return x - v;
-}
-
 // Orders one wavefront's LDS writes before its later LDS reads by other lanes
 // of the same wavefront (a wavefront's LDS instructions execute in order; this
 // keeps the compiler from moving them across).
