@@ -326,7 +326,7 @@ static void launch_a3_form(ppmg_plan* plan, const DevMesh& m, const PosT* xp, co
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(k_a3_tile<LL, PosT, ValT, AccT, BOXED>, A3_THREADS));
     const int tx = (m.N[0] + A3_TX - 1) / A3_TX, ty = (m.N[1] + A3_TY - 1) / A3_TY;
-    const int tz = pick_z_chunk((int64_t)tx * ty, m.N[2], slots.load(), 1);
+    const int tz = pick_z_chunk_balanced((int64_t)tx * ty, m.N[2], slots.load(), 32);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
     hipLaunchKernelGGL((k_a3_tile<LL, PosT, ValT, AccT, BOXED>), grid, dim3(A3_THREADS), 0, s, m,
                        plan->start.as<int>(), tile_perm(plan), xp, up, g, fg, gu, gx, lda, c0, tz, acc);

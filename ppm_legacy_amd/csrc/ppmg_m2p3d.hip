@@ -308,7 +308,8 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
 }
 
 // One launch of the gather for components c0 .. c0+LL-1; the z chunk comes
-// from the resident workgroup count (each chunk loads 3 extra node planes).
+// from the resident workgroup count (pick_z_chunk_balanced: each chunk loads
+// 3 extra node planes, cheap beside the balance short chunks give).
 template <int KS, int LL, typename PosT, typename ValT, typename AccT, bool OPEN, bool MIX>
 static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
                            const ValT* field, ValT* up, hipStream_t s) {
@@ -318,7 +319,7 @@ static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, co
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(kernel, M3_THREADS));
     const int tx = (m.nbin[0] + TX - 1) / TX, ty = (m.nbin[1] + M3_TY - 1) / M3_TY;
-    const int tz = pick_z_chunk((int64_t)tx * ty, m.nbin[2], slots.load(), 1);
+    const int tz = pick_z_chunk_balanced((int64_t)tx * ty, m.nbin[2], slots.load(), 32);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.nbin[2] + tz - 1) / tz)));
     hipLaunchKernelGGL(kernel, grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
                        tile_perm(plan), xp, field, fg, up, fg.lda, c0, tz);

@@ -492,6 +492,62 @@ def test_m2p2d_tiled(core, oracle, bc, N, prec, kernel, monkeypatch):
             assert _rel(out.cpu().numpy(), ref) < TOL[prec]
 
 
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+@pytest.mark.parametrize("prec", [F64, F32C])
+@pytest.mark.parametrize("spread", ["one_cell", "blob", "mixed"])
+def test_tiled_crowded_cells(core, oracle, kernel, prec, spread, monkeypatch):
+    """Cells holding tens to hundreds of particles, where the tile kernels
+    leave the round structure (the deposit's one-at-a-time path with its
+    particles loaded 64 at a time, the gather's one-particle-per-lane pass):
+    P2M on every node and M2P at every particle against the general path and
+    the oracle, and two launches bitwise equal.  one_cell: 300 particles in
+    one cell beside a sparse uniform background; blob: a Gaussian blob a few
+    cells wide; mixed: several crowded cells in one 16-cell row plus light
+    ones."""
+    N = 24
+    rng = np.random.default_rng({"one_cell": 1, "blob": 2, "mixed": 3}[spread] + 10 * kernel)
+    h = 1.0 / N
+    bg = rng.uniform(0.0, 1.0, size=(3000, 3))
+    if spread == "one_cell":
+        dense = (np.array([9, 10, 11]) + rng.uniform(0.05, 0.95, size=(300, 3))) * h
+    elif spread == "blob":
+        dense = np.mod(0.4 + 0.03 * rng.normal(size=(20000, 3)), 1.0)
+    else:
+        cells = np.array([[2 + 2 * k, 7, 13] for k in range(6)] + [[3, 8, 13], [5, 6, 12]])
+        counts = rng.integers(20, 140, size=len(cells))
+        dense = np.concatenate([(c + rng.uniform(0.02, 0.98, size=(n, 3))) * h for c, n in zip(cells, counts)])
+    xp = np.concatenate([bg, dense])[rng.permutation(len(bg) + len(dense))]
+    up = rng.normal(size=(len(xp), 3))
+    vdt = _val_dtype(prec)
+    mesh = core.Mesh(3, N + 1)
+    x, u = _dev(xp, torch.float64), _dev(up, vdt)
+    f1 = torch.empty(mesh.compact_shape(3), dtype=vdt, device="cuda")
+    core.p2m(mesh, x, u, f1, [0] * 3, kernel, prec)
+    f2 = torch.empty_like(f1)
+    core.p2m(mesh, x, u, f2, [0] * 3, kernel, prec)
+    assert torch.equal(f1, f2)
+    field = _dev(rng.normal(size=mesh.compact_shape(3)), vdt)
+    o1 = torch.empty((len(xp), 3), dtype=vdt, device="cuda")
+    core.m2p(mesh, x, field, [0] * 3, o1, kernel, prec)
+    o2 = torch.empty_like(o1)
+    core.m2p(mesh, x, field, [0] * 3, o2, kernel, prec)
+    assert torch.equal(o1, o2)
+    monkeypatch.setenv("PPMG_GENERAL", "1")
+    fg = torch.empty_like(f1)
+    core.p2m(mesh, x, u, fg, [0] * 3, kernel, prec)
+    og = torch.empty_like(o1)
+    core.m2p(mesh, x, field, [0] * 3, og, kernel, prec)
+    monkeypatch.delenv("PPMG_GENERAL")
+    assert _rel(f1.cpu().numpy(), fg.cpu().numpy()) < GEN_TOL[prec]
+    assert _rel(o1.cpu().numpy(), og.cpu().numpy()) < GEN_TOL[prec]
+    t = oracle.topo(3, [N + 1] * 3)
+    info, ref = oracle.p2m(t, kernel, xp, up, [1, 1, 1])
+    assert info == 0
+    assert _rel(f1.double().cpu().numpy(), oracle.unique_nodes(ref, [1, 1, 1], 3)) < TOL[prec]
+    qref = bf.m2p_periodic(xp, field.double().cpu().numpy(), N, kernel)
+    assert _rel(o1.double().cpu().numpy(), qref) < TOL[prec]
+
+
 @pytest.mark.parametrize("bc", [[1] * 6, [2] * 6, [1, 1, 1, 1, 2, 2]], ids=["periodic", "all_open", "open_z"])
 @pytest.mark.parametrize("N", [2, 3, 4])
 def test_smallest_meshes(core, oracle, bc, N, monkeypatch):
