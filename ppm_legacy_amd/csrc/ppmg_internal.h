@@ -382,6 +382,7 @@ struct ppmg_plan {
     int64_t np = 0;
     int64_t ncells = 0;
     bool sorted_input = false;   // last binning found the input already in cell order
+    int64_t crowded_cells = 0;   // estimate (1-in-64 sample) of the last binning's cells holding nine or more particles
     bool perm_identity = false;  // the permutation of the last binning is the identity ...
     bool perm_written = true;    // ... and has (or has not yet) been written into `perm`
     hipStream_t bin_stream = nullptr;   // stream of the last binning

@@ -422,7 +422,13 @@ static void launch_p3_shape(ppmg_plan* plan, const DevMesh& m, const PosT* xp, c
         slots.store(resident_workgroups(k_p3_tile<KS, LL, PosT, ValT, TY, NW, OPEN, TAB>, 64 * NW));
     const int Rx = m.N[0] + !m.periodic[0], Ry = m.N[1] + !m.periodic[1], Rz = m.N[2] + !m.periodic[2];
     const int tx = (Rx + TT - 1) / TT, ty = (Ry + TY - 1) / TY;
-    const int tz = pick_z_chunk((int64_t)tx * ty, Rz, slots.load(), H);
+    // crowded cells (more than 1/1024 of the cells hold nine or more
+    // particles): shorter chunks keep the workgroups of crowded tiles short
+    // (clustered 512^3: 13.8 against 15.6 ms); the lattice keeps the chunk
+    // that costs the least halo (4.56 against 4.82 ms with 16 planes)
+    const int64_t ncells = (int64_t)m.nbin[0] * m.nbin[1] * m.nbin[2];
+    const int tz = plan->crowded_cells > ncells / 1024 ? pick_z_chunk_balanced((int64_t)tx * ty, Rz, slots.load(), 16)
+                                                       : pick_z_chunk((int64_t)tx * ty, Rz, slots.load(), H);
     dim3 grid((unsigned)((int64_t)tx * ty * ((Rz + tz - 1) / tz)));
     hipLaunchKernelGGL((k_p3_tile<KS, LL, PosT, ValT, TY, NW, OPEN, TAB>), grid, dim3(64 * NW), 0, s, m,
                        plan->start.as<int>(), perm, xp, up, fg, c0, tz, field, wt);
