@@ -386,6 +386,7 @@ bool adjoint3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg
 int adjoint3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
                    const FieldGeom& fg, const void* g, void* gu, void* gx, hipStream_t s) {
     if (np <= 0) return 0;
+    if (materialize_heads(plan, s)) return PPMG_ERROR_FATAL;
     if (prec == PPMG_PREC_F64)
         return adjoint3d_typed<double, double>(plan, m, np, (const double*)xp, lda, (const double*)up,
                                                (const double*)g, fg, (double*)gu, (double*)gx, s);

@@ -43,6 +43,7 @@ static bool injected_alloc_failure(size_t n) {
 
 static thread_local Arena* g_arena = nullptr;
 void bind_arena(Arena* a) { g_arena = a; }
+bool arena_bound() { return g_arena != nullptr; }
 
 int DevBuf::ensure(size_t n) {
     if (n <= bytes && p) return 0;
@@ -228,7 +229,8 @@ int ppmg_plan_create(ppmg_plan** plan) {
 int ppmg_plan_destroy(ppmg_plan* plan) {
     if (!plan) return 0;
     DevBuf* bufs[] = {&plan->key, &plan->slot, &plan->count, &plan->start, &plan->perm, &plan->wsorted,
-                      &plan->usorted, &plan->scan_tmp, &plan->err, &plan->misc, &plan->misc2, &plan->gaps, &plan->multi, &plan->multi_n, &plan->ghost};
+                      &plan->usorted, &plan->scan_tmp, &plan->err, &plan->misc, &plan->misc2, &plan->gaps, &plan->multi, &plan->multi_n, &plan->ghost,
+                      &plan->table};
     for (DevBuf* b : bufs) b->release();
     if (plan->ev_init)
         for (auto& e : plan->ev) (void)hipEventDestroy(e);
@@ -283,6 +285,12 @@ int ppmg_map_field_ghost(const ppmg_mesh* mesh, int prec, const ppmg_field* fiel
 int ppmg_plan_sort_view(const ppmg_plan* plan, const int** key, const int** perm, const int** start, int64_t* np,
                         int64_t* ncells) {
     if (!plan) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "plan is NULL");
+    if ((perm || start) && plan->bin_table) {
+        // the binning left a slot table: build the heads and the permutation
+        ppmg_plan* w = const_cast<ppmg_plan*>(plan);
+        if (materialize_heads(w, w->bin_stream)) return g_status.info;
+        PPMG_HIP(hipStreamSynchronize(w->bin_stream));
+    }
     if (perm && plan->perm_identity && !plan->perm_written) {
         // the binning left the identity permutation implicit: write it now
         ppmg_plan* w = const_cast<ppmg_plan*>(plan);
@@ -323,15 +331,17 @@ int ppmg_p2m(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     hipStream_t s = as_stream(stream);
     int64_t nbad = 0;
     plan->have_forward = false;
+    // PPMG_GENERAL=1 forces the general node-gather path (used to cross-check the fast path)
+    const char* gen = std::getenv("PPMG_GENERAL");
+    const bool force_general = gen && gen[0] && gen[0] != '0';
     mark(plan, s, true);
-    if (bin_particles(plan, m, prec, np, xp, s, &nbad)) return g_status.info;
+    if (bin_particles(plan, m, prec, np, xp, s, &nbad,
+                      !force_general && p2m3d_table_applicable(m, kernel, prec, fg)))
+        return g_status.info;
     if (nbad > 0)
         return fail(PPMG_ERROR_FATAL, PPMG_ERR_PART_UNASS,
                     "MAJOR PROBLEM: " + std::to_string(nbad) + " particles lie in no sub");
     mark(plan, s);
-    // PPMG_GENERAL=1 forces the general node-gather path (used to cross-check the fast path)
-    const char* gen = std::getenv("PPMG_GENERAL");
-    const bool force_general = gen && gen[0] && gen[0] != '0';
     if (!force_general && p2m3d_fast_applicable(m, kernel, prec, fg)) {
         if (p2m3d_fast(plan, m, kernel, prec, np, xp, fg, up, field->data, nullptr, s)) return g_status.info;
     } else if (!force_general && p2m2d_fast_applicable(m, kernel, fg)) {

@@ -372,17 +372,24 @@ struct Arena {
     size_t used = 0;
 };
 void bind_arena(Arena* a);   // nullptr unbinds
+bool arena_bound();          // an arena is bound on the calling thread
 
 }  // namespace ppmg
 
 // Opaque plan: scratch of the binning and the forward record for the adjoint.
 struct ppmg_plan {
     ppmg::DevBuf key, slot, count, start, perm, wsorted, usorted, scan_tmp, err, misc, misc2, gaps, multi,
-        multi_n, ghost;
+        multi_n, ghost, table;
     int64_t np = 0;
     int64_t ncells = 0;
     bool sorted_input = false;   // last binning found the input already in cell order
     int64_t crowded_cells = 0;   // estimate (1-in-64 sample) of the last binning's cells holding nine or more particles
+    // The last binning left a slot table (ppmg::TABLE_SLOTS int32 planes of
+    // ncells: particle r of cell c at table[r * ncells + c], with the byte
+    // counts in `count`) instead of `start` and `perm`; materialize_heads()
+    // builds those two from it.
+    bool bin_table = false;
+    int table_skip = 0;          // binnings to run without the table after one overflowed it
     bool perm_identity = false;  // the permutation of the last binning is the identity ...
     bool perm_written = true;    // ... and has (or has not yet) been written into `perm`
     hipStream_t bin_stream = nullptr;   // stream of the last binning

@@ -21,6 +21,8 @@ int create_part_general(ppmg_plan* plan, const DevMesh& m, int prec, const Field
 // 3D tiled deposit (ppmg_p2m3d.hip), M'4 or BSP2: per axis (periodic or open)
 // the compact layout of a periodic axis or a stored box holding the real range 0..N.
 bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg);
+// ... and it reads a slot-table binning (bin_particles' `table`) there.
+bool p2m3d_table_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg);
 // wtab: NULL (weights from the positions), or, for M'4 only, the remesh weight
 // tables (four per particle and axis, ppm_rmsh_comp_weights' layout).
 int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp,

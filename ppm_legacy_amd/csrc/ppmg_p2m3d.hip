@@ -93,14 +93,27 @@ __device__ __forceinline__ void shift_in(const ValT (&wx)[KS], const ValT (&wz)[
 // TAB: the 1D weights come from the caller's tables (ppm_rmsh_remesh's
 // wx1/wx2/wx3, four per particle and axis) instead of the positions.
 //
+// TABLE: the binning is a slot table (byte counts `bt.count`, particle r of
+// cell c at bt.table[r * ncells + c]) instead of heads and a permutation: a
+// cell's count and its first particle load together, one dependent load less
+// before the particle's own.
+
+// A slot-table binning as the tiled kernels read it.
+struct BinTable {
+    const uint32_t* count;
+    const int* table;
+    int64_t ncells;
+};
+//
 // KS = 2 (BSP2) is the same march with a 1-cell halo: the window is 16 x
 // (TY+1) cells for 15 x TY nodes, lane ex finishes node ex-1 from lanes ex
 // and ex-1, cell plane cz adds to node planes cz and cz+1 (a ring of two).
 
-template <int KS, int LDA, typename PosT, typename ValT, int TY, int NW, bool OPEN, bool TAB>
+template <int KS, int LDA, typename PosT, typename ValT, int TY, int NW, bool OPEN, bool TAB, bool TABLE>
 __global__ void __launch_bounds__(64 * NW)
-k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
-          const ValT* __restrict__ up, FieldGeom fg, int c0, int tz, ValT* __restrict__ field, WTab<ValT> wt) {
+k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, BinTable bt,
+          const PosT* __restrict__ xp, const ValT* __restrict__ up, FieldGeom fg, int c0, int tz,
+          ValT* __restrict__ field, WTab<ValT> wt) {
     static_assert(KS == 4 || !TAB, "weight tables are M'4 only");
     constexpr int H = Stencil<KS>::H, B = Stencil<KS>::B;
     constexpr int TT = 16 - H;                     // owned nodes per tile along x
@@ -167,16 +180,34 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const int cz0 = z0 - (H - B), cz1 = z1 - 1 + B;
 
     // The cell heads of the step after next are loaded while the current
-    // (plane, chunk) step computes.
-    auto heads = [&](int cz, int j, int& f, int& n) {
+    // (plane, chunk) step computes.  (f, n) is the cell's range [f, n) of the
+    // permutation, or (TABLE) its key and its count word, with t its first
+    // particle (loaded even for an empty cell: the table is allocated there).
+    auto heads = [&](int cz, int j, int& f, int& n, int& t) {
         f = 0;
         n = 0;
+        t = 0;
         const int czw = cz <= cz1 ? wcell(cz, 2) : -1;
         if (czw >= 0 && (!OPEN || keyxy[j] >= 0)) {
             const int key = (int)(plane_cells * czw + keyxy[j]);
-            f = start[key];
-            n = start[key + 1];   // end, subtracted where used: the load stays in flight
+            if constexpr (TABLE) {
+                f = key;
+                n = (int)bt.count[count_word(key)];   // decoded where used: the load stays in flight
+                t = bt.table[key];
+            } else {
+                f = start[key];
+                n = start[key + 1];   // end, subtracted where used: the load stays in flight
+            }
         }
+    };
+    // particles in the cell of (f, n), and its particle r < that count
+    auto count_of = [&](int f, int n) {
+        if constexpr (TABLE) return (int)(((unsigned)n >> count_shift(f)) & 255u);
+        else return n - f;
+    };
+    auto entry = [&](int f, int r) {
+        if constexpr (TABLE) return bt.table[(int64_t)r * bt.ncells + f];
+        else return perm ? perm[f + r] : f + r;
     };
     // position (or weights) and strengths of particle p (an index into the
     // caller's arrays) when has, zeros otherwise
@@ -223,32 +254,37 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         }
     };
     // The permutation entry of a step's first particle is looked up one step
-    // ahead, and its position and strengths at the end of the step before.
-    auto lookup = [&](int f, int n) { return n > f ? (perm ? perm[f] : f) : 0; };
-    int f1, n1, f2, n2;
-    heads(cz0, 0, f1, n1);
-    if (NC > 1) heads(cz0, 1, f2, n2);
-    else heads(cz0 + 1, 0, f2, n2);
-    int pn = lookup(f1, n1);
+    // ahead (TABLE: it came with the heads), and its position and strengths at
+    // the end of the step before.
+    auto lookup = [&](int f, int n, int t) {
+        if constexpr (TABLE) return count_of(f, n) > 0 ? t : 0;
+        else return n > f ? (perm ? perm[f] : f) : 0;
+    };
+    int f1, n1, t1, f2, n2, t2;
+    heads(cz0, 0, f1, n1, t1);
+    if (NC > 1) heads(cz0, 1, f2, n2, t2);
+    else heads(cz0 + 1, 0, f2, n2, t2);
+    int pn = lookup(f1, n1, t1);
     // Round 0's particle of the next step is loaded at the end of a step, while
     // the wavefront waits at the phase barrier; it is not live during the rounds.
     PD xn[NPD];
     ValT un[LDA];
-    load(pn, n1 > f1, xn, un);
+    load(pn, count_of(f1, n1) > 0, xn, un);
 
     for (int cz = cz0; cz <= cz1; ++cz) {
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
-            const int first = f1, cnt = n1 - f1;   // n: end of the cell's range
+            const int first = f1, cnt = count_of(f1, n1);
             // take this step's heads (loaded two steps ago) before the next loads
             // are issued, so that waiting for them does not wait for those loads too
             asm volatile("" ::"v"(first), "v"(cnt) : "memory");
             f1 = f2;
             n1 = n2;
-            pn = lookup(f1, n1);
-            if (j + 2 < NC) heads(cz, j + 2, f2, n2);
-            else if (j + 2 - NC < NC) heads(cz + 1, j + 2 - NC, f2, n2);
-            else heads(cz + 2, 0, f2, n2);
+            t1 = t2;
+            pn = lookup(f1, n1, t1);
+            if (j + 2 < NC) heads(cz, j + 2, f2, n2, t2);
+            else if (j + 2 - NC < NC) heads(cz + 1, j + 2 - NC, f2, n2, t2);
+            else heads(cz + 2, 0, f2, n2, t2);
             const int ly0 = ey4[j] - H;              // node row of stencil offset oy = 0
             // one round: every lane deposits one particle of its cell (zero
             // strength where it has none)
@@ -308,7 +344,7 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                 if (__popcll(act) > P3_SERIAL) {
                     PD x[NPD];
                     ValT u[LDA];
-                    const int p = cnt > r ? (perm ? perm[first + r] : first + r) : 0;
+                    const int p = cnt > r ? entry(first, r) : 0;
                     load(p, cnt > r, x, u);
                     round(x, u);
                     continue;
@@ -328,7 +364,7 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                     for (int rr = r; rr < sc; ++rr) {
                         PD x[NPD];
                         ValT u[LDA], wyz, wxo;
-                        load(perm ? perm[sf + rr] : sf + rr, true, x, u);
+                        load(entry(sf, rr), true, x, u);
                         if constexpr (TAB) {
                             wyz = x[4 + oy] * x[8 + oz];
                             wxo = x[ox];
@@ -346,7 +382,7 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                 } while (act != 0ull);
                 break;
             }
-            load(pn, n1 > f1, xn, un);
+            load(pn, count_of(f1, n1) > 0, xn, un);
             // end of a phase: the other phase's chunks share node rows with this one's
             if (NW > 1 || j == NC - 1) __syncthreads();
         }
@@ -392,6 +428,8 @@ k_p3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     }
 }
 
+static int p3_rows();
+
 // Per axis the stored box is either the compact layout of a periodic axis
 // (the N unique nodes, lo = 0, ext = N) or holds the whole real range 0..N
 // (lo <= 0 <= N <= lo+ext-1: the reference's 1-g..Nm+g box is lo = -g,
@@ -409,17 +447,26 @@ bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGe
     return (int64_t)fg.ext[0] * fg.ext[1] * fg.lda * vsize < ((int64_t)1 << 31);
 }
 
+// The deposit reads a slot-table binning: the compact layout of a periodic
+// mesh (no stored-box shell to fill from the heads) with the default tile.
+bool p2m3d_table_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg) {
+    if (!p2m3d_fast_applicable(m, kernel, prec, fg)) return false;
+    for (int d = 0; d < 3; ++d)
+        if (!(m.periodic[d] && fg.lo[d] == 0 && fg.ext[d] == m.N[d])) return false;
+    return kernel == PPMG_KERNEL_BSP2 || p3_rows() == P3_TY;
+}
+
 // One launch of the deposit for components c0 .. c0+LL-1 with tiles of
 // (16 - H) x TY nodes and NW wavefronts per tile.  The z chunk is chosen from
 // the device's resident workgroup count (queried once per instantiation):
 // each chunk marches H extra cell planes.
-template <int KS, int LL, typename PosT, typename ValT, int TY, int NW, bool OPEN, bool TAB>
+template <int KS, int LL, typename PosT, typename ValT, int TY, int NW, bool OPEN, bool TAB, bool TABLE = false>
 static void launch_p3_shape(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const int* perm,
                             const FieldGeom& fg, int c0, ValT* field, const WTab<ValT>& wt, hipStream_t s) {
     constexpr int H = Stencil<KS>::H, TT = 16 - H;
     static std::atomic<int> slots{0};
     if (slots.load() == 0)
-        slots.store(resident_workgroups(k_p3_tile<KS, LL, PosT, ValT, TY, NW, OPEN, TAB>, 64 * NW));
+        slots.store(resident_workgroups(k_p3_tile<KS, LL, PosT, ValT, TY, NW, OPEN, TAB, TABLE>, 64 * NW));
     const int Rx = m.N[0] + !m.periodic[0], Ry = m.N[1] + !m.periodic[1], Rz = m.N[2] + !m.periodic[2];
     const int tx = (Rx + TT - 1) / TT, ty = (Ry + TY - 1) / TY;
     // crowded cells (more than 1/1024 of the cells hold nine or more
@@ -430,8 +477,9 @@ static void launch_p3_shape(ppmg_plan* plan, const DevMesh& m, const PosT* xp, c
     const int tz = plan->crowded_cells > ncells / 1024 ? pick_z_chunk_balanced((int64_t)tx * ty, Rz, slots.load(), 16)
                                                        : pick_z_chunk((int64_t)tx * ty, Rz, slots.load(), H);
     dim3 grid((unsigned)((int64_t)tx * ty * ((Rz + tz - 1) / tz)));
-    hipLaunchKernelGGL((k_p3_tile<KS, LL, PosT, ValT, TY, NW, OPEN, TAB>), grid, dim3(64 * NW), 0, s, m,
-                       plan->start.as<int>(), perm, xp, up, fg, c0, tz, field, wt);
+    const BinTable bt{plan->count.as<uint32_t>(), plan->table.as<int>(), plan->ncells};
+    hipLaunchKernelGGL((k_p3_tile<KS, LL, PosT, ValT, TY, NW, OPEN, TAB, TABLE>), grid, dim3(64 * NW), 0, s, m,
+                       plan->start.as<int>(), perm, bt, xp, up, fg, c0, tz, field, wt);
 }
 
 // Tile height: PPMG_P2M_ROWS (9, 13, 21 or 29 node rows: 1, 2, 3 or 4
@@ -470,6 +518,14 @@ static void launch_p3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, co
 template <int KS, int LL, typename PosT, typename ValT, bool TAB>
 static void launch_p3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const int* perm,
                       const FieldGeom& fg, int c0, ValT* field, const WTab<ValT>& wt, hipStream_t s) {
+    if constexpr (!TAB) {
+        // a slot table (p2m3d_table_applicable: periodic, default tile)
+        if (plan->bin_table) {
+            launch_p3_shape<KS, LL, PosT, ValT, KS == 2 ? 23 : P3_TY, 3, false, false, true>(plan, m, xp, up, perm, fg,
+                                                                                            c0, field, wt, s);
+            return;
+        }
+    }
     if (m.periodic[0] && m.periodic[1] && m.periodic[2])
         launch_p3_open<KS, LL, PosT, ValT, false, TAB>(plan, m, xp, up, perm, fg, c0, field, wt, s);
     else
@@ -485,6 +541,10 @@ static int p2m3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT
     if (wtab)
         for (int d = 0; d < 3; ++d) wt.w[d] = wtab[d];
     (void)np;
+    // the slot table is read on a periodic mesh with the default tile, without
+    // weight tables; elsewhere the heads and permutation are built from it
+    if (plan->bin_table && (wtab || !(m.periodic[0] && m.periodic[1] && m.periodic[2]) || (KS == 4 && p3_rows() != P3_TY)))
+        if (materialize_heads(plan, s)) return 1;
     const int* pk = tile_perm(plan);
     for (int c0 = 0; c0 < lda; c0 += 4) {
         int L = lda - c0 < 4 ? lda - c0 : 4;

@@ -14,17 +14,37 @@ struct RankGeom {
     double rdx[3];    // Nm / (xmax - xmin) in the position precision
 };
 
+// Byte-wide cell counts of the binning.  In every block of 256 cells, cell c
+// sits in word c mod 64, byte (c / 64) mod 4: the 64 consecutive cells a
+// wavefront of lattice-ordered particles hits fall in 64 different words, so
+// the atomic counts of neighbouring lanes do not contend for one word.  The
+// array is padded to whole blocks and zeroed, so reads need no bounds.
+__host__ __device__ inline int64_t count_word(int64_t c) { return ((c >> 8) << 6) + (c & 63); }
+__host__ __device__ inline unsigned count_shift(int64_t c) { return (unsigned)((c >> 6) & 3) * 8u; }
+
+// Slot table of a binning (plan->bin_table): particle r (0 <= r < count) of
+// cell c, in increasing particle index, is table[r * ncells + c].  A cell
+// with more than TABLE_SLOTS particles sends the binning to the heads and
+// permutation instead.  Eight slots hold every cell of a lattice jittered by
+// less than half a cell and displaced by a locally uniform shift: a cell then
+// meets at most two particles per axis.
+constexpr int TABLE_SLOTS = 8;
+
 int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, int64_t total, hipStream_t s);
 int fill_zero(void* p, size_t bytes, hipStream_t s);
 // Bins np particles into the plan (key, perm, start); *nbad = particles outside the sub.
+// table: the caller reads a slot table (plan->bin_table) where the binning
+// can leave one instead of the heads and the permutation.
 int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp,
-                  hipStream_t s, int64_t* nbad);
+                  hipStream_t s, int64_t* nbad, bool table = false);
+// Heads and permutation of a binning that left a slot table (no-op otherwise).
+int materialize_heads(ppmg_plan* plan, hipStream_t s);
 // The binning leaves the identity permutation implicit when the input is
 // already in cell order.  Tiled kernels take tile_perm(): nullptr then, and
 // they read particle f at sorted position f; other readers of plan->perm call
 // materialize_perm() first.
 inline const int* tile_perm(const ppmg_plan* plan) { return plan->perm_identity ? nullptr : plan->perm.as<int>(); }
-int materialize_perm(ppmg_plan* plan, hipStream_t s);
+int materialize_perm(ppmg_plan* plan, hipStream_t s);   // calls materialize_heads() first
 // Input binned in cell order leaves the keys unwritten too; readers of
 // plan->key call materialize_keys() first (it reads the positions of that
 // binning, which must still hold them).

@@ -6,6 +6,9 @@
 // scatter use integer atomics, which place the particles of one cell in an
 // arbitrary order; a per-cell pass then restores increasing particle index,
 // so the permutation equals the reference's lpdx exactly.
+#include <climits>
+#include <cstdlib>
+
 #include "ppmg_internal.h"
 #include "ppmg_sort.h"
 
@@ -13,7 +16,8 @@ namespace ppmg {
 
 // Binning status words: err[0] particles outside the sub, err[1] a descent
 // seen, err[2] queued runs of empty cells, err[3] a byte count overflowed,
-// err[8..71] sampled crowded cells.
+// err[4] a cell with more particles than the slot table holds, err[8..71]
+// sampled crowded cells.
 constexpr int ERR_WORDS = 72;
 
 constexpr int SCAN_BLOCK = 256;
@@ -178,14 +182,6 @@ int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, int64_t tota
     return 0;
 }
 
-// Byte-wide cell counts of the binning.  In every block of 256 cells, cell c
-// sits in word c mod 64, byte (c / 64) mod 4: the 64 consecutive cells a
-// wavefront of lattice-ordered particles hits fall in 64 different words, so
-// the atomic counts of neighbouring lanes do not contend for one word.  The
-// array is padded to whole blocks and zeroed, so reads need no bounds.
-__device__ __forceinline__ int64_t count_word(int64_t c) { return ((c >> 8) << 6) + (c & 63); }
-__device__ __forceinline__ unsigned count_shift(int64_t c) { return (unsigned)((c >> 6) & 3) * 8u; }
-
 // Counts of the 16 cells base .. base+15 (base a multiple of 16): 16
 // consecutive words of one block, one byte of each.
 __device__ __forceinline__ void load_counts16(const uint32_t* __restrict__ words, int64_t base, int (&c)[16]) {
@@ -314,18 +310,23 @@ __device__ __forceinline__ void set_heads(int* __restrict__ start, int lo, int h
 // wavefronts of unsorted input do not queue on one word.
 //   SLOT (unsorted input expected): write the keys and take each particle's
 //     arrival slot in its cell by an atomic count (byte-wide: slots 0..254).
+//   SLOT and TABLE: no keys or slots written; particle p, arriving r-th in
+//     cell k, is written to table[r * ncells + k] (r < TABLE_SLOTS; a later
+//     arrival sets err[4]).
 //   !SLOT (input expected in cell order): the first particle of each run of
 //     equal keys writes its index as the head of its cell and of the empty
 //     cells before it (back to its predecessor's cell); the last particle
 //     closes the array with np.  No counts, no scan, no keys written, and the
 //     permutation is the identity.  Valid only if no descent is found (else
 //     the caller bins again on the SLOT path).
-template <int DIM, typename PosT, bool SLOT>
+template <int DIM, typename PosT, bool SLOT, bool TABLE>
 __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64_t np, DevMesh m, int64_t ncells,
                                               int* __restrict__ key, uint8_t* __restrict__ slot,
                                               uint32_t* __restrict__ count, int* __restrict__ start,
                                               int* __restrict__ err, int4* __restrict__ gaps, int gapcap,
-                                              int* __restrict__ multi, int* __restrict__ multi_n) {
+                                              int* __restrict__ multi, int* __restrict__ multi_n,
+                                              int* __restrict__ table) {
+    static_assert(SLOT || !TABLE, "the slot table needs the arrival slots");
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool live = p < np;
@@ -360,18 +361,23 @@ __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64
         if (threadIdx.x == 0) nlisted = ncrowd = 0;
         int sl = -1;
         if (live) {
-            key[p] = k;
+            if (!TABLE) key[p] = k;
             if (k >= 0) {
                 // byte-wide counts (count_word / count_shift)
                 const unsigned sh = count_shift(k);
                 sl = (int)((atomicAdd(&count[count_word(k)], 1u << sh) >> sh) & 255u);
-                slot[p] = (uint8_t)sl;
+                if (!TABLE) slot[p] = (uint8_t)sl;
+                else if (sl < TABLE_SLOTS) table[(int64_t)sl * ncells + k] = (int)p;
             }
         }
         // a cell's 256th arrival carries into the next cell's byte: err[3]
         // sends the caller to the full-width count
         const uint64_t ov = __ballot(sl == 255);
         if (ov && lane == 0) err[3] = 1;
+        if (TABLE) {
+            const uint64_t full = __ballot(sl >= TABLE_SLOTS);
+            if (full && lane == 0 && *(volatile int*)&err[4] == 0) err[4] = 1;
+        }
         __syncthreads();
         if (sl == 1) listed[atomicAdd(&nlisted, 1)] = k;
         if ((blockIdx.x & 63) == 0) {
@@ -541,6 +547,52 @@ __global__ void __launch_bounds__(64) k_stabilize_listed(const int* __restrict__
     }
 }
 
+// The same for the listed cells of a slot table: the (at most TABLE_SLOTS)
+// entries of a cell are sorted in registers.
+__global__ void __launch_bounds__(64) k_stabilize_table(const uint32_t* __restrict__ count,
+                                                        const int* __restrict__ multi,
+                                                        const int* __restrict__ multi_n, int64_t nlists,
+                                                        int64_t ncells, int* __restrict__ table) {
+    const int64_t l0 = (int64_t)blockIdx.x * STAB_GROUP;
+    int off[STAB_GROUP + 1];
+    off[0] = 0;
+#pragma unroll
+    for (int j = 0; j < STAB_GROUP; ++j) off[j + 1] = off[j] + (l0 + j < nlists ? multi_n[l0 + j] : 0);
+    for (int q = (int)threadIdx.x; q < off[STAB_GROUP]; q += 64) {
+        int j = 0;
+        while (q >= off[j + 1]) ++j;
+        const int c = multi[(l0 + j) * 256 + (q - off[j])];
+        const int n = (int)((count[count_word(c)] >> count_shift(c)) & 255u);
+        int v[TABLE_SLOTS];
+#pragma unroll
+        for (int r = 0; r < TABLE_SLOTS; ++r) v[r] = r < n ? table[(int64_t)r * ncells + c] : INT_MAX;
+        // odd-even transposition sort (the padding INT_MAX stays at the end)
+#pragma unroll
+        for (int pass = 0; pass < TABLE_SLOTS; ++pass)
+#pragma unroll
+            for (int r = pass & 1; r + 1 < TABLE_SLOTS; r += 2) {
+                const int a = min(v[r], v[r + 1]), b = max(v[r], v[r + 1]);
+                v[r] = a;
+                v[r + 1] = b;
+            }
+#pragma unroll
+        for (int r = 0; r < TABLE_SLOTS; ++r)
+            if (r < n) table[(int64_t)r * ncells + c] = v[r];
+    }
+}
+
+// Permutation of a slot-table binning from its heads: the particles of cell c
+// go to perm[start[c] ..].
+__global__ void __launch_bounds__(256) k_table_perm(const uint32_t* __restrict__ count, const int* __restrict__ start,
+                                                    const int* __restrict__ table, int64_t ncells,
+                                                    int* __restrict__ perm) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const int n = (int)((count[count_word(c)] >> count_shift(c)) & 255u);
+    const int b = start[c];
+    for (int r = 0; r < n; ++r) perm[b + r] = table[(int64_t)r * ncells + c];
+}
+
 __global__ void k_to_one_based(int* __restrict__ a, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] += 1;
@@ -578,7 +630,7 @@ __global__ void __launch_bounds__(256) k_keys_only(const PosT* __restrict__ xp, 
 }
 
 int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp,
-                  hipStream_t s, int64_t* nbad) {
+                  hipStream_t s, int64_t* nbad, bool table) {
     const int64_t ncells = (int64_t)m.nbin[0] * m.nbin[1] * m.nbin[2];
     const int64_t gapcap = np / 16 + 4096;
     const size_t npb = (size_t)(np > 0 ? np : 1);
@@ -595,6 +647,7 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
         return PPMG_ERROR_FATAL;
     plan->bin_stream = s;
     plan->prebinned = false;   // a binning left by ppmg_m2p_prebin is overwritten
+    plan->bin_table = false;
     plan->np = np;
     plan->ncells = ncells;
     plan->bin_xp = xp;
@@ -613,6 +666,17 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     // fresh from create_part) predicts right and does each step once.
     const bool predict_sorted = plan->sorted_input;
     if (!predict_sorted && fill_zero(plan->count.p, count_alloc, s)) return PPMG_ERROR_FATAL;
+    // Unsorted input for a reader of the slot table: the key pass fills the
+    // table, and no keys, scan or scatter follow.  A plan whose table
+    // overflowed bins its next 16 inputs without one.  Allocation-free calls
+    // take the heads (the table needs 32 bytes per cell), and so does every
+    // call under PPMG_NO_TABLE=1 (used to cross-check the two layouts).
+    const char* no_table = std::getenv("PPMG_NO_TABLE");
+    if (no_table && no_table[0] && no_table[0] != '0') table = false;
+    const bool use_table = table && !predict_sorted && plan->table_skip == 0 && !arena_bound();
+    if (table && !predict_sorted && plan->table_skip > 0) --plan->table_skip;
+    if (use_table && plan->table.ensure(sizeof(int) * TABLE_SLOTS * (size_t)ncells)) return PPMG_ERROR_FATAL;
+    int* tab = use_table ? plan->table.as<int>() : nullptr;
     int* key = plan->key.as<int>();
     uint8_t* slot8 = plan->slot.as<uint8_t>();
     uint32_t* count8 = plan->count.as<uint32_t>();
@@ -623,16 +687,23 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     int* multi_n = plan->multi_n.as<int>();
     {
         const unsigned g = nblk(np, 256);
-#define KEYS(D, T, S) hipLaunchKernelGGL((k_keys<D, T, S>), dim3(g), dim3(256), 0, s, (const T*)xp, np, m, ncells, key, \
-                                         slot8, count8, start, err, gaps, (int)gapcap, multi, multi_n)
+#define KEYS(D, T, S, TB) hipLaunchKernelGGL((k_keys<D, T, S, TB>), dim3(g), dim3(256), 0, s, (const T*)xp, np, m, \
+                                             ncells, key, slot8, count8, start, err, gaps, (int)gapcap, multi, multi_n, tab)
+#define KEYS_T(D, T)                                   \
+    do {                                               \
+        if (predict_sorted) KEYS(D, T, false, false);  \
+        else if (use_table) KEYS(D, T, true, true);    \
+        else KEYS(D, T, true, false);                  \
+    } while (0)
         const bool single = prec == PPMG_PREC_F32;
         if (m.dim == 3) {
-            if (single) { if (predict_sorted) KEYS(3, float, false); else KEYS(3, float, true); }
-            else { if (predict_sorted) KEYS(3, double, false); else KEYS(3, double, true); }
+            if (single) KEYS_T(3, float);
+            else KEYS_T(3, double);
         } else {
-            if (single) { if (predict_sorted) KEYS(2, float, false); else KEYS(2, float, true); }
-            else { if (predict_sorted) KEYS(2, double, false); else KEYS(2, double, true); }
+            if (single) KEYS_T(2, float);
+            else KEYS_T(2, double);
         }
+#undef KEYS_T
 #undef KEYS
         PPMG_HIP(hipGetLastError());
     }
@@ -661,7 +732,25 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
         // wrong prediction (or more long gaps than the queue holds): bin
         // again on the counting path
         plan->sorted_input = false;
-        return bin_particles(plan, m, prec, np, xp, s, nbad);
+        return bin_particles(plan, m, prec, np, xp, s, nbad, table);
+    }
+    if (use_table) {
+        if (herr[4]) {
+            // a cell holds more than TABLE_SLOTS particles: bin again with
+            // keys and slots
+            plan->table_skip = 16;
+            return bin_particles(plan, m, prec, np, xp, s, nbad, false);
+        }
+        // the listed cells (two or more particles) in increasing particle index
+        hipLaunchKernelGGL(k_stabilize_table, dim3(nblk(nblk(np, 256), STAB_GROUP)), dim3(64), 0, s, count8, multi,
+                           multi_n, (int64_t)nblk(np, 256), ncells, tab);
+        PPMG_HIP(hipGetLastError());
+        plan->bin_table = true;
+        plan->perm_identity = false;
+        plan->perm_written = false;
+        plan->keys_written = false;
+        plan->sorted_input = herr[1] == 0;
+        return 0;
     }
     if (!herr[3]) {
         if (exclusive_scan_u8(count8, ncells, start, plan->scan_tmp, np, s)) return PPMG_ERROR_FATAL;
@@ -717,7 +806,23 @@ int materialize_keys(ppmg_plan* plan, hipStream_t s) {
     return 0;
 }
 
+int materialize_heads(ppmg_plan* plan, hipStream_t s) {
+    if (!plan->bin_table) return 0;
+    const int64_t ncells = plan->ncells, np = plan->np;
+    int* start = plan->start.as<int>();
+    const uint32_t* count8 = plan->count.as<uint32_t>();
+    if (exclusive_scan_u8(count8, ncells, start, plan->scan_tmp, np, s)) return PPMG_ERROR_FATAL;
+    hipLaunchKernelGGL(k_table_perm, dim3(nblk(ncells, 256)), dim3(256), 0, s, count8, start,
+                       plan->table.as<int>(), ncells, plan->perm.as<int>());
+    PPMG_HIP(hipGetLastError());
+    plan->bin_table = false;
+    plan->perm_identity = false;
+    plan->perm_written = true;
+    return 0;
+}
+
 int materialize_perm(ppmg_plan* plan, hipStream_t s) {
+    if (materialize_heads(plan, s)) return PPMG_ERROR_FATAL;
     if (plan->perm_identity && !plan->perm_written && plan->np > 0) {
         hipLaunchKernelGGL(k_identity, dim3(nblk(plan->np, 256)), dim3(256), 0, s, plan->np, plan->perm.as<int>());
         PPMG_HIP(hipGetLastError());
@@ -764,7 +869,10 @@ int sort_rows(int dim, int prec, int64_t np, void* xp, const int* lpdx1, const i
 int rank_particles(ppmg_plan* plan, const RankGeom& g, int prec, int64_t np, const void* xp,
                    int* lpdx, int* lhbx, int64_t* icount, hipStream_t s) {
     int64_t ncells = (int64_t)g.ntot[0] * g.ntot[1] * g.ntot[2];
-    plan->prebinned = false;   // the plan's scratch is reused below
+    // the plan's scratch is reused below: a slot table's counts are
+    // overwritten, so its heads are built first
+    if (materialize_heads(plan, s)) return PPMG_ERROR_FATAL;
+    plan->prebinned = false;
     if (plan->key.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
         plan->slot.ensure(sizeof(int) * (size_t)(np > 0 ? np : 1)) ||
         plan->count.ensure(sizeof(int) * (size_t)ncells) || plan->err.ensure(sizeof(int) * 4))
@@ -804,6 +912,8 @@ int rank_particles(ppmg_plan* plan, const RankGeom& g, int prec, int64_t np, con
     }
     hipLaunchKernelGGL(k_to_one_based, dim3(nblk(ncells + 1, 256)), dim3(256), 0, s, lhbx, ncells + 1);
     PPMG_HIP(hipGetLastError());
+    // the binning's keys were overwritten: recomputed from its positions when read
+    plan->keys_written = false;
     *icount = inside;
     return 0;
 }

@@ -169,6 +169,61 @@ def test_binning_direct_heads(core, oracle, case):
         assert _rel(out.cpu().numpy(), oref) < 1e-12, (case, call)
 
 
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+@pytest.mark.parametrize("prec", [F64, F32C])
+def test_binning_slot_table(core, oracle, kernel, prec, monkeypatch):
+    """Unsorted input for the tiled 3D deposit on the compact layout is binned
+    into a slot table (up to eight particles per cell, in increasing index)
+    instead of heads and a permutation.  One plan is fed, in shuffled order:
+    an advected lattice (table); the same with 40 particles moved into one
+    cell (the table overflows, the binning runs again with keys and slots, and
+    the plan bins its next 16 inputs without a table); the lattice for those
+    16 calls; the lattice again (table).  Every field must equal bitwise the
+    field of the same particles binned with heads (PPMG_NO_TABLE=1), whose
+    per-node addition order is the same, and the oracle's within tolerance;
+    the adjoint (heads built from the table) must equal the heads binning's
+    bitwise; sort_view must give numpy's stable argsort, keys and heads."""
+    N, lda = 24, 3
+    rng = np.random.default_rng(31 + kernel)
+    lattice = wl.advect(wl.jittered_lattice(N, seed=31), wl.fourier_modes(31), N)[0]
+    crowded = lattice.copy()
+    crowded[:40] = (np.array([5, 6, 7]) + rng.uniform(0.1, 0.9, size=(40, 3))) / N
+    t = oracle.topo(3, [N + 1] * 3)
+    mesh = core.Mesh(3, N + 1)
+    plan = core.Plan()
+    calls = [("lattice", True)] + [("crowded", True)] + [("lattice", False)] * 16 + [("lattice", True)]
+    for i, (case, check) in enumerate(calls):
+        base = lattice if case == "lattice" else crowded
+        o = rng.permutation(len(base))
+        xp = base[o]
+        up = wl.strengths(i, len(xp), lda)
+        x, u = _dev(xp, _pos_dtype(prec)), _dev(up, _val_dtype(prec))
+        f = torch.empty(mesh.compact_shape(lda), dtype=_val_dtype(prec), device="cuda")
+        core.p2m(mesh, x, u, f, [0] * 3, kernel, prec, plan)
+        monkeypatch.setenv("PPMG_NO_TABLE", "1")
+        href = torch.empty_like(f)
+        hplan = core.Plan()
+        core.p2m(mesh, x, u, href, [0] * 3, kernel, prec, hplan)
+        monkeypatch.delenv("PPMG_NO_TABLE")
+        assert torch.equal(f, href), (case, i)
+        if not check:
+            continue
+        info, ref = oracle.p2m(t, kernel, xp, up, [1, 1, 1])
+        assert info == 0
+        assert _rel(f.cpu().numpy(), oracle.unique_nodes(ref, [1, 1, 1], 3)) < TOL[prec], (case, i)
+        g = _dev(rng.normal(size=f.shape), _val_dtype(prec))
+        gu, gx = core.p2m_adjoint(mesh, x, u, g, [0] * 3, plan, kernel, prec)
+        hu, hx = core.p2m_adjoint(mesh, x, u, g, [0] * 3, hplan, kernel, prec)
+        assert torch.equal(gu, hu) and torch.equal(gx, hx), (case, i)
+        cells = (xp * N).astype(np.int64)
+        key = cells[:, 0] + N * (cells[:, 1] + N * cells[:, 2])
+        k, perm, start = plan.sort_view()
+        assert np.array_equal(k.cpu().numpy(), key), (case, i)
+        assert np.array_equal(perm.cpu().numpy(), np.argsort(key, kind="stable")), (case, i)
+        assert np.array_equal(start.cpu().numpy(),
+                              np.concatenate([[0], np.cumsum(np.bincount(key, minlength=N ** 3))])), (case, i)
+
+
 # ----------------------------------------------------------- variants --
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
