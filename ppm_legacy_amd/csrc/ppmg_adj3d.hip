@@ -79,11 +79,13 @@ __device__ __forceinline__ double box_cotangent(const DevMesh& m, const FieldGeo
 // BOXED: the cotangent is on a stored box (some axis holds 0..N with its
 // ghosts, the reference's field_up(1-g:Nm+g)); the ring fetch reads it
 // through box_cotangent.  Otherwise it is the compact layout.
-template <int LDA, typename PosT, typename ValT, typename AccT, bool BOXED>
+// TABLE: the forward's binning is a slot table (byte counts, particle r of
+// cell c at bt.table[r * ncells + c]) instead of heads and a permutation.
+template <int LDA, typename PosT, typename ValT, typename AccT, bool BOXED, bool TABLE>
 __global__ void __launch_bounds__(A3_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
-k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
-          const ValT* __restrict__ up, const ValT* __restrict__ g, FieldGeom fg, ValT* __restrict__ grad_u,
-          ValT* __restrict__ grad_x, int lda, int c0, int tz, int accumulate_x) {
+k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, BinTable bt,
+          const PosT* __restrict__ xp, const ValT* __restrict__ up, const ValT* __restrict__ g, FieldGeom fg,
+          ValT* __restrict__ grad_u, ValT* __restrict__ grad_x, int lda, int c0, int tz, int accumulate_x) {
     __shared__ ValT ring[4 * LDA * A3_PL];   // [plane][c][ry][nx]
     const int tid = threadIdx.x;
     const int ex = tid & 15, row = tid >> 4;
@@ -165,16 +167,37 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     // entry of a plane's first particle one plane ahead, and that particle's
     // position and strengths are loaded at the end of the plane before (while
     // the workgroup refills the ring; they are not live during the rounds).
-    auto heads = [&](int z, int& f, int& n) {
+    // (f, n) is the cell's range [f, n) of the permutation, or (TABLE) its
+    // key and count word, with t its first particle.
+    auto heads = [&](int z, int& f, int& n, int& t) {
         f = 0;
         n = 0;
+        t = 0;
         if (plane_lane && z < z1) {
             const int key = (int)((int64_t)z * plane) + key0;
-            f = start[key];
-            n = start[key + 1];   // end, subtracted where used: the load stays in flight
+            if constexpr (TABLE) {
+                f = key;
+                n = (int)bt.count[count_word(key)];   // decoded where used: the load stays in flight
+                t = bt.table[key];
+            } else {
+                f = start[key];
+                n = start[key + 1];   // end, subtracted where used: the load stays in flight
+            }
         }
     };
-    auto lookup = [&](int f, int n) { return n > 0 ? (perm ? perm[f] : f) : 0; };
+    auto count_of = [&](int f, int n) {
+        if constexpr (TABLE) return (int)(((unsigned)n >> count_shift(f)) & 255u);
+        else return n - f;
+    };
+    // the cell's first particle, and its particle r < count
+    auto lookup = [&](int f, int n, int t) {
+        if constexpr (TABLE) return count_of(f, n) > 0 ? t : 0;
+        else return n > f ? (perm ? perm[f] : f) : 0;
+    };
+    auto entry = [&](int f, int r) {
+        if constexpr (TABLE) return bt.table[(int64_t)r * bt.ncells + f];
+        else return perm ? perm[f + r] : f + r;
+    };
     auto particle = [&](int p, int n, PosT (&x)[3], ValT (&u)[LDA]) {
         x[0] = x[1] = x[2] = PosT(0);
 #pragma unroll
@@ -187,24 +210,25 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             for (int c = 0; c < LDA; ++c) u[c] = up[(int64_t)p * lda + c0 + c];
         }
     };
-    int first_n, cnt_n, first_nn, cnt_nn;
-    heads(z0, first_n, cnt_n);
-    heads(z0 + 1, first_nn, cnt_nn);
-    int pn = lookup(first_n, cnt_n - first_n);
+    int first_n, cnt_n, t_n, first_nn, cnt_nn, t_nn;
+    heads(z0, first_n, cnt_n, t_n);
+    heads(z0 + 1, first_nn, cnt_nn, t_nn);
+    int pn = lookup(first_n, cnt_n, t_n);
     PosT xn[3];
     ValT un[LDA];
-    particle(pn, cnt_n - first_n, xn, un);
+    particle(pn, count_of(first_n, cnt_n), xn, un);
     __syncthreads();
 
     for (int cz = z0; cz < z1; ++cz) {
-        const int first = first_n, cnt = cnt_n - first_n;   // cnt_n: end of the range
+        const int first = first_n, cnt = count_of(first_n, cnt_n);
         // take this plane's heads before the next loads are issued, so that
         // waiting for them does not wait for those loads too
         asm volatile("" ::"v"(first), "v"(cnt) : "memory");
         first_n = first_nn;
         cnt_n = cnt_nn;
-        const int pnext = lookup(first_n, cnt_n - first_n);   // next plane's first particle
-        heads(cz + 2, first_nn, cnt_nn);
+        t_n = t_nn;
+        const int pnext = lookup(first_n, cnt_n, t_n);   // next plane's first particle
+        heads(cz + 2, first_nn, cnt_nn, t_nn);
         const bool more = cz + 1 < z1;
         if (!BOXED && more) fetch_plane(cz + 3);
         auto round = [&](int p, bool has, const PosT (&x)[3], const ValT (&uv)[LDA]) {
@@ -305,12 +329,12 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         for (int r = 1; __ballot(cnt > r) != 0ull; ++r) {
             PosT x[3];
             ValT u[LDA];
-            const int p = lookup(first + r, cnt - r);
+            const int p = cnt > r ? entry(first, r) : 0;
             particle(p, cnt - r, x, u);
             round(p, cnt > r, x, u);
         }
         pn = pnext;
-        particle(pn, cnt_n - first_n, xn, un);   // next plane's round 0
+        particle(pn, count_of(first_n, cnt_n), xn, un);   // next plane's round 0
         __syncthreads();   // every row is done with plane cz-1: its slot takes plane cz+3
         if (more) {
             if constexpr (BOXED) fetch_store_boxed(cz + 3);
@@ -320,16 +344,18 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     }
 }
 
-template <int LL, typename PosT, typename ValT, typename AccT, bool BOXED>
+template <int LL, typename PosT, typename ValT, typename AccT, bool BOXED, bool TABLE = false>
 static void launch_a3_form(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const ValT* g,
                            const FieldGeom& fg, ValT* gu, ValT* gx, int lda, int c0, int acc, hipStream_t s) {
     static std::atomic<int> slots{0};
-    if (slots.load() == 0) slots.store(resident_workgroups(k_a3_tile<LL, PosT, ValT, AccT, BOXED>, A3_THREADS));
+    if (slots.load() == 0)
+        slots.store(resident_workgroups(k_a3_tile<LL, PosT, ValT, AccT, BOXED, TABLE>, A3_THREADS));
     const int tx = (m.N[0] + A3_TX - 1) / A3_TX, ty = (m.N[1] + A3_TY - 1) / A3_TY;
     const int tz = pick_z_chunk_balanced((int64_t)tx * ty, m.N[2], slots.load(), 32);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.N[2] + tz - 1) / tz)));
-    hipLaunchKernelGGL((k_a3_tile<LL, PosT, ValT, AccT, BOXED>), grid, dim3(A3_THREADS), 0, s, m,
-                       plan->start.as<int>(), tile_perm(plan), xp, up, g, fg, gu, gx, lda, c0, tz, acc);
+    const BinTable bt{plan->count.as<uint32_t>(), plan->table.as<int>(), plan->ncells};
+    hipLaunchKernelGGL((k_a3_tile<LL, PosT, ValT, AccT, BOXED, TABLE>), grid, dim3(A3_THREADS), 0, s, m,
+                       plan->start.as<int>(), tile_perm(plan), bt, xp, up, g, fg, gu, gx, lda, c0, tz, acc);
 }
 
 // The compact layout of every axis, or a stored box on some axis.
@@ -342,8 +368,10 @@ static bool a3_compact(const DevMesh& m, const FieldGeom& fg) {
 template <int LL, typename PosT, typename ValT, typename AccT>
 static void launch_a3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const ValT* g,
                       const FieldGeom& fg, ValT* gu, ValT* gx, int lda, int c0, int acc, hipStream_t s) {
-    if (a3_compact(m, fg)) launch_a3_form<LL, PosT, ValT, AccT, false>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
-    else launch_a3_form<LL, PosT, ValT, AccT, true>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
+    if (!a3_compact(m, fg)) launch_a3_form<LL, PosT, ValT, AccT, true>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
+    else if (plan->bin_table)
+        launch_a3_form<LL, PosT, ValT, AccT, false, true>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
+    else launch_a3_form<LL, PosT, ValT, AccT, false>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
 }
 
 // One launch per three components (the position gradient accumulates over
@@ -386,7 +414,8 @@ bool adjoint3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg
 int adjoint3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
                    const FieldGeom& fg, const void* g, void* gu, void* gx, hipStream_t s) {
     if (np <= 0) return 0;
-    if (materialize_heads(plan, s)) return PPMG_ERROR_FATAL;
+    // the compact cotangent reads a slot table as it is; a stored box's takes the heads
+    if (!a3_compact(m, fg) && materialize_heads(plan, s)) return PPMG_ERROR_FATAL;
     if (prec == PPMG_PREC_F64)
         return adjoint3d_typed<double, double>(plan, m, np, (const double*)xp, lda, (const double*)up,
                                                (const double*)g, fg, (double*)gu, (double*)gx, s);

@@ -97,13 +97,6 @@ __device__ __forceinline__ void shift_in(const ValT (&wx)[KS], const ValT (&wz)[
 // cell c at bt.table[r * ncells + c]) instead of heads and a permutation: a
 // cell's count and its first particle load together, one dependent load less
 // before the particle's own.
-
-// A slot-table binning as the tiled kernels read it.
-struct BinTable {
-    const uint32_t* count;
-    const int* table;
-    int64_t ncells;
-};
 //
 // KS = 2 (BSP2) is the same march with a 1-cell halo: the window is 16 x
 // (TY+1) cells for 15 x TY nodes, lane ex finishes node ex-1 from lanes ex

@@ -30,6 +30,13 @@ __host__ __device__ inline unsigned count_shift(int64_t c) { return (unsigned)((
 // meets at most two particles per axis.
 constexpr int TABLE_SLOTS = 8;
 
+// A slot-table binning as the tiled kernels read it.
+struct BinTable {
+    const uint32_t* count;
+    const int* table;
+    int64_t ncells;
+};
+
 int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, int64_t total, hipStream_t s);
 int fill_zero(void* p, size_t bytes, hipStream_t s);
 // Bins np particles into the plan (key, perm, start); *nbad = particles outside the sub.
