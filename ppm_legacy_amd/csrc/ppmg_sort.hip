@@ -303,11 +303,17 @@ __device__ __forceinline__ void set_heads(int* __restrict__ start, int lo, int h
 
 // Keys of all particles.  err[0] counts the particles in no cell; err[1] is
 // set to 1 when a descent key[p] < key[p-1] exists (none: the input is already
-// in cell order, e.g. particles made by create_part or a lattice).  Lanes get
-// their predecessor's key by a shuffle; lane 0 of each wavefront recomputes it
-// from the previous particle's position, so descents across wavefront seams
-// are caught here too.  The flag is read before it is written, so the many
-// wavefronts of unsorted input do not queue on one word.
+// in cell order, e.g. particles made by create_part or a lattice).  A
+// wavefront takes KEYS_PPT runs of 64 consecutive particles: all their
+// positions are loaded first, then (SLOT) all their atomics issued, so each
+// lane keeps KEYS_PPT loads and atomics in flight instead of waiting for one
+// round trip at a time (one particle per thread streams the positions at
+// 2.9 TB/s, bound by the atomic's latency after each load).  Lanes get their
+// predecessor's key by a shuffle, lane 0 from the run before (lane 63) or,
+// for the first run, from the previous particle's position, so descents
+// across runs and wavefront seams are caught here too.  The flag is read
+// before it is written, so the many wavefronts of unsorted input do not
+// queue on one word.
 //   SLOT (unsorted input expected): write the keys and take each particle's
 //     arrival slot in its cell by an atomic count (byte-wide: slots 0..254).
 //   SLOT and TABLE: no keys or slots written; particle p, arriving r-th in
@@ -319,6 +325,9 @@ __device__ __forceinline__ void set_heads(int* __restrict__ start, int lo, int h
 //     closes the array with np.  No counts, no scan, no keys written, and the
 //     permutation is the identity.  Valid only if no descent is found (else
 //     the caller bins again on the SLOT path).
+constexpr int KEYS_PPT = 4;                    // runs of 64 particles per wavefront
+constexpr int KEYS_BLOCK = 256 * KEYS_PPT;     // particles per key workgroup (and list capacity)
+
 template <int DIM, typename PosT, bool SLOT, bool TABLE>
 __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64_t np, DevMesh m, int64_t ncells,
                                               int* __restrict__ key, uint8_t* __restrict__ slot,
@@ -327,63 +336,101 @@ __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64
                                               int* __restrict__ multi, int* __restrict__ multi_n,
                                               int* __restrict__ table) {
     static_assert(SLOT || !TABLE, "the slot table needs the arrival slots");
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const bool live = p < np;
-    // the particle before this wavefront's first, at a wavefront-uniform
-    // address: read alongside the lanes' own positions, not after them
-    const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
-    const bool has_seam = p0 > 0 && p0 <= np;
+    // first particle of this wavefront's runs (wavefront-uniform)
+    const int64_t w0 = ((int64_t)blockIdx.x * 256 + __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u)) * KEYS_PPT;
+    const bool has_seam = w0 > 0 && w0 <= np;
     // all loads first (the key arithmetic branches on mesh flags, which would
     // otherwise keep each load behind the previous component's arithmetic)
-    PosT xv[DIM], sv[DIM];
+    // (past the end a lane reads the last particle again, unconditionally:
+    // no branch between the loads)
+    PosT xv[KEYS_PPT][DIM], sv[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        xv[d] = live ? xp[p * DIM + d] : PosT(0);
-        sv[d] = has_seam ? xp[(p0 - 1) * DIM + d] : PosT(0);
+    for (int j = 0; j < KEYS_PPT; ++j) {
+        const int64_t p = min(w0 + j * 64 + lane, np - 1);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) xv[j][d] = xp[p * DIM + d];
     }
-    const int seam = has_seam ? cell_key<DIM, PosT>(sv, m) : -1;
-    const int k = live ? cell_key<DIM, PosT>(xv, m) : -1;
-    int left = __shfl_up(k, 1);
-    if (lane == 0) left = seam;
-    const bool bad = live && k < 0;
-    const bool desc = live && p > 0 && k < left;
-    const uint64_t bb = __ballot(bad), bd = __ballot(desc);
-    if (lane == 0) {
-        if (bb) atomicAdd(&err[0], (int)__popcll(bb));
-        if (bd && *(volatile int*)&err[1] == 0) err[1] = 1;
-    }
-    if (SLOT) {
-        // arrival slot; the particle that arrives second in a cell lists the
-        // cell in its workgroup's block of `multi` for k_stabilize_listed
-        __shared__ int listed[256];
-        __shared__ int nlisted, ncrowd;
-        if (threadIdx.x == 0) nlisted = ncrowd = 0;
-        int sl = -1;
-        if (live) {
-            if (!TABLE) key[p] = k;
-            if (k >= 0) {
-                // byte-wide counts (count_word / count_shift)
-                const unsigned sh = count_shift(k);
-                sl = (int)((atomicAdd(&count[count_word(k)], 1u << sh) >> sh) & 255u);
-                if (!TABLE) slot[p] = (uint8_t)sl;
-                else if (sl < TABLE_SLOTS) table[(int64_t)sl * ncells + k] = (int)p;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) sv[d] = has_seam ? xp[(w0 - 1) * DIM + d] : PosT(0);
+    int k[KEYS_PPT];
+    int nbad = 0;
+    uint64_t anydesc = 0;
+    bool desc[KEYS_PPT];
+    int prev = has_seam ? cell_key<DIM, PosT>(sv, m) : -1;   // key before run j's lane 0
+#pragma unroll
+    for (int j = 0; j < KEYS_PPT; ++j) {
+        const int64_t p = w0 + j * 64 + lane;
+        const bool live = p < np;
+        k[j] = live ? cell_key<DIM, PosT>(xv[j], m) : -1;
+        int left = __shfl_up(k[j], 1);
+        if (lane == 0) left = prev;
+        prev = __builtin_amdgcn_readlane(k[j], 63);
+        desc[j] = live && p > 0 && k[j] < left;
+        nbad += (int)__popcll(__ballot(live && k[j] < 0));
+        anydesc |= __ballot(desc[j]);
+        if (!SLOT) {
+            // heads: the first particle of a run of equal keys
+            if (live && k[j] >= 0 && !desc[j]) {
+                if (k[j] != left) set_heads(start, left + 1, k[j], (int)p, err, gaps, gapcap);
+                if (p == np - 1) set_heads(start, k[j] + 1, (int)ncells, (int)np, err, gaps, gapcap);
             }
         }
-        // a cell's 256th arrival carries into the next cell's byte: err[3]
-        // sends the caller to the full-width count
-        const uint64_t ov = __ballot(sl == 255);
-        if (ov && lane == 0) err[3] = 1;
-        if (TABLE) {
-            const uint64_t full = __ballot(sl >= TABLE_SLOTS);
-            if (full && lane == 0 && *(volatile int*)&err[4] == 0) err[4] = 1;
+    }
+    // (last on the SLOT path: the flag's read waits for every memory
+    // operation issued before it)
+    auto report = [&]() {
+        if (lane == 0) {
+            if (nbad) atomicAdd(&err[0], nbad);
+            if (anydesc && *(volatile int*)&err[1] == 0) err[1] = 1;
         }
+    };
+    if (!SLOT) report();
+    if constexpr (SLOT) {
+        // arrival slots, all runs' atomics in flight together: issued without
+        // branches (a lane with no cell adds 0 to word 0) and decoded after
+        // the last one, so no atomic waits for the one before
+        uint32_t raw[KEYS_PPT];
+#pragma unroll
+        for (int j = 0; j < KEYS_PPT; ++j) {
+            // byte-wide counts (count_word / count_shift)
+            const bool in = k[j] >= 0;
+            raw[j] = atomicAdd(&count[in ? count_word(k[j]) : 0], in ? 1u << count_shift(k[j]) : 0u);
+        }
+        int sl[KEYS_PPT];
+#pragma unroll
+        for (int j = 0; j < KEYS_PPT; ++j) sl[j] = k[j] >= 0 ? (int)((raw[j] >> count_shift(k[j])) & 255u) : -1;
+        // The particle that arrives second in a cell lists the cell in its
+        // workgroup's block of `multi` for the stabilisation.
+        __shared__ int listed[KEYS_BLOCK];
+        __shared__ int nlisted, ncrowd;
+        if (threadIdx.x == 0) nlisted = ncrowd = 0;
+        bool ov = false, full = false;
+        int crowd = 0;
+#pragma unroll
+        for (int j = 0; j < KEYS_PPT; ++j) {
+            const int64_t p = w0 + j * 64 + lane;
+            if (p < np) {
+                if (!TABLE) {
+                    key[p] = k[j];
+                    if (k[j] >= 0) slot[p] = (uint8_t)sl[j];
+                } else if (sl[j] >= 0 && sl[j] < TABLE_SLOTS) {
+                    table[(int64_t)sl[j] * ncells + k[j]] = (int)p;
+                }
+            }
+            // a cell's 256th arrival carries into the next cell's byte: err[3]
+            // sends the caller to the full-width count
+            ov |= sl[j] == 255;
+            full |= sl[j] >= TABLE_SLOTS;
+            crowd += sl[j] == 8;
+        }
+        if (__ballot(ov) && lane == 0) err[3] = 1;
+        if (TABLE && __ballot(full) && lane == 0 && *(volatile int*)&err[4] == 0) err[4] = 1;
         __syncthreads();
-        if (sl == 1) listed[atomicAdd(&nlisted, 1)] = k;
-        if ((blockIdx.x & 63) == 0) {
-            const uint64_t crowd = __ballot(sl == 8);
-            if (crowd && lane == 0) atomicAdd(&ncrowd, (int)__popcll(crowd));
-        }
+#pragma unroll
+        for (int j = 0; j < KEYS_PPT; ++j)
+            if (sl[j] == 1) listed[atomicAdd(&nlisted, 1)] = k[j];
+        if ((blockIdx.x & 63) == 0 && crowd > 0) atomicAdd(&ncrowd, crowd);
         __syncthreads();
         const int n = nlisted;
         // err[8 .. 71]: the cells whose ninth particle arrived in one key
@@ -393,13 +440,10 @@ __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64
         // workgroup on a single word 0.85 ms)
         if (threadIdx.x == 0 && ncrowd > 0 && (blockIdx.x & 63) == 0)
             atomicAdd(&err[8 + ((blockIdx.x >> 6) & 63)], ncrowd);
-        if ((int)threadIdx.x < n) multi[(int64_t)blockIdx.x * 256 + threadIdx.x] = listed[threadIdx.x];
+        for (int i = (int)threadIdx.x; i < n; i += 256) multi[(int64_t)blockIdx.x * KEYS_BLOCK + i] = listed[i];
         if (threadIdx.x == 0) multi_n[blockIdx.x] = n;
-        return;
+        report();
     }
-    if (!live || k < 0 || desc) return;
-    if (k != left) set_heads(start, left + 1, k, (int)p, err, gaps, gapcap);
-    if (p == np - 1) set_heads(start, k + 1, (int)ncells, (int)np, err, gaps, gapcap);
 }
 
 // Long runs of empty cells queued by k_keys: one workgroup per entry.
@@ -518,9 +562,10 @@ __global__ void __launch_bounds__(256) k_stabilize(const int* __restrict__ start
 }
 
 // The same for the cells k_keys listed (those with two or more particles).
-// A key workgroup lists a few cells at most, so one wavefront takes the lists
-// of STAB_GROUP consecutive key workgroups, spread over its lanes.
-constexpr int STAB_GROUP = 8;
+// A key workgroup lists a few hundred cells at most (about 280 of its 1024
+// particles on an advected lattice), so one wavefront takes the lists of
+// STAB_GROUP consecutive key workgroups, spread over its lanes.
+constexpr int STAB_GROUP = 2;
 __global__ void __launch_bounds__(64) k_stabilize_listed(const int* __restrict__ start,
                                                          const int* __restrict__ multi,
                                                          const int* __restrict__ multi_n, int64_t nlists,
@@ -533,7 +578,7 @@ __global__ void __launch_bounds__(64) k_stabilize_listed(const int* __restrict__
     for (int q = (int)threadIdx.x; q < off[STAB_GROUP]; q += 64) {
         int j = 0;
         while (q >= off[j + 1]) ++j;
-        const int c = multi[(l0 + j) * 256 + (q - off[j])];
+        const int c = multi[(l0 + j) * KEYS_BLOCK + (q - off[j])];
         const int b = start[c], e = start[c + 1];
         for (int i = b + 1; i < e; ++i) {
             const int v = perm[i];
@@ -561,7 +606,7 @@ __global__ void __launch_bounds__(64) k_stabilize_table(const uint32_t* __restri
     for (int q = (int)threadIdx.x; q < off[STAB_GROUP]; q += 64) {
         int j = 0;
         while (q >= off[j + 1]) ++j;
-        const int c = multi[(l0 + j) * 256 + (q - off[j])];
+        const int c = multi[(l0 + j) * KEYS_BLOCK + (q - off[j])];
         const int n = (int)((count[count_word(c)] >> count_shift(c)) & 255u);
         int v[TABLE_SLOTS];
 #pragma unroll
@@ -641,8 +686,8 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
         plan->perm.ensure(sizeof(int) * npb) || plan->count.ensure(count_alloc) ||
         plan->start.ensure(sizeof(int) * (size_t)(ncells + 1)) ||
         plan->gaps.ensure(sizeof(int4) * (size_t)gapcap) ||
-        plan->multi.ensure(sizeof(int) * (size_t)nblk(np, 256) * 256) ||
-        plan->multi_n.ensure(sizeof(int) * (size_t)nblk(np, 256)) ||
+        plan->multi.ensure(sizeof(int) * (size_t)nblk(np, KEYS_BLOCK) * KEYS_BLOCK) ||
+        plan->multi_n.ensure(sizeof(int) * (size_t)nblk(np, KEYS_BLOCK)) ||
         plan->err.ensure(sizeof(int) * ERR_WORDS))
         return PPMG_ERROR_FATAL;
     plan->bin_stream = s;
@@ -686,7 +731,7 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     int* multi = plan->multi.as<int>();
     int* multi_n = plan->multi_n.as<int>();
     {
-        const unsigned g = nblk(np, 256);
+        const unsigned g = nblk(np, KEYS_BLOCK);
 #define KEYS(D, T, S, TB) hipLaunchKernelGGL((k_keys<D, T, S, TB>), dim3(g), dim3(256), 0, s, (const T*)xp, np, m, \
                                              ncells, key, slot8, count8, start, err, gaps, (int)gapcap, multi, multi_n, tab)
 #define KEYS_T(D, T)                                   \
@@ -742,8 +787,8 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
             return bin_particles(plan, m, prec, np, xp, s, nbad, false);
         }
         // the listed cells (two or more particles) in increasing particle index
-        hipLaunchKernelGGL(k_stabilize_table, dim3(nblk(nblk(np, 256), STAB_GROUP)), dim3(64), 0, s, count8, multi,
-                           multi_n, (int64_t)nblk(np, 256), ncells, tab);
+        hipLaunchKernelGGL(k_stabilize_table, dim3(nblk(nblk(np, KEYS_BLOCK), STAB_GROUP)), dim3(64), 0, s, count8,
+                           multi, multi_n, (int64_t)nblk(np, KEYS_BLOCK), ncells, tab);
         PPMG_HIP(hipGetLastError());
         plan->bin_table = true;
         plan->perm_identity = false;
@@ -777,8 +822,8 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
             hipLaunchKernelGGL(k_stabilize, dim3(nblk(ncells, 256)), dim3(256), 0, s, start, ncells,
                                plan->perm.as<int>());
         else
-            hipLaunchKernelGGL(k_stabilize_listed, dim3(nblk(nblk(np, 256), STAB_GROUP)), dim3(64), 0, s, start,
-                               multi, multi_n, (int64_t)nblk(np, 256), plan->perm.as<int>());
+            hipLaunchKernelGGL(k_stabilize_listed, dim3(nblk(nblk(np, KEYS_BLOCK), STAB_GROUP)), dim3(64), 0, s,
+                               start, multi, multi_n, (int64_t)nblk(np, KEYS_BLOCK), plan->perm.as<int>());
     }
     PPMG_HIP(hipGetLastError());
     // input in cell order: the permutation is the identity, written only on demand
