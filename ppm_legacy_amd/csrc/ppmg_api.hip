@@ -230,7 +230,7 @@ int ppmg_plan_destroy(ppmg_plan* plan) {
     if (!plan) return 0;
     DevBuf* bufs[] = {&plan->key, &plan->slot, &plan->count, &plan->start, &plan->perm, &plan->wsorted,
                       &plan->usorted, &plan->scan_tmp, &plan->err, &plan->misc, &plan->misc2, &plan->gaps, &plan->multi, &plan->multi_n, &plan->ghost,
-                      &plan->table};
+                      &plan->table, &plan->rhist, &plan->rkey[0], &plan->rkey[1], &plan->rval[0], &plan->rval[1]};
     for (DevBuf* b : bufs) b->release();
     if (plan->ev_init)
         for (auto& e : plan->ev) (void)hipEventDestroy(e);

@@ -379,7 +379,8 @@ bool arena_bound();          // an arena is bound on the calling thread
 // Opaque plan: scratch of the binning and the forward record for the adjoint.
 struct ppmg_plan {
     ppmg::DevBuf key, slot, count, start, perm, wsorted, usorted, scan_tmp, err, misc, misc2, gaps, multi,
-        multi_n, ghost, table;
+        multi_n, ghost, table, rhist;
+    ppmg::DevBuf rkey[2], rval[2];   // ping-pong pairs of the radix-sort binning of crowded input
     int64_t np = 0;
     int64_t ncells = 0;
     bool sorted_input = false;   // last binning found the input already in cell order

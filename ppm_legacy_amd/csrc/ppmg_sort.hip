@@ -674,6 +674,264 @@ __global__ void __launch_bounds__(256) k_keys_only(const PosT* __restrict__ xp, 
     key[p] = cell_key<DIM, PosT>(xv, m);
 }
 
+// ------------------------------------------------------ crowded input ---
+// Cells of tens of particles in no particular order (clustered particles):
+// k_keys' atomic slots contend on the crowded cells' count words, the scatter
+// writes at random and the per-cell insertion sorts grow with the square of
+// the occupancy.  Such input is binned by a stable LSD radix sort of the
+// (cell key, particle) pairs instead, RS_BITS bits a pass: no atomics on
+// crowded words, writes in runs, and a stable sort of the keys in particle
+// order is the reference's permutation as it stands (ppm_util_rank3d.f:
+// 505-532).  The heads follow from the runs of the sorted keys.
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // pairs per workgroup
+constexpr int RS_BITS = 8;
+constexpr int RS_BUCKETS = 1 << RS_BITS;
+static_assert(RS_BUCKETS == RS_THREADS, "one digit per thread in the tile loops");
+
+// Keys of all particles (key[p]), the particles in no cell (err[0]) and the
+// descent flag (err[1]), as k_keys reports them.
+template <int DIM, typename PosT>
+__global__ void __launch_bounds__(256) k_keys_plain(const PosT* __restrict__ xp, int64_t np, DevMesh m,
+                                                    int* __restrict__ key, int* __restrict__ err) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool live = p < np;
+    const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
+    const bool has_seam = p0 > 0 && p0 <= np;
+    PosT xv[DIM], sv[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        xv[d] = live ? xp[p * DIM + d] : PosT(0);
+        sv[d] = has_seam ? xp[(p0 - 1) * DIM + d] : PosT(0);
+    }
+    const int seam = has_seam ? cell_key<DIM, PosT>(sv, m) : -1;
+    const int k = live ? cell_key<DIM, PosT>(xv, m) : -1;
+    int left = __shfl_up(k, 1);
+    if (lane == 0) left = seam;
+    const uint64_t bb = __ballot(live && k < 0), bd = __ballot(live && p > 0 && k < left);
+    if (live) key[p] = k;
+    if (lane == 0) {
+        if (bb) atomicAdd(&err[0], (int)__popcll(bb));
+        if (bd && *(volatile int*)&err[1] == 0) err[1] = 1;
+    }
+}
+
+// Lanes of the wavefront holding the same digit as this lane (among `live`).
+__device__ __forceinline__ uint64_t digit_peers(int d, bool live) {
+    uint64_t peers = __ballot(live);
+#pragma unroll
+    for (int b = 0; b < RS_BITS; ++b) {
+        const uint64_t on = __ballot((d >> b) & 1);
+        peers &= ((d >> b) & 1) ? on : ~on;
+    }
+    return peers;
+}
+
+// Digit counts of every tile, digit-major: hist[d * ntiles + tile].  Tiles
+// are numbered XCD-contiguously (xcd_tile), as in k_rs_scatter: the tiles
+// one XCD sorts are neighbours, and so are their runs of each digit.
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const int* __restrict__ keys, int64_t n, int shift,
+                                                        int* __restrict__ hist, int64_t ntiles) {
+    __shared__ int h[RS_BUCKETS];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int64_t tile = xcd_tile(blockIdx.x, (int64_t)gridDim.x);
+    h[tid] = 0;
+    __syncthreads();
+    const int64_t t0 = tile * RS_TILE;
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int64_t i = t0 + j * RS_THREADS + tid;
+        const bool live = i < n;
+        const int d = live ? (keys[i] >> shift) & (RS_BUCKETS - 1) : 0;
+        // one LDS add per digit and wavefront
+        const uint64_t peers = digit_peers(d, live);
+        if (live && (int)__builtin_ctzll(peers) == lane) atomicAdd(&h[d], (int)__popcll(peers));
+    }
+    __syncthreads();
+    hist[(int64_t)tid * ntiles + tile] = h[tid];
+}
+
+// One stable pass.  The pairs of a tile are first ordered by digit in LDS: a
+// tile is taken RS_THREADS pairs a round in index order, and a pair's place
+// is its digit's start in the tile plus its rank among the tile's pairs of
+// that digit (the count from earlier rounds, earlier wavefronts of its round
+// and earlier lanes of its wavefront).  The tile then writes its digit runs
+// at their offsets (offs: the exclusive scan of hist), consecutive threads to
+// consecutive addresses.  vin nullptr: the values are the pairs' indices
+// (the first pass).
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const int* __restrict__ kin, const int* __restrict__ vin,
+                                                           int64_t n, int shift, const int* __restrict__ hist,
+                                                           const int* __restrict__ offs, int64_t ntiles,
+                                                           int* __restrict__ kout, int* __restrict__ vout) {
+    constexpr int NWAVE = RS_THREADS / 64;
+    __shared__ int base[RS_BUCKETS];     // next place of each digit in the tile's order
+    __shared__ int shift_out[RS_BUCKETS];   // global position minus place, per digit
+    __shared__ int wcnt[NWAVE][RS_BUCKETS];
+    __shared__ int sk[RS_TILE], sv[RS_TILE];
+    __shared__ int64_t sh[RS_THREADS / 64 + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t tile = xcd_tile(blockIdx.x, (int64_t)gridDim.x);
+    const int64_t t0 = tile * RS_TILE;
+    const int tn = (int)min((int64_t)RS_TILE, n - t0);
+    // thread t = digit t: its start in the tile (exclusive scan of the tile's counts)
+    int64_t lb;
+    block_exclusive_scan((int64_t)hist[(int64_t)tid * ntiles + tile], sh, lb);
+    base[tid] = (int)lb;
+    shift_out[tid] = offs[(int64_t)tid * ntiles + tile] - (int)lb;
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w) wcnt[w][tid] = 0;
+    const uint64_t lt = lane ? ~0ull >> (64 - lane) : 0ull;   // lanes below this one
+    __syncthreads();
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int64_t i = t0 + j * RS_THREADS + tid;
+        const bool live = i < n;
+        const int k = live ? kin[i] : 0;
+        const int v = live ? (vin ? vin[i] : (int)i) : 0;
+        const int d = (k >> shift) & (RS_BUCKETS - 1);
+        const uint64_t peers = digit_peers(d, live);
+        if (live && (int)__builtin_ctzll(peers) == lane) wcnt[wave][d] = (int)__popcll(peers);
+        __syncthreads();
+        if (live) {
+            int r = base[d] + (int)__popcll(peers & lt);
+            for (int w = 0; w < wave; ++w) r += wcnt[w][d];
+            sk[r] = k;
+            sv[r] = v;
+        }
+        __syncthreads();
+        // thread t owns digit t: advance its place, clear its counts
+        int add = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVE; ++w) {
+            add += wcnt[w][tid];
+            wcnt[w][tid] = 0;
+        }
+        base[tid] += add;
+        __syncthreads();
+    }
+    for (int q = tid; q < tn; q += RS_THREADS) {
+        const int k = sk[q];
+        const int o = shift_out[(k >> shift) & (RS_BUCKETS - 1)] + q;
+        kout[o] = k;
+        vout[o] = sv[q];
+    }
+}
+
+// Cell counts from the sorted keys: each run of equal keys adds its length
+// to cnt[key] (zeroed) by two atomics at its ends, on distinct runs' words;
+// a run of one particle stores its count.
+// (Writing the heads straight from the runs, as the cell-order path does,
+// took longer on crowded input: most cells are empty, and a lane fills each
+// run of empty cells before its own or queues it on one counter word.)  One
+// workgroup in 64 counts its runs of nine or more into err[8 .. 71] (the
+// same sample k_keys takes).
+__global__ void __launch_bounds__(256) k_rs_runs(const int* __restrict__ ks, int64_t n, int* __restrict__ cnt,
+                                                 int* __restrict__ err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool crowd = false;
+    if (i < n) {
+        const int k = ks[i];
+        const bool first = i == 0 || ks[i - 1] != k, last = i == n - 1 || ks[i + 1] != k;
+        if (first && last) {
+            cnt[k] = 1;
+        } else {
+            if (first) atomicAdd(&cnt[k], -(int)i);
+            if (last) atomicAdd(&cnt[k], (int)i + 1);
+        }
+        crowd = last && i >= 8 && ks[i - 8] == k;
+    }
+    if ((blockIdx.x & 63) == 0) {
+        const uint64_t c = __ballot(crowd);
+        if (c && (threadIdx.x & 63) == 0) atomicAdd(&err[8 + ((blockIdx.x >> 6) & 63)], (int)__popcll(c));
+    }
+}
+
+// Radix-sort binning of crowded input (see above); the keys pass has run.
+static int bin_radix(ppmg_plan* plan, int64_t np, int64_t ncells, hipStream_t s, const int** sorted_keys) {
+    const int64_t ntiles = (np + RS_TILE - 1) / RS_TILE;
+    if (plan->rkey[0].ensure(sizeof(int) * (size_t)np) || plan->rkey[1].ensure(sizeof(int) * (size_t)np) ||
+        plan->rval[0].ensure(sizeof(int) * (size_t)np) || plan->rval[1].ensure(sizeof(int) * (size_t)np) ||
+        plan->rhist.ensure(sizeof(int) * (size_t)(2 * (RS_BUCKETS * ntiles + 4))))
+        return PPMG_ERROR_FATAL;
+    int bits = 1;
+    while (bits < 31 && ((int64_t)1 << bits) < ncells) ++bits;
+    const int passes = (bits + RS_BITS - 1) / RS_BITS;
+    int* hist = plan->rhist.as<int>();
+    int* offs = hist + RS_BUCKETS * ntiles + 4;   // 16-byte aligned, n + 1 entries
+    const int* kin = plan->key.as<int>();
+    const int* vin = nullptr;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = pass * RS_BITS;
+        const bool last = pass == passes - 1;
+        int* kout = plan->rkey[pass & 1].as<int>();
+        int* vout = last ? plan->perm.as<int>() : plan->rval[pass & 1].as<int>();
+        hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, np, shift, hist, ntiles);
+        if (exclusive_scan(hist, RS_BUCKETS * ntiles, offs, plan->scan_tmp, np, s)) return PPMG_ERROR_FATAL;
+        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, vin, np, shift, hist,
+                           offs, ntiles, kout, vout);
+        kin = kout;
+        vin = vout;
+    }
+    *sorted_keys = kin;
+    PPMG_HIP(hipGetLastError());
+    return 0;
+}
+
+// Heads of the sorted keys ks (run lengths into zeroed counts, then their
+// exclusive scan), and the plan's crowded-cell estimate from the sample.
+static int heads_from_sorted(ppmg_plan* plan, const int* ks, int64_t np, int64_t ncells, hipStream_t s) {
+    int* err = plan->err.as<int>();
+    if (plan->count.ensure(sizeof(int) * (size_t)ncells)) return PPMG_ERROR_FATAL;
+    int* cnt = plan->count.as<int>();
+    if (fill_zero(cnt, sizeof(int) * (size_t)ncells, s)) return PPMG_ERROR_FATAL;
+    hipLaunchKernelGGL(k_rs_runs, dim3(nblk(np, 256)), dim3(256), 0, s, ks, np, cnt, err);
+    if (exclusive_scan(cnt, ncells, plan->start.as<int>(), plan->scan_tmp, np, s)) return PPMG_ERROR_FATAL;
+    PPMG_HIP(hipGetLastError());
+    int sample[64];
+    PPMG_HIP(hipMemcpyAsync(sample, err + 8, sizeof(sample), hipMemcpyDeviceToHost, s));
+    PPMG_HIP(hipStreamSynchronize(s));
+    int64_t sampled = 0;
+    for (int v : sample) sampled += v;
+    plan->crowded_cells = 64 * sampled;
+    return 0;
+}
+
+// The binning of crowded input: keys pass, then (input not in cell order)
+// the radix sort, and the heads; the plan's crowded-cell estimate is taken
+// again from the sorted runs.
+static int bin_crowded(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, hipStream_t s,
+                       int64_t* nbad) {
+    const int64_t ncells = plan->ncells;
+    int* key = plan->key.as<int>();
+    int* err = plan->err.as<int>();
+    const unsigned g = nblk(np, 256);
+    if (m.dim == 3) {
+        if (prec == PPMG_PREC_F32) hipLaunchKernelGGL((k_keys_plain<3, float>), dim3(g), dim3(256), 0, s, (const float*)xp, np, m, key, err);
+        else hipLaunchKernelGGL((k_keys_plain<3, double>), dim3(g), dim3(256), 0, s, (const double*)xp, np, m, key, err);
+    } else {
+        if (prec == PPMG_PREC_F32) hipLaunchKernelGGL((k_keys_plain<2, float>), dim3(g), dim3(256), 0, s, (const float*)xp, np, m, key, err);
+        else hipLaunchKernelGGL((k_keys_plain<2, double>), dim3(g), dim3(256), 0, s, (const double*)xp, np, m, key, err);
+    }
+    PPMG_HIP(hipGetLastError());
+    int herr[2] = {0, 0};
+    PPMG_HIP(hipMemcpyAsync(herr, err, sizeof(herr), hipMemcpyDeviceToHost, s));
+    PPMG_HIP(hipStreamSynchronize(s));
+    *nbad = herr[0];
+    if (herr[0]) return 0;
+    // input already in cell order: the keys are the sorted keys, the
+    // permutation the identity (written on demand)
+    const bool sorted = herr[1] == 0;
+    const int* ks = key;
+    if (!sorted && bin_radix(plan, np, ncells, s, &ks)) return PPMG_ERROR_FATAL;
+    if (heads_from_sorted(plan, ks, np, ncells, s)) return PPMG_ERROR_FATAL;
+    plan->perm_identity = sorted;
+    plan->perm_written = !sorted;
+    plan->keys_written = true;
+    plan->sorted_input = sorted;
+    return 0;
+}
+
 int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp,
                   hipStream_t s, int64_t* nbad, bool table) {
     const int64_t ncells = (int64_t)m.nbin[0] * m.nbin[1] * m.nbin[2];
@@ -710,7 +968,6 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     // the same kind of particles each call (advected particles, or particles
     // fresh from create_part) predicts right and does each step once.
     const bool predict_sorted = plan->sorted_input;
-    if (!predict_sorted && fill_zero(plan->count.p, count_alloc, s)) return PPMG_ERROR_FATAL;
     // Unsorted input for a reader of the slot table: the key pass fills the
     // table, and no keys, scan or scatter follow.  A plan whose table
     // overflowed bins its next 16 inputs without one.  Allocation-free calls
@@ -720,8 +977,17 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     if (no_table && no_table[0] && no_table[0] != '0') table = false;
     const bool use_table = table && !predict_sorted && plan->table_skip == 0 && !arena_bound();
     if (table && !predict_sorted && plan->table_skip > 0) --plan->table_skip;
+    // Crowded input (the last binning's sample found a cell of nine or more
+    // particles per 64 particles) is binned by the radix sort; PPMG_RADIX=1
+    // forces that path for input not predicted in cell order, PPMG_RADIX=0
+    // rules it out (both for tests).  Allocation-free calls keep the counting
+    // path (the sort needs four more int32 per particle).
+    const char* rx = std::getenv("PPMG_RADIX");
+    const bool radix = rx && rx[0] ? rx[0] == '1' : plan->crowded_cells * 64 > np;
+    if (!predict_sorted && radix && !arena_bound()) return bin_crowded(plan, m, prec, np, xp, s, nbad);
     if (use_table && plan->table.ensure(sizeof(int) * TABLE_SLOTS * (size_t)ncells)) return PPMG_ERROR_FATAL;
     int* tab = use_table ? plan->table.as<int>() : nullptr;
+    if (!predict_sorted && fill_zero(plan->count.p, count_alloc, s)) return PPMG_ERROR_FATAL;
     int* key = plan->key.as<int>();
     uint8_t* slot8 = plan->slot.as<uint8_t>();
     uint32_t* count8 = plan->count.as<uint32_t>();

@@ -224,6 +224,55 @@ def test_binning_slot_table(core, oracle, kernel, prec, monkeypatch):
                               np.concatenate([[0], np.cumsum(np.bincount(key, minlength=N ** 3))])), (case, i)
 
 
+@pytest.mark.parametrize("dim,N", [(3, 32), (3, 300), (2, 96)])
+def test_binning_crowded_radix(core, oracle, dim, N, monkeypatch):
+    """Crowded input is binned by a stable radix sort of (cell key, particle)
+    pairs (8 bits a pass: two passes at 32^3 and 96^2 cells, four at 300^3),
+    with the heads from the sorted runs; a plan takes that path after a
+    binning that sampled many crowded cells, and PPMG_RADIX=1 forces it.
+    One plan is fed a Gaussian blob (hundreds per cell) over a uniform
+    background: shuffled (counting path), shuffled twice more and then in
+    cell order under PPMG_RADIX=1 (radix; in cell order the keys pass finds
+    no descent: identity permutation), and in cell order again (the
+    cell-order path).  Every call: keys, permutation and heads equal numpy's
+    stable argsort and bincount, and the field equals bitwise that of a
+    fresh plan under PPMG_RADIX=0 (counting path), whose per-node addition
+    order is the same; at 32^3 and 96^2 also the oracle's field."""
+    rng = np.random.default_rng(900 + N)
+    nb, nu = (6000, 3000) if N < 200 else (60000, 30000)
+    base = np.concatenate([rng.normal(0.4, 0.6 / N, size=(nb, dim)) % 1.0, rng.uniform(0, 1, size=(nu, dim))])
+    cells = np.minimum((base * N).astype(np.int64), N - 1)
+    key = cells[:, 0] + N * cells[:, 1] + (N * N * cells[:, 2] if dim == 3 else 0)
+    ncells = N ** dim
+    assert np.bincount(key).max() > 50
+    mesh = core.Mesh(dim, N + 1)
+    plan = core.Plan()
+    t = oracle.topo(dim, [N + 1] * dim) if N < 200 else None
+    for call, order in enumerate(["shuffled", "shuffled", "shuffled", "cell_sorted", "cell_sorted"]):
+        o = np.argsort(key, kind="stable") if order == "cell_sorted" else rng.permutation(len(base))
+        xp, k = base[o], key[o]
+        up = rng.normal(size=(len(xp), 3))
+        x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
+        f = torch.empty(mesh.compact_shape(3), dtype=torch.float64, device="cuda")
+        if call in (1, 2, 3):
+            monkeypatch.setenv("PPMG_RADIX", "1")
+        core.p2m(mesh, x, u, f, [0] * dim, MP4, F64, plan)
+        monkeypatch.setenv("PPMG_RADIX", "0")
+        kk, perm, start = plan.sort_view()
+        assert np.array_equal(kk.cpu().numpy(), k), (order, call)
+        assert np.array_equal(perm.cpu().numpy(), np.argsort(k, kind="stable")), (order, call)
+        assert np.array_equal(start.cpu().numpy(),
+                              np.concatenate([[0], np.cumsum(np.bincount(k, minlength=ncells))])), (order, call)
+        ref = torch.empty_like(f)
+        core.p2m(mesh, x, u, ref, [0] * dim, MP4, F64)
+        monkeypatch.delenv("PPMG_RADIX")
+        assert torch.equal(f, ref), (order, call)
+        if t is not None:
+            info, oref = oracle.p2m(t, MP4, xp, up, [1] * dim)
+            assert info == 0
+            assert _rel(f.cpu().numpy(), oracle.unique_nodes(oref, [1] * dim, dim)) < 1e-12, (order, call)
+
+
 # ----------------------------------------------------------- variants --
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
