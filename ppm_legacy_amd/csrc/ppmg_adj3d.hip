@@ -368,10 +368,14 @@ static bool a3_compact(const DevMesh& m, const FieldGeom& fg) {
 template <int LL, typename PosT, typename ValT, typename AccT>
 static void launch_a3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const ValT* g,
                       const FieldGeom& fg, ValT* gu, ValT* gx, int lda, int c0, int acc, hipStream_t s) {
-    if (!a3_compact(m, fg)) launch_a3_form<LL, PosT, ValT, AccT, true>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
-    else if (plan->bin_table)
-        launch_a3_form<LL, PosT, ValT, AccT, false, true>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
-    else launch_a3_form<LL, PosT, ValT, AccT, false>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
+    const bool compact = a3_compact(m, fg);
+    if (plan->bin_table) {
+        if (compact) launch_a3_form<LL, PosT, ValT, AccT, false, true>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
+        else launch_a3_form<LL, PosT, ValT, AccT, true, true>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
+    } else {
+        if (compact) launch_a3_form<LL, PosT, ValT, AccT, false>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
+        else launch_a3_form<LL, PosT, ValT, AccT, true>(plan, m, xp, up, g, fg, gu, gx, lda, c0, acc, s);
+    }
 }
 
 // One launch per three components (the position gradient accumulates over
@@ -414,8 +418,6 @@ bool adjoint3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg
 int adjoint3d_fast(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp, int lda, const void* up,
                    const FieldGeom& fg, const void* g, void* gu, void* gx, hipStream_t s) {
     if (np <= 0) return 0;
-    // the compact cotangent reads a slot table as it is; a stored box's takes the heads
-    if (!a3_compact(m, fg) && materialize_heads(plan, s)) return PPMG_ERROR_FATAL;
     if (prec == PPMG_PREC_F64)
         return adjoint3d_typed<double, double>(plan, m, np, (const double*)xp, lda, (const double*)up,
                                                (const double*)g, fg, (double*)gu, (double*)gx, s);

@@ -425,7 +425,8 @@ int ppmg_m2p_prebin(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, con
     if (m.dim != 3) return 0;
     int64_t nbad = 0;
     plan->have_forward = false;
-    if (bin_particles(plan, m, prec, np, xp, as_stream(stream), &nbad)) return g_status.info;
+    if (bin_particles(plan, m, prec, np, xp, as_stream(stream), &nbad, m2p3d_table_applicable(m)))
+        return g_status.info;
     plan->prebinned = true;
     plan->prebin_xp = xp;
     plan->prebin_np = np;

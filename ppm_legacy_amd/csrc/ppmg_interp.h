@@ -39,6 +39,8 @@ int p2m2d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, const vo
 // holds their binning); returns 1 without touching up when a particle lies
 // outside the sub.
 bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
+// ... and it reads a slot-table binning (bin_particles' `table`) there.
+bool m2p3d_table_applicable(const DevMesh& m);
 int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp,
                const FieldGeom& fg, const void* field, void* up, bool prebinned, hipStream_t s);
 

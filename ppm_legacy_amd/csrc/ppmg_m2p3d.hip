@@ -64,10 +64,13 @@ __device__ __forceinline__ int raw_node(const DevMesh& m, const FieldGeom& fg, i
 // with 4 wavefronts against 3.18 ms with 3 (136 VGPRs unconstrained).
 // MIX: some open axis beside a periodic axis stored as a box, where whether a
 // node takes the periodic node map depends on all three of its coordinates.
-template <int KS, int LDA, typename PosT, typename ValT, typename AccT, int WPE, bool OPEN, bool MIX>
+// TABLE: the binning is a slot table (byte counts, particle r of cell c at
+// bt.table[r * ncells + c]) instead of heads and a permutation.
+template <int KS, int LDA, typename PosT, typename ValT, typename AccT, int WPE, bool OPEN, bool MIX, bool TABLE>
 __global__ void __launch_bounds__(M3_THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
-k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
-          const ValT* __restrict__ field, FieldGeom fg, ValT* __restrict__ up, int lda, int c0, int tz) {
+k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, BinTable bt,
+          const PosT* __restrict__ xp, const ValT* __restrict__ field, FieldGeom fg, ValT* __restrict__ up, int lda,
+          int c0, int tz) {
     constexpr int B = M3Geom<KS>::B, M3_TX = M3Geom<KS>::TX, M3_PL = M3Geom<KS>::PL;
     constexpr int RP = KS;                   // ring planes
     __shared__ ValT ring[RP * LDA * M3_PL];  // [plane][c][ry][nx]
@@ -155,16 +158,37 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     // Cell heads run two planes ahead of the computation, the permutation
     // entry of a plane's first particle one plane ahead, and its position is
     // loaded at the end of the plane before.
-    auto heads = [&](int z, int& f, int& n) {
+    // (f, n) is the cell's range [f, n) of the permutation, or (TABLE) its
+    // key and count word, with t its first particle.
+    auto heads = [&](int z, int& f, int& n, int& t) {
         f = 0;
         n = 0;
+        t = 0;
         if (plane_lane && z < z1) {
             const int key = (int)((int64_t)z * plane) + key0;
-            f = start[key];
-            n = start[key + 1];   // end, subtracted where used: the load stays in flight
+            if constexpr (TABLE) {
+                f = key;
+                n = (int)bt.count[count_word(key)];   // decoded where used: the load stays in flight
+                t = bt.table[key];
+            } else {
+                f = start[key];
+                n = start[key + 1];   // end, subtracted where used: the load stays in flight
+            }
         }
     };
-    auto lookup = [&](int f, int n) { return n > 0 ? (perm ? perm[f] : f) : 0; };
+    auto count_of = [&](int f, int n) {
+        if constexpr (TABLE) return (int)(((unsigned)n >> count_shift(f)) & 255u);
+        else return n - f;
+    };
+    // the cell's first particle, and its particle r < count
+    auto lookup = [&](int f, int n, int t) {
+        if constexpr (TABLE) return count_of(f, n) > 0 ? t : 0;
+        else return n > f ? (perm ? perm[f] : f) : 0;
+    };
+    auto entry = [&](int f, int r) {
+        if constexpr (TABLE) return bt.table[(int64_t)r * bt.ncells + f];
+        else return perm ? perm[f + r] : f + r;
+    };
     auto position = [&](int p, int n, PosT (&x)[3]) {
         x[0] = x[1] = x[2] = PosT(0);
         if (n > 0) {
@@ -173,25 +197,26 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             x[2] = xp[(int64_t)p * 3 + 2];
         }
     };
-    int first_n, cnt_n, first_nn, cnt_nn;
-    heads(z0, first_n, cnt_n);
-    heads(z0 + 1, first_nn, cnt_nn);
+    int first_n, cnt_n, t_n, first_nn, cnt_nn, t_nn;
+    heads(z0, first_n, cnt_n, t_n);
+    heads(z0 + 1, first_nn, cnt_nn, t_nn);
     // Round 0's particle of a plane is loaded at the end of the plane before,
     // while the workgroup refills the ring.
-    int pn = lookup(first_n, cnt_n - first_n);
+    int pn = lookup(first_n, cnt_n, t_n);
     PosT xn[3];
-    position(pn, cnt_n - first_n, xn);
+    position(pn, count_of(first_n, cnt_n), xn);
     __syncthreads();
 
     for (int cz = z0; cz < z1; ++cz) {
-        const int first = first_n, cnt = cnt_n - first_n;   // cnt_n: end of the range
+        const int first = first_n, cnt = count_of(first_n, cnt_n);
         // take this plane's heads before the next loads are issued, so that
         // waiting for them does not wait for those loads too
         asm volatile("" ::"v"(first), "v"(cnt) : "memory");
         first_n = first_nn;
         cnt_n = cnt_nn;
-        const int pnext = lookup(first_n, cnt_n - first_n);   // next plane's first particle
-        heads(cz + 2, first_nn, cnt_nn);
+        t_n = t_nn;
+        const int pnext = lookup(first_n, cnt_n, t_n);   // next plane's first particle
+        heads(cz + 2, first_nn, cnt_nn, t_nn);
         const bool more = cz + 1 < z1;
         const int next = cz - B + KS;            // the plane that replaces plane cz - B
         if (more) fetch_plane(next);
@@ -261,12 +286,12 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         if (__ballot(cnt > 0) != 0ull) round(pn, cnt > 0, xn);
         for (int r = 1; __ballot(cnt > r) != 0ull; ++r) {
             PosT x[3];
-            const int p = lookup(first + r, cnt - r);
+            const int p = cnt > r ? entry(first, r) : 0;
             position(p, cnt - r, x);
             round(p, cnt > r, x);
         }
         pn = pnext;
-        position(pn, cnt_n - first_n, xn);   // next plane's round 0
+        position(pn, count_of(first_n, cnt_n), xn);   // next plane's round 0
         __syncthreads();   // every row is done with plane cz-1: its slot takes plane cz+3
         if (more) store_plane(next);
         __syncthreads();
@@ -307,22 +332,27 @@ bool m2p3d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
     return (int64_t)fg.ext[0] * fg.ext[1] * fg.lda * 8 < ((int64_t)1 << 31);
 }
 
+// The gather reads a slot-table binning on a periodic mesh (whatever the
+// field's layout).
+bool m2p3d_table_applicable(const DevMesh& m) { return m.dim == 3 && m.periodic[0] && m.periodic[1] && m.periodic[2]; }
+
 // One launch of the gather for components c0 .. c0+LL-1; the z chunk comes
 // from the resident workgroup count (pick_z_chunk_balanced: each chunk loads
 // 3 extra node planes, cheap beside the balance short chunks give).
-template <int KS, int LL, typename PosT, typename ValT, typename AccT, bool OPEN, bool MIX>
+template <int KS, int LL, typename PosT, typename ValT, typename AccT, bool OPEN, bool MIX, bool TABLE = false>
 static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
                            const ValT* field, ValT* up, hipStream_t s) {
     constexpr int TX = M3Geom<KS>::TX;
     constexpr int WPE = KS == 4 && (LL == 4 || MIX) ? 3 : 4;
-    auto kernel = k_m3_tile<KS, LL, PosT, ValT, AccT, WPE, OPEN, MIX>;
+    auto kernel = k_m3_tile<KS, LL, PosT, ValT, AccT, WPE, OPEN, MIX, TABLE>;
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(kernel, M3_THREADS));
     const int tx = (m.nbin[0] + TX - 1) / TX, ty = (m.nbin[1] + M3_TY - 1) / M3_TY;
     const int tz = pick_z_chunk_balanced((int64_t)tx * ty, m.nbin[2], slots.load(), 32);
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.nbin[2] + tz - 1) / tz)));
+    const BinTable bt{plan->count.as<uint32_t>(), plan->table.as<int>(), plan->ncells};
     hipLaunchKernelGGL(kernel, grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
-                       tile_perm(plan), xp, field, fg, up, fg.lda, c0, tz);
+                       tile_perm(plan), bt, xp, field, fg, up, fg.lda, c0, tz);
 }
 
 // OPEN: some axis is open (a fully periodic mesh compiles without the clamp);
@@ -330,7 +360,10 @@ static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, co
 template <int KS, int LL, typename PosT, typename ValT, typename AccT>
 static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
                       const ValT* field, ValT* up, hipStream_t s) {
-    if (m.periodic[0] && m.periodic[1] && m.periodic[2])
+    // a slot table is binned on periodic meshes only (m2p3d_fast)
+    if (plan->bin_table)
+        launch_m3_open<KS, LL, PosT, ValT, AccT, false, false, true>(plan, m, xp, fg, c0, field, up, s);
+    else if (m.periodic[0] && m.periodic[1] && m.periodic[2])
         launch_m3_open<KS, LL, PosT, ValT, AccT, false, false>(plan, m, xp, fg, c0, field, up, s);
     else if (fg.swap[0] || fg.swap[1] || fg.swap[2])
         launch_m3_open<KS, LL, PosT, ValT, AccT, true, true>(plan, m, xp, fg, c0, field, up, s);
@@ -367,8 +400,10 @@ static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT
 int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp,
                const FieldGeom& fg, const void* field, void* up, bool prebinned, hipStream_t s) {
     int64_t nbad = prebinned ? plan->prebin_nbad : 0;
-    if (!prebinned && bin_particles(plan, m, prec, np, xp, s, &nbad)) return PPMG_ERROR_FATAL;
+    if (!prebinned && bin_particles(plan, m, prec, np, xp, s, &nbad, m2p3d_table_applicable(m))) return PPMG_ERROR_FATAL;
     if (nbad > 0) return 1;
+    // (a prebinning of a mesh the gather cannot read as a table)
+    if (plan->bin_table && !m2p3d_table_applicable(m) && materialize_heads(plan, s)) return PPMG_ERROR_FATAL;
     mark(plan, s);
     if (np <= 0) return 0;
 #define TYPED(KS)                                                                                              \

@@ -440,12 +440,11 @@ bool p2m3d_fast_applicable(const DevMesh& m, int kernel, int prec, const FieldGe
     return (int64_t)fg.ext[0] * fg.ext[1] * fg.lda * vsize < ((int64_t)1 << 31);
 }
 
-// The deposit reads a slot-table binning: the compact layout of a periodic
-// mesh (no stored-box shell to fill from the heads) with the default tile.
+// The deposit (and the stored box's ghost shell) reads a slot-table binning
+// on a periodic mesh, in either layout, with the default tile.
 bool p2m3d_table_applicable(const DevMesh& m, int kernel, int prec, const FieldGeom& fg) {
     if (!p2m3d_fast_applicable(m, kernel, prec, fg)) return false;
-    for (int d = 0; d < 3; ++d)
-        if (!(m.periodic[d] && fg.lo[d] == 0 && fg.ext[d] == m.N[d])) return false;
+    if (!(m.periodic[0] && m.periodic[1] && m.periodic[2])) return false;
     return kernel == PPMG_KERNEL_BSP2 || p3_rows() == P3_TY;
 }
 

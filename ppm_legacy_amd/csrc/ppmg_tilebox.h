@@ -62,12 +62,13 @@ __global__ void __launch_bounds__(256) k_box_dup(FieldGeom fg, int3 n, int3 r, i
 // (y, z real).  Weights and products in the value precision, sums in FP64;
 // with weight tables (wt.w[0] set) the weights are the tables'.  KS: stencil
 // width (4 M'4, 2 BSP2); DIM 2: one plane (no z stencil), positions of two
-// coordinates.
-template <int DIM, int KS, typename PosT, typename ValT, int LL>
+// coordinates.  TABLE: the binning is a slot table (3D only) instead of
+// heads and a permutation; a cell's particles come in the same order.
+template <int DIM, int KS, typename PosT, typename ValT, int LL, bool TABLE>
 __global__ void __launch_bounds__(256) k_box_shell(DevMesh m, FieldGeom fg, int3 r, const int* __restrict__ start,
-                                                  const int* __restrict__ perm, const PosT* __restrict__ xp,
-                                                  const ValT* __restrict__ up, int c0, ValT* __restrict__ field,
-                                                  int64_t nshell, WTab<ValT> wt) {
+                                                  const int* __restrict__ perm, BinTable bt,
+                                                  const PosT* __restrict__ xp, const ValT* __restrict__ up, int c0,
+                                                  ValT* __restrict__ field, int64_t nshell, WTab<ValT> wt) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nshell) return;
     const int e0 = fg.ext[0], e1 = fg.ext[1], e2 = fg.ext[2];
@@ -106,9 +107,10 @@ __global__ void __launch_bounds__(256) k_box_shell(DevMesh m, FieldGeom fg, int3
                 const int cx = n[0] + B - ox;
                 if (cx < 0 || cx >= m.nbin[0]) continue;
                 const int key = cx + m.nbin[0] * (cy + m.nbin[1] * cz);
-                const int b = start[key], e = start[key + 1];
+                const int b = TABLE ? 0 : start[key];
+                const int e = TABLE ? (int)((bt.count[count_word(key)] >> count_shift(key)) & 255u) : start[key + 1];
                 for (int q = b; q < e; ++q) {
-                    const int p = perm ? perm[q] : q;
+                    const int p = TABLE ? bt.table[(int64_t)q * bt.ncells + key] : perm ? perm[q] : q;
                     ValT w;
                     if (DIM == 3 && KS == 4 && wt.w[0]) {
                         w = (wt.w[0][(int64_t)p * 4 + ox] * wt.w[1][(int64_t)p * 4 + oy]) * wt.w[2][(int64_t)p * 4 + oz];
@@ -155,14 +157,28 @@ int fill_stored_box(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const Val
     const int64_t nshell = (int64_t)fg.ext[0] * fg.ext[1] * fg.ext[2] - (int64_t)R[0] * R[1] * R[2];
     if (nshell > 0) {
         const unsigned g = (unsigned)((nshell + 255) / 256);
+        const BinTable bt{plan->count.as<uint32_t>(), plan->table.as<int>(), plan->ncells};
+        // a slot table is binned for 3D deposits only
+        const bool table = DIM == 3 && plan->bin_table;
+        if (!table && materialize_heads(plan, s)) return 1;
         for (int c0 = 0; c0 < lda; c0 += 4) {
             const int L = lda - c0 < 4 ? lda - c0 : 4;
-#define SH(LL) hipLaunchKernelGGL((k_box_shell<DIM, KS, PosT, ValT, LL>), dim3(g), dim3(256), 0, s, m, fg, r3, \
-                                  plan->start.as<int>(), tile_perm(plan), xp, up, c0, field, nshell, wt)
-            if (L == 1) SH(1);
-            else if (L == 2) SH(2);
-            else if (L == 3) SH(3);
-            else SH(4);
+#define SH(LL, TB) hipLaunchKernelGGL((k_box_shell<DIM, KS, PosT, ValT, LL, TB>), dim3(g), dim3(256), 0, s, m, fg, r3, \
+                                      plan->start.as<int>(), tile_perm(plan), bt, xp, up, c0, field, nshell, wt)
+#define SHL(LL)                               \
+    do {                                      \
+        if constexpr (DIM == 3) {             \
+            if (table) SH(LL, true);          \
+            else SH(LL, false);               \
+        } else {                              \
+            SH(LL, false);                    \
+        }                                     \
+    } while (0)
+            if (L == 1) SHL(1);
+            else if (L == 2) SHL(2);
+            else if (L == 3) SHL(3);
+            else SHL(4);
+#undef SHL
 #undef SH
         }
     }

@@ -171,10 +171,13 @@ def test_binning_direct_heads(core, oracle, case):
 
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
 @pytest.mark.parametrize("prec", [F64, F32C])
-def test_binning_slot_table(core, oracle, kernel, prec, monkeypatch):
-    """Unsorted input for the tiled 3D deposit on the compact layout is binned
-    into a slot table (up to eight particles per cell, in increasing index)
-    instead of heads and a permutation.  One plan is fed, in shuffled order:
+@pytest.mark.parametrize("lo", [0, -2])
+def test_binning_slot_table(core, oracle, kernel, prec, lo, monkeypatch):
+    """Unsorted input for the tiled 3D deposit on a periodic mesh (compact
+    layout, lo = 0, or the reference's stored box 1-g..Nm+g, lo = -2, whose
+    ghost shell and adjoint read the table too) is binned into a slot table
+    (up to eight particles per cell, in increasing index) instead of heads
+    and a permutation.  One plan is fed, in shuffled order:
     an advected lattice (table); the same with 40 particles moved into one
     cell (the table overflows, the binning runs again with keys and slots, and
     the plan bins its next 16 inputs without a table); the lattice for those
@@ -184,8 +187,9 @@ def test_binning_slot_table(core, oracle, kernel, prec, monkeypatch):
     the adjoint (heads built from the table) must equal the heads binning's
     bitwise; sort_view must give numpy's stable argsort, keys and heads."""
     N, lda = 24, 3
-    rng = np.random.default_rng(31 + kernel)
+    rng = np.random.default_rng(31 + kernel - lo)
     lattice = wl.advect(wl.jittered_lattice(N, seed=31), wl.fourier_modes(31), N)[0]
+    g = [-lo] * 3
     crowded = lattice.copy()
     crowded[:40] = (np.array([5, 6, 7]) + rng.uniform(0.1, 0.9, size=(40, 3))) / N
     t = oracle.topo(3, [N + 1] * 3)
@@ -198,22 +202,27 @@ def test_binning_slot_table(core, oracle, kernel, prec, monkeypatch):
         xp = base[o]
         up = wl.strengths(i, len(xp), lda)
         x, u = _dev(xp, _pos_dtype(prec)), _dev(up, _val_dtype(prec))
-        f = torch.empty(mesh.compact_shape(lda), dtype=_val_dtype(prec), device="cuda")
-        core.p2m(mesh, x, u, f, [0] * 3, kernel, prec, plan)
+        shape = mesh.compact_shape(lda) if lo == 0 else mesh.padded_shape(lda, g)
+        f = torch.empty(shape, dtype=_val_dtype(prec), device="cuda")
+        core.p2m(mesh, x, u, f, [lo] * 3, kernel, prec, plan)
         monkeypatch.setenv("PPMG_NO_TABLE", "1")
         href = torch.empty_like(f)
         hplan = core.Plan()
-        core.p2m(mesh, x, u, href, [0] * 3, kernel, prec, hplan)
+        core.p2m(mesh, x, u, href, [lo] * 3, kernel, prec, hplan)
         monkeypatch.delenv("PPMG_NO_TABLE")
         assert torch.equal(f, href), (case, i)
         if not check:
             continue
-        info, ref = oracle.p2m(t, kernel, xp, up, [1, 1, 1])
+        if lo == 0:
+            info, ref = oracle.p2m(t, kernel, xp, up, [1, 1, 1])
+            ref = oracle.unique_nodes(ref, [1, 1, 1], 3)
+        else:
+            info, ref = oracle.p2m(t, kernel, xp, up, g)   # every stored node, ghosts raw
         assert info == 0
-        assert _rel(f.cpu().numpy(), oracle.unique_nodes(ref, [1, 1, 1], 3)) < TOL[prec], (case, i)
-        g = _dev(rng.normal(size=f.shape), _val_dtype(prec))
-        gu, gx = core.p2m_adjoint(mesh, x, u, g, [0] * 3, plan, kernel, prec)
-        hu, hx = core.p2m_adjoint(mesh, x, u, g, [0] * 3, hplan, kernel, prec)
+        assert _rel(f.cpu().numpy(), ref) < TOL[prec], (case, i)
+        cot = _dev(rng.normal(size=f.shape), _val_dtype(prec))
+        gu, gx = core.p2m_adjoint(mesh, x, u, cot, [lo] * 3, plan, kernel, prec)
+        hu, hx = core.p2m_adjoint(mesh, x, u, cot, [lo] * 3, hplan, kernel, prec)
         assert torch.equal(gu, hu) and torch.equal(gx, hx), (case, i)
         cells = (xp * N).astype(np.int64)
         key = cells[:, 0] + N * (cells[:, 1] + N * cells[:, 2])
@@ -222,6 +231,48 @@ def test_binning_slot_table(core, oracle, kernel, prec, monkeypatch):
         assert np.array_equal(perm.cpu().numpy(), np.argsort(key, kind="stable")), (case, i)
         assert np.array_equal(start.cpu().numpy(),
                               np.concatenate([[0], np.cumsum(np.bincount(key, minlength=N ** 3))])), (case, i)
+
+
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+@pytest.mark.parametrize("lo", [0, -2])
+def test_m2p_slot_table(core, oracle, kernel, lo, monkeypatch):
+    """The tiled 3D gather on a periodic mesh reads a slot-table binning of
+    unsorted particles (any field layout), directly or left by
+    ppmg_m2p_prebin.  One plan gathers at an advected lattice (table), at the
+    same with 40 particles in one cell (the table overflows: heads), and at
+    the lattice again after a prebinning; every result equals bitwise the
+    gather of the same particles binned with heads (PPMG_NO_TABLE=1), since
+    each particle's sum does not depend on the binning, and the oracle's
+    within 1e-12."""
+    N, lda = 24, 3
+    rng = np.random.default_rng(61 + kernel - lo)
+    lattice = wl.advect(wl.jittered_lattice(N, seed=61), wl.fourier_modes(61), N)[0]
+    crowded = lattice.copy()
+    crowded[:40] = (np.array([3, 9, 17]) + rng.uniform(0.1, 0.9, size=(40, 3))) / N
+    uniq = rng.normal(size=(N, N, N, lda))
+    mesh = core.Mesh(3, N + 1)
+    if lo == 0:
+        field = _dev(uniq, torch.float64)
+    else:
+        g = -lo
+        idx = np.arange(-g, N + 1 + g) % N
+        field = _dev(uniq[np.ix_(idx, idx, idx)], torch.float64)
+    plan = core.Plan()
+    for i, (case, prebin) in enumerate([("lattice", False), ("crowded", False), ("lattice", True)]):
+        base = lattice if case == "lattice" else crowded
+        xp = base[rng.permutation(len(base))]
+        x = _dev(xp, torch.float64)
+        out = torch.empty((len(xp), lda), dtype=torch.float64, device="cuda")
+        if prebin:
+            core.m2p_prebin(mesh, x, plan, kernel, F64)
+        core.m2p(mesh, x, field, [lo] * 3, out, kernel, F64, plan)
+        monkeypatch.setenv("PPMG_NO_TABLE", "1")
+        ref = torch.empty_like(out)
+        core.m2p(mesh, x, field, [lo] * 3, ref, kernel, F64)
+        monkeypatch.delenv("PPMG_NO_TABLE")
+        assert torch.equal(out, ref), (case, i)
+        oref = bf.m2p_periodic(xp, uniq, N, kernel)
+        assert _rel(out.cpu().numpy(), oref) < 1e-12, (case, i)
 
 
 @pytest.mark.parametrize("dim,N", [(3, 32), (3, 300), (2, 96)])
