@@ -170,7 +170,7 @@ def test_binning_direct_heads(core, oracle, case):
 
 
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
-@pytest.mark.parametrize("prec", [F64, F32C])
+@pytest.mark.parametrize("prec", [F64, F32C, F32])
 @pytest.mark.parametrize("lo", [0, -2])
 def test_binning_slot_table(core, oracle, kernel, prec, lo, monkeypatch):
     """Unsorted input for the tiled 3D deposit on a periodic mesh (compact
@@ -224,7 +224,8 @@ def test_binning_slot_table(core, oracle, kernel, prec, lo, monkeypatch):
         gu, gx = core.p2m_adjoint(mesh, x, u, cot, [lo] * 3, plan, kernel, prec)
         hu, hx = core.p2m_adjoint(mesh, x, u, cot, [lo] * 3, hplan, kernel, prec)
         assert torch.equal(gu, hu) and torch.equal(gx, hx), (case, i)
-        cells = (xp * N).astype(np.int64)
+        # keys from the positions in the form's precision
+        cells = ((xp.astype(np.float32) * np.float32(N)) if prec == F32 else xp * N).astype(np.int64)
         key = cells[:, 0] + N * (cells[:, 1] + N * cells[:, 2])
         k, perm, start = plan.sort_view()
         assert np.array_equal(k.cpu().numpy(), key), (case, i)
@@ -275,8 +276,8 @@ def test_m2p_slot_table(core, oracle, kernel, lo, monkeypatch):
         assert _rel(out.cpu().numpy(), oref) < 1e-12, (case, i)
 
 
-@pytest.mark.parametrize("dim,N", [(3, 32), (3, 300), (2, 96)])
-def test_binning_crowded_radix(core, oracle, dim, N, monkeypatch):
+@pytest.mark.parametrize("dim,N,prec", [(3, 32, F64), (3, 300, F64), (2, 96, F64), (3, 32, F32), (2, 96, F32)])
+def test_binning_crowded_radix(core, oracle, dim, N, prec, monkeypatch):
     """Crowded input is binned by a stable radix sort of (cell key, particle)
     pairs (8 bits a pass: two passes at 32^3 and 96^2 cells, four at 300^3),
     with the heads from the sorted runs; a plan takes that path after a
@@ -288,11 +289,15 @@ def test_binning_crowded_radix(core, oracle, dim, N, monkeypatch):
     cell-order path).  Every call: keys, permutation and heads equal numpy's
     stable argsort and bincount, and the field equals bitwise that of a
     fresh plan under PPMG_RADIX=0 (counting path), whose per-node addition
-    order is the same; at 32^3 and 96^2 also the oracle's field."""
-    rng = np.random.default_rng(900 + N)
+    order is the same; at 32^3 and 96^2 also the oracle's field.  FP32:
+    positions and strengths in single precision (keys from the single
+    precision product)."""
+    rng = np.random.default_rng(900 + N + prec)
     nb, nu = (6000, 3000) if N < 200 else (60000, 30000)
     base = np.concatenate([rng.normal(0.4, 0.6 / N, size=(nb, dim)) % 1.0, rng.uniform(0, 1, size=(nu, dim))])
-    cells = np.minimum((base * N).astype(np.int64), N - 1)
+    odt = np.float32 if prec == F32 else np.float64
+    base = base.astype(odt)
+    cells = np.minimum((base * odt(N)).astype(np.int64), N - 1)
     key = cells[:, 0] + N * cells[:, 1] + (N * N * cells[:, 2] if dim == 3 else 0)
     ncells = N ** dim
     assert np.bincount(key).max() > 50
@@ -302,12 +307,12 @@ def test_binning_crowded_radix(core, oracle, dim, N, monkeypatch):
     for call, order in enumerate(["shuffled", "shuffled", "shuffled", "cell_sorted", "cell_sorted"]):
         o = np.argsort(key, kind="stable") if order == "cell_sorted" else rng.permutation(len(base))
         xp, k = base[o], key[o]
-        up = rng.normal(size=(len(xp), 3))
-        x, u = _dev(xp, torch.float64), _dev(up, torch.float64)
-        f = torch.empty(mesh.compact_shape(3), dtype=torch.float64, device="cuda")
+        up = rng.normal(size=(len(xp), 3)).astype(odt)
+        x, u = _dev(xp, _pos_dtype(prec)), _dev(up, _val_dtype(prec))
+        f = torch.empty(mesh.compact_shape(3), dtype=_val_dtype(prec), device="cuda")
         if call in (1, 2, 3):
             monkeypatch.setenv("PPMG_RADIX", "1")
-        core.p2m(mesh, x, u, f, [0] * dim, MP4, F64, plan)
+        core.p2m(mesh, x, u, f, [0] * dim, MP4, prec, plan)
         monkeypatch.setenv("PPMG_RADIX", "0")
         kk, perm, start = plan.sort_view()
         assert np.array_equal(kk.cpu().numpy(), k), (order, call)
@@ -315,13 +320,13 @@ def test_binning_crowded_radix(core, oracle, dim, N, monkeypatch):
         assert np.array_equal(start.cpu().numpy(),
                               np.concatenate([[0], np.cumsum(np.bincount(k, minlength=ncells))])), (order, call)
         ref = torch.empty_like(f)
-        core.p2m(mesh, x, u, ref, [0] * dim, MP4, F64)
+        core.p2m(mesh, x, u, ref, [0] * dim, MP4, prec)
         monkeypatch.delenv("PPMG_RADIX")
         assert torch.equal(f, ref), (order, call)
         if t is not None:
-            info, oref = oracle.p2m(t, MP4, xp, up, [1] * dim)
+            info, oref = oracle.p2m(t, MP4, xp, up, [1] * dim, dtype=odt)
             assert info == 0
-            assert _rel(f.cpu().numpy(), oracle.unique_nodes(oref, [1] * dim, dim)) < 1e-12, (order, call)
+            assert _rel(f.cpu().numpy(), oracle.unique_nodes(oref, [1] * dim, dim)) < TOL[prec], (order, call)
 
 
 # ----------------------------------------------------------- variants --
