@@ -170,21 +170,6 @@ static int make_field(const ppmg_field* in, const DevMesh& m, FieldGeom* f) {
 
 static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
-// Two descriptors that bin and weigh particles identically (bitwise equal
-// origin and inverse spacing).
-static bool same_mesh(const DevMesh& a, const DevMesh& b) {
-    if (a.dim != b.dim) return false;
-    for (int d = 0; d < 3; ++d) {
-        if (a.N[d] != b.N[d] || a.nbin[d] != b.nbin[d] || a.periodic[d] != b.periodic[d] ||
-            a.upper_open[d] != b.upper_open[d])
-            return false;
-        if (std::memcmp(&a.minv[d], &b.minv[d], sizeof(double)) != 0 ||
-            std::memcmp(&a.maxv[d], &b.maxv[d], sizeof(double)) != 0 ||
-            std::memcmp(&a.dxi[d], &b.dxi[d], sizeof(double)) != 0)
-            return false;
-    }
-    return true;
-}
 
 static int check_plan(ppmg_plan* plan) {
     if (!plan) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "plan is NULL");
@@ -425,7 +410,9 @@ int ppmg_m2p_prebin(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, con
     if (m.dim != 3) return 0;
     int64_t nbad = 0;
     plan->have_forward = false;
-    if (bin_particles(plan, m, prec, np, xp, as_stream(stream), &nbad, m2p3d_table_applicable(m)))
+    // the tiled gather checks a lattice taken without a pass (m2p3d_fast)
+    if (bin_particles(plan, m, prec, np, xp, as_stream(stream), &nbad, m2p3d_table_applicable(m),
+                      m2p3d_table_applicable(m)))
         return g_status.info;
     plan->prebinned = true;
     plan->prebin_xp = xp;

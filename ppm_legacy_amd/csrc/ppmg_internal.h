@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <cstring>
 #include <string>
 
 #include "ppmg.h"
@@ -376,6 +377,24 @@ bool arena_bound();          // an arena is bound on the calling thread
 
 }  // namespace ppmg
 
+namespace ppmg {
+// Two descriptors that bin and weigh particles identically (bitwise equal
+// origin and inverse spacing).
+inline bool same_mesh(const DevMesh& a, const DevMesh& b) {
+    if (a.dim != b.dim) return false;
+    for (int d = 0; d < 3; ++d) {
+        if (a.N[d] != b.N[d] || a.nbin[d] != b.nbin[d] || a.periodic[d] != b.periodic[d] ||
+            a.upper_open[d] != b.upper_open[d])
+            return false;
+        if (std::memcmp(&a.minv[d], &b.minv[d], sizeof(double)) != 0 ||
+            std::memcmp(&a.maxv[d], &b.maxv[d], sizeof(double)) != 0 ||
+            std::memcmp(&a.dxi[d], &b.dxi[d], sizeof(double)) != 0)
+            return false;
+    }
+    return true;
+}
+}  // namespace ppmg
+
 // Opaque plan: scratch of the binning and the forward record for the adjoint.
 struct ppmg_plan {
     ppmg::DevBuf key, slot, count, start, perm, wsorted, usorted, scan_tmp, err, misc, misc2, gaps, multi,
@@ -391,6 +410,15 @@ struct ppmg_plan {
     // builds those two from it.
     bool bin_table = false;
     int table_skip = 0;          // binnings to run without the table after one overflowed it
+    // The last binning found a lattice in cell order (particle p in cell p,
+    // np == ncells) on lattice_mesh in precision lattice_prec; bin_lattice:
+    // the current binning is that lattice, taken without a pass and not yet
+    // checked (only a reader that checks every particle's cell accepts it).
+    bool lattice_pred = false;
+    int lattice_prec = 0;
+    ppmg::DevMesh lattice_mesh{};
+    bool bin_lattice = false;
+    bool lattice_checked = false;   // ... and a gather has checked it
     bool perm_identity = false;  // the permutation of the last binning is the identity ...
     bool perm_written = true;    // ... and has (or has not yet) been written into `perm`
     hipStream_t bin_stream = nullptr;   // stream of the last binning

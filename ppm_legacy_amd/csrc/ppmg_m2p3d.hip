@@ -66,11 +66,17 @@ __device__ __forceinline__ int raw_node(const DevMesh& m, const FieldGeom& fg, i
 // node takes the periodic node map depends on all three of its coordinates.
 // TABLE: the binning is a slot table (byte counts, particle r of cell c at
 // bt.table[r * ncells + c]) instead of heads and a permutation.
-template <int KS, int LDA, typename PosT, typename ValT, typename AccT, int WPE, bool OPEN, bool MIX, bool TABLE>
+// LAT: the binning is taken to be a lattice in cell order (particle c in cell
+// c, one per cell) without having been computed: every particle's cell key
+// is checked against its cell, and a mismatch sets *lat_err (the caller then
+// bins and gathers again).
+template <int KS, int LDA, typename PosT, typename ValT, typename AccT, int WPE, bool OPEN, bool MIX, bool TABLE,
+          bool LAT>
 __global__ void __launch_bounds__(M3_THREADS) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, BinTable bt,
           const PosT* __restrict__ xp, const ValT* __restrict__ field, FieldGeom fg, ValT* __restrict__ up, int lda,
-          int c0, int tz) {
+          int c0, int tz, int* __restrict__ lat_err) {
+    static_assert(!(TABLE && LAT), "one binning form");
     constexpr int B = M3Geom<KS>::B, M3_TX = M3Geom<KS>::TX, M3_PL = M3Geom<KS>::PL;
     constexpr int RP = KS;                   // ring planes
     __shared__ ValT ring[RP * LDA * M3_PL];  // [plane][c][ry][nx]
@@ -166,7 +172,10 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         t = 0;
         if (plane_lane && z < z1) {
             const int key = (int)((int64_t)z * plane) + key0;
-            if constexpr (TABLE) {
+            if constexpr (LAT) {
+                f = key;
+                n = 1;
+            } else if constexpr (TABLE) {
                 f = key;
                 n = (int)bt.count[count_word(key)];   // decoded where used: the load stays in flight
                 t = bt.table[key];
@@ -177,16 +186,19 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         }
     };
     auto count_of = [&](int f, int n) {
-        if constexpr (TABLE) return (int)(((unsigned)n >> count_shift(f)) & 255u);
+        if constexpr (LAT) return n;
+        else if constexpr (TABLE) return (int)(((unsigned)n >> count_shift(f)) & 255u);
         else return n - f;
     };
     // the cell's first particle, and its particle r < count
     auto lookup = [&](int f, int n, int t) {
-        if constexpr (TABLE) return count_of(f, n) > 0 ? t : 0;
+        if constexpr (LAT) return f;
+        else if constexpr (TABLE) return count_of(f, n) > 0 ? t : 0;
         else return n > f ? (perm ? perm[f] : f) : 0;
     };
     auto entry = [&](int f, int r) {
-        if constexpr (TABLE) return bt.table[(int64_t)r * bt.ncells + f];
+        if constexpr (LAT) return f + r;
+        else if constexpr (TABLE) return bt.table[(int64_t)r * bt.ncells + f];
         else return perm ? perm[f + r] : f + r;
     };
     auto position = [&](int p, int n, PosT (&x)[3]) {
@@ -225,11 +237,27 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             AccT wx[KS], wy[KS], wz[KS];
 #pragma unroll
             for (int o = 0; o < KS; ++o) wx[o] = wy[o] = wz[o] = AccT(0);
+            bool lat_off = false;
             if (has) {
-                int ci;
-                tile_weights<KS, AccT>((AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci), wx);
-                tile_weights<KS, AccT>((AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci), wy);
-                tile_weights<KS, AccT>((AccT)cell_offset<PosT>(x[2], (PosT)m.minv[2], (PosT)m.dxi[2], ci), wz);
+                int ci[3];
+                tile_weights<KS, AccT>((AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci[0]), wx);
+                tile_weights<KS, AccT>((AccT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci[1]), wy);
+                tile_weights<KS, AccT>((AccT)cell_offset<PosT>(x[2], (PosT)m.minv[2], (PosT)m.dxi[2], ci[2]), wz);
+                if constexpr (LAT) {
+                    // The particle taken for this lane's cell must lie in it: the
+                    // cell key's test on a periodic mesh (inside [min, max], INT
+                    // truncation, index N wrapping to 0), from the cell indices
+                    // the weights took.  No branch here: the flag is stored with
+                    // the results (a store amid the weights kept the ring reads
+                    // from being scheduled beside them).
+                    const int cell[3] = {cx, cy, cz};
+                    bool in = true;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        in = in && x[d] >= (PosT)m.minv[d] && x[d] <= (PosT)m.maxv[d] &&
+                             (ci[d] == cell[d] || (ci[d] == m.N[d] && cell[d] == 0));
+                    lat_off = !in;
+                }
             }
             // (y,z) weights of the particle whose stencil takes this lane's column
             // as x offset k: that of lane ex + B - k (M'4: ex+1, ex, ex-1, ex-2)
@@ -281,6 +309,7 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             if (has) {
 #pragma unroll
                 for (int c = 0; c < LDA; ++c) up[(int64_t)p * lda + c0 + c] = res[c];
+                if (LAT && lat_off) *lat_err = 1;
             }
         };
         if (__ballot(cnt > 0) != 0ull) round(pn, cnt > 0, xn);
@@ -339,12 +368,13 @@ bool m2p3d_table_applicable(const DevMesh& m) { return m.dim == 3 && m.periodic[
 // One launch of the gather for components c0 .. c0+LL-1; the z chunk comes
 // from the resident workgroup count (pick_z_chunk_balanced: each chunk loads
 // 3 extra node planes, cheap beside the balance short chunks give).
-template <int KS, int LL, typename PosT, typename ValT, typename AccT, bool OPEN, bool MIX, bool TABLE = false>
+template <int KS, int LL, typename PosT, typename ValT, typename AccT, bool OPEN, bool MIX, bool TABLE = false,
+          bool LAT = false>
 static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
                            const ValT* field, ValT* up, hipStream_t s) {
     constexpr int TX = M3Geom<KS>::TX;
     constexpr int WPE = KS == 4 && (LL == 4 || MIX) ? 3 : 4;
-    auto kernel = k_m3_tile<KS, LL, PosT, ValT, AccT, WPE, OPEN, MIX, TABLE>;
+    auto kernel = k_m3_tile<KS, LL, PosT, ValT, AccT, WPE, OPEN, MIX, TABLE, LAT>;
     static std::atomic<int> slots{0};
     if (slots.load() == 0) slots.store(resident_workgroups(kernel, M3_THREADS));
     const int tx = (m.nbin[0] + TX - 1) / TX, ty = (m.nbin[1] + M3_TY - 1) / M3_TY;
@@ -352,7 +382,7 @@ static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, co
     dim3 grid((unsigned)((int64_t)tx * ty * ((m.nbin[2] + tz - 1) / tz)));
     const BinTable bt{plan->count.as<uint32_t>(), plan->table.as<int>(), plan->ncells};
     hipLaunchKernelGGL(kernel, grid, dim3(M3_THREADS), 0, s, m, plan->start.as<int>(),
-                       tile_perm(plan), bt, xp, field, fg, up, fg.lda, c0, tz);
+                       tile_perm(plan), bt, xp, field, fg, up, fg.lda, c0, tz, plan->err.as<int>() + 6);
 }
 
 // OPEN: some axis is open (a fully periodic mesh compiles without the clamp);
@@ -360,8 +390,10 @@ static void launch_m3_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, co
 template <int KS, int LL, typename PosT, typename ValT, typename AccT>
 static void launch_m3(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
                       const ValT* field, ValT* up, hipStream_t s) {
-    // a slot table is binned on periodic meshes only (m2p3d_fast)
-    if (plan->bin_table)
+    // a slot table or an unchecked lattice is binned on periodic meshes only (m2p3d_fast)
+    if (plan->bin_lattice)
+        launch_m3_open<KS, LL, PosT, ValT, AccT, false, false, false, true>(plan, m, xp, fg, c0, field, up, s);
+    else if (plan->bin_table)
         launch_m3_open<KS, LL, PosT, ValT, AccT, false, false, true>(plan, m, xp, fg, c0, field, up, s);
     else if (m.periodic[0] && m.periodic[1] && m.periodic[2])
         launch_m3_open<KS, LL, PosT, ValT, AccT, false, false>(plan, m, xp, fg, c0, field, up, s);
@@ -397,15 +429,23 @@ static int m2p3d_typed(ppmg_plan* plan, const DevMesh& m, int64_t np, const PosT
 // issues at the FP32 rate, so the FP32-compensated form keeps FP32 only where
 // it saves bytes (field in HBM and LDS, output) and sums its 64 products in
 // FP64 registers -- more accurate than compensated FP32 at no extra cost.
+//
+// A plan whose last binning found a lattice in cell order (particle p in
+// cell p, one per cell: a fresh lattice of remeshed particles) takes the
+// next one on the same mesh as that lattice without a binning pass: the
+// gather checks every particle's cell key and, should one differ, the call
+// bins and gathers again.
 int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp,
                const FieldGeom& fg, const void* field, void* up, bool prebinned, hipStream_t s) {
+    const bool periodic = m2p3d_table_applicable(m);
     int64_t nbad = prebinned ? plan->prebin_nbad : 0;
-    if (!prebinned && bin_particles(plan, m, prec, np, xp, s, &nbad, m2p3d_table_applicable(m))) return PPMG_ERROR_FATAL;
+    if (!prebinned && bin_particles(plan, m, prec, np, xp, s, &nbad, periodic, periodic)) return PPMG_ERROR_FATAL;
     if (nbad > 0) return 1;
-    // (a prebinning of a mesh the gather cannot read as a table)
-    if (plan->bin_table && !m2p3d_table_applicable(m) && materialize_heads(plan, s)) return PPMG_ERROR_FATAL;
+    // (a prebinning of a mesh the gather cannot read as a table or check as a lattice)
+    if ((plan->bin_table || plan->bin_lattice) && !periodic && materialize_heads(plan, s)) return PPMG_ERROR_FATAL;
     mark(plan, s);
     if (np <= 0) return 0;
+    auto gather = [&]() -> int {
 #define TYPED(KS)                                                                                              \
     do {                                                                                                       \
         if (prec == PPMG_PREC_F64)                                                                             \
@@ -417,9 +457,26 @@ int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t 
         return m2p3d_typed<KS, float, float, double>(plan, m, np, (const float*)xp, fg, (const float*)field,   \
                                                      (float*)up, s);                                           \
     } while (0)
-    if (kernel == PPMG_KERNEL_BSP2) TYPED(2);
-    TYPED(4);
+        if (kernel == PPMG_KERNEL_BSP2) TYPED(2);
+        TYPED(4);
 #undef TYPED
+    };
+    if (!plan->bin_lattice) return gather();
+    int* lat_err = plan->err.as<int>() + 6;
+    if (fill_zero(lat_err, sizeof(int), s)) return PPMG_ERROR_FATAL;
+    if (gather()) return PPMG_ERROR_FATAL;
+    int off = 0;
+    PPMG_HIP(hipMemcpyAsync(&off, lat_err, sizeof(int), hipMemcpyDeviceToHost, s));
+    PPMG_HIP(hipStreamSynchronize(s));
+    if (!off) {
+        plan->lattice_checked = true;   // the plan's binning is the lattice (heads written on demand)
+        return 0;
+    }
+    // not that lattice: bin and gather again
+    plan->lattice_pred = false;
+    if (bin_particles(plan, m, prec, np, xp, s, &nbad, periodic, false)) return PPMG_ERROR_FATAL;
+    if (nbad > 0) return 1;
+    return gather();
 }
 
 }  // namespace ppmg

@@ -37,13 +37,38 @@ struct BinTable {
     int64_t ncells;
 };
 
+// Cell key of one particle with the index arithmetic of ppm_interp_p2m.f:
+// 800-822 (x0 = (x - min)*dxi, INT truncation) and the sub test of :535-561;
+// -1 when the particle lies in no cell of the sub.
+template <int DIM, typename PosT>
+__device__ __forceinline__ int cell_key(const PosT (&x)[DIM], const DevMesh& m) {
+    bool ok = true;
+    int cell[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        PosT v = x[d];
+        PosT mn = (PosT)m.minv[d];
+        PosT mx = (PosT)m.maxv[d];
+        if (!(v >= mn && v <= mx)) ok = false;
+        if (!(v < mx || !m.upper_open[d])) ok = false;
+        PosT x0 = scaled_coord<PosT>(v, mn, (PosT)m.dxi[d]);
+        int i = ok ? (int)x0 : 0;
+        if (m.periodic[d] && i >= m.N[d]) i -= m.N[d];
+        if (i < 0 || i >= m.nbin[d]) ok = false;
+        cell[d] = i;
+    }
+    return ok ? cell[0] + m.nbin[0] * (cell[1] + m.nbin[1] * cell[2]) : -1;
+}
+
 int exclusive_scan(const int* in, int64_t n, int* out, DevBuf& tmp, int64_t total, hipStream_t s);
 int fill_zero(void* p, size_t bytes, hipStream_t s);
 // Bins np particles into the plan (key, perm, start); *nbad = particles outside the sub.
 // table: the caller reads a slot table (plan->bin_table) where the binning
-// can leave one instead of the heads and the permutation.
+// can leave one instead of the heads and the permutation.  lattice: the
+// caller checks every particle's cell (err[6]), so a plan that predicts a
+// lattice in cell order may take it without a pass (plan->bin_lattice).
 int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp,
-                  hipStream_t s, int64_t* nbad, bool table = false);
+                  hipStream_t s, int64_t* nbad, bool table = false, bool lattice = false);
 // Heads and permutation of a binning that left a slot table (no-op otherwise).
 int materialize_heads(ppmg_plan* plan, hipStream_t s);
 // The binning leaves the identity permutation implicit when the input is

@@ -16,8 +16,9 @@ namespace ppmg {
 
 // Binning status words: err[0] particles outside the sub, err[1] a descent
 // seen, err[2] queued runs of empty cells, err[3] a byte count overflowed,
-// err[4] a cell with more particles than the slot table holds, err[8..71]
-// sampled crowded cells.
+// err[4] a cell with more particles than the slot table holds, err[5] a
+// particle p not in cell p, err[6] (the gather's check of a lattice
+// binning) the same, err[8..71] sampled crowded cells.
 constexpr int ERR_WORDS = 72;
 
 constexpr int SCAN_BLOCK = 256;
@@ -258,29 +259,6 @@ static int exclusive_scan_u8(const uint32_t* in, int64_t n, int* out, DevBuf& tm
 }
 
 // ------------------------------------------------------------ binning ---
-// Cell key of one particle with the index arithmetic of ppm_interp_p2m.f:
-// 800-822 (x0 = (x - min)*dxi, INT truncation) and the sub test of :535-561;
-// -1 when the particle lies in no cell of the sub.
-template <int DIM, typename PosT>
-__device__ __forceinline__ int cell_key(const PosT (&x)[DIM], const DevMesh& m) {
-    bool ok = true;
-    int cell[3] = {0, 0, 0};
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        PosT v = x[d];
-        PosT mn = (PosT)m.minv[d];
-        PosT mx = (PosT)m.maxv[d];
-        if (!(v >= mn && v <= mx)) ok = false;
-        if (!(v < mx || !m.upper_open[d])) ok = false;
-        PosT x0 = scaled_coord<PosT>(v, mn, (PosT)m.dxi[d]);
-        int i = ok ? (int)x0 : 0;
-        if (m.periodic[d] && i >= m.N[d]) i -= m.N[d];
-        if (i < 0 || i >= m.nbin[d]) ok = false;
-        cell[d] = i;
-    }
-    return ok ? cell[0] + m.nbin[0] * (cell[1] + m.nbin[1] * cell[2]) : -1;
-}
-
 // Heads of the cells lo..hi (all empty but possibly the last) := v.  Short
 // runs are written by the lane itself; longer ones are queued for
 // k_fill_gaps in pieces of GAP_CHUNK cells (err[2] counts the entries; an
@@ -355,7 +333,7 @@ __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64
     for (int d = 0; d < DIM; ++d) sv[d] = has_seam ? xp[(w0 - 1) * DIM + d] : PosT(0);
     int k[KEYS_PPT];
     int nbad = 0;
-    uint64_t anydesc = 0;
+    uint64_t anydesc = 0, notlat = 0;
     bool desc[KEYS_PPT];
     int prev = has_seam ? cell_key<DIM, PosT>(sv, m) : -1;   // key before run j's lane 0
 #pragma unroll
@@ -369,6 +347,7 @@ __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64
         desc[j] = live && p > 0 && k[j] < left;
         nbad += (int)__popcll(__ballot(live && k[j] < 0));
         anydesc |= __ballot(desc[j]);
+        notlat |= __ballot(live && k[j] != (int)p);
         if (!SLOT) {
             // heads: the first particle of a run of equal keys
             if (live && k[j] >= 0 && !desc[j]) {
@@ -383,6 +362,7 @@ __global__ void __launch_bounds__(256) k_keys(const PosT* __restrict__ xp, int64
         if (lane == 0) {
             if (nbad) atomicAdd(&err[0], nbad);
             if (anydesc && *(volatile int*)&err[1] == 0) err[1] = 1;
+            if (notlat && *(volatile int*)&err[5] == 0) err[5] = 1;
         }
     };
     if (!SLOT) report();
@@ -461,6 +441,11 @@ __global__ void __launch_bounds__(256) k_count(const int* __restrict__ key, int6
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     slot[p] = atomicAdd(&count[key[p]], 1);
+}
+
+__global__ void __launch_bounds__(256) k_iota(int* __restrict__ a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (int)i;
 }
 
 __global__ void __launch_bounds__(256) k_identity(int64_t np, int* __restrict__ perm) {
@@ -933,7 +918,7 @@ static int bin_crowded(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, 
 }
 
 int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const void* xp,
-                  hipStream_t s, int64_t* nbad, bool table) {
+                  hipStream_t s, int64_t* nbad, bool table, bool lattice) {
     const int64_t ncells = (int64_t)m.nbin[0] * m.nbin[1] * m.nbin[2];
     const int64_t gapcap = np / 16 + 4096;
     const size_t npb = (size_t)(np > 0 ? np : 1);
@@ -951,6 +936,8 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     plan->bin_stream = s;
     plan->prebinned = false;   // a binning left by ppmg_m2p_prebin is overwritten
     plan->bin_table = false;
+    plan->bin_lattice = false;
+    plan->lattice_checked = false;
     plan->np = np;
     plan->ncells = ncells;
     plan->bin_xp = xp;
@@ -964,6 +951,19 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
         return fill_zero(plan->start.p, sizeof(int) * (size_t)(ncells + 1), s);
     }
     if (fill_zero(plan->err.p, sizeof(int) * ERR_WORDS, s)) return PPMG_ERROR_FATAL;
+    // A lattice in cell order (particle p in cell p, np == ncells), as the last
+    // binning of this plan found on the same mesh: for a reader that checks
+    // every particle's cell itself (`lattice`), no pass at all; the reader
+    // reports a particle outside its cell in err[6] and bins again.
+    if (lattice && plan->lattice_pred && np == ncells && plan->lattice_prec == prec &&
+        same_mesh(plan->lattice_mesh, m)) {
+        plan->bin_lattice = true;
+        plan->perm_identity = true;
+        plan->perm_written = false;
+        plan->keys_written = false;
+        return 0;
+    }
+    plan->lattice_pred = false;
     // The plan remembers whether its last input was in cell order; a plan fed
     // the same kind of particles each call (advected particles, or particles
     // fresh from create_part) predicts right and does each step once.
@@ -1026,6 +1026,9 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     for (int i = 8; i < ERR_WORDS; ++i) sampled += herr[i];
     plan->crowded_cells = 64 * sampled;   // estimate from the 1-in-64 sample
     if (herr[0]) return 0;
+    plan->lattice_pred = herr[5] == 0 && np == ncells;
+    plan->lattice_prec = prec;
+    plan->lattice_mesh = m;
     const bool sorted = herr[1] == 0;
     if (predict_sorted) {
         if (sorted && herr[2] <= gapcap) {
@@ -1043,14 +1046,14 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
         // wrong prediction (or more long gaps than the queue holds): bin
         // again on the counting path
         plan->sorted_input = false;
-        return bin_particles(plan, m, prec, np, xp, s, nbad, table);
+        return bin_particles(plan, m, prec, np, xp, s, nbad, table, false);
     }
     if (use_table) {
         if (herr[4]) {
             // a cell holds more than TABLE_SLOTS particles: bin again with
             // keys and slots
             plan->table_skip = 16;
-            return bin_particles(plan, m, prec, np, xp, s, nbad, false);
+            return bin_particles(plan, m, prec, np, xp, s, nbad, false, false);
         }
         // the listed cells (two or more particles) in increasing particle index
         hipLaunchKernelGGL(k_stabilize_table, dim3(nblk(nblk(np, KEYS_BLOCK), STAB_GROUP)), dim3(64), 0, s, count8,
@@ -1118,6 +1121,20 @@ int materialize_keys(ppmg_plan* plan, hipStream_t s) {
 }
 
 int materialize_heads(ppmg_plan* plan, hipStream_t s) {
+    if (plan->bin_lattice) {
+        if (!plan->lattice_checked) {
+            // a lattice binning no gather has checked: bin the positions
+            int64_t nbad = 0;
+            return bin_particles(plan, plan->bin_mesh, plan->bin_prec, plan->np, plan->bin_xp, s, &nbad, false,
+                                 false);
+        }
+        // a checked lattice: cell c holds particle c
+        hipLaunchKernelGGL(k_iota, dim3(nblk(plan->ncells + 1, 256)), dim3(256), 0, s, plan->start.as<int>(),
+                           plan->ncells + 1);
+        PPMG_HIP(hipGetLastError());
+        plan->bin_lattice = false;
+        return 0;
+    }
     if (!plan->bin_table) return 0;
     const int64_t ncells = plan->ncells, np = plan->np;
     int* start = plan->start.as<int>();

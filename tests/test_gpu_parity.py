@@ -276,6 +276,68 @@ def test_m2p_slot_table(core, oracle, kernel, lo, monkeypatch):
         assert _rel(out.cpu().numpy(), oref) < 1e-12, (case, i)
 
 
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
+@pytest.mark.parametrize("prec", [F64, F32C, F32])
+def test_m2p_lattice_without_binning(core, oracle, kernel, prec):
+    """A plan whose last binning found a lattice in cell order (particle p in
+    cell p, one per cell) gathers at the next such input without a binning
+    pass, checking each particle's cell inside the gather; a particle out of
+    its cell sends the call back to the binning.  One plan is fed: the
+    lattice (binned; the lattice is recorded), the same (no pass), through
+    ppmg_m2p_prebin (no pass), with particles 0 and 1 swapped (checked, then
+    binned again), with particle 7 outside the domain (the general path), the
+    lattice again.  Every output equals bitwise that of a fresh plan (a
+    particle's sum does not depend on the binning) and the oracle's; after a
+    checked call sort_view gives the identity heads and permutation."""
+    N, lda = 20, 3
+    rng = np.random.default_rng(83 + kernel + 10 * prec)
+    lattice = wl.jittered_lattice(N, seed=83)
+    cells = (lattice * N).astype(np.int64)
+    assert np.array_equal(cells[:, 0] + N * (cells[:, 1] + N * cells[:, 2]), np.arange(N ** 3))
+    uniq = rng.normal(size=(N, N, N, lda))
+    vdt = _val_dtype(prec)
+    mesh = core.Mesh(3, N + 1)
+    field = _dev(uniq, vdt)
+    swapped = lattice.copy()
+    swapped[[0, 1]] = swapped[[1, 0]]
+    outside = lattice.copy()
+    outside[7] = [1.5, 0.5, 0.5]
+    plan = core.Plan()
+    for i, (case, xp, prebin) in enumerate([("lattice", lattice, False), ("lattice", lattice, False),
+                                            ("prebin", lattice, True), ("swapped", swapped, False),
+                                            ("lattice", lattice, False), ("outside", outside, False),
+                                            ("lattice", lattice, False)]):
+        x = _dev(xp, _pos_dtype(prec))
+        out = torch.empty((len(xp), lda), dtype=vdt, device="cuda")
+        if prebin:
+            core.m2p_prebin(mesh, x, plan, kernel, prec)
+        if case == "outside":
+            # whatever a fresh plan does with a particle outside the domain
+            # (an error, or the general path's values), this plan does too
+            ref = torch.empty_like(out)
+            try:
+                core.m2p(mesh, x, field, [0] * 3, ref, kernel, prec)
+            except RuntimeError as e:
+                with pytest.raises(RuntimeError) as e2:
+                    core.m2p(mesh, x, field, [0] * 3, out, kernel, prec, plan)
+                assert str(e2.value) == str(e)
+            else:
+                core.m2p(mesh, x, field, [0] * 3, out, kernel, prec, plan)
+                assert torch.equal(out, ref), (case, i)
+            continue
+        core.m2p(mesh, x, field, [0] * 3, out, kernel, prec, plan)
+        ref = torch.empty_like(out)
+        core.m2p(mesh, x, field, [0] * 3, ref, kernel, prec)
+        assert torch.equal(out, ref), (case, i)
+        oref = bf.m2p_periodic(xp, uniq, N, kernel)
+        assert _rel(out.double().cpu().numpy(), oref) < TOL[prec], (case, i)
+        if case == "lattice" and i >= 1:
+            k, perm, start = plan.sort_view()
+            assert np.array_equal(start.cpu().numpy(), np.arange(N ** 3 + 1)), (case, i)
+            assert np.array_equal(perm.cpu().numpy(), np.arange(N ** 3)), (case, i)
+            assert np.array_equal(k.cpu().numpy(), np.arange(N ** 3)), (case, i)
+
+
 @pytest.mark.parametrize("dim,N,prec", [(3, 32, F64), (3, 300, F64), (2, 96, F64), (3, 32, F32), (2, 96, F32)])
 def test_binning_crowded_radix(core, oracle, dim, N, prec, monkeypatch):
     """Crowded input is binned by a stable radix sort of (cell key, particle)
