@@ -321,7 +321,8 @@ int ppmg_p2m(const ppmg_mesh* mesh, int kernel, int prec, int64_t np, const void
     const bool force_general = gen && gen[0] && gen[0] != '0';
     mark(plan, s, true);
     if (bin_particles(plan, m, prec, np, xp, s, &nbad,
-                      !force_general && p2m3d_table_applicable(m, kernel, prec, fg)))
+                      !force_general &&
+                          (p2m3d_table_applicable(m, kernel, prec, fg) || p2m2d_table_applicable(m, kernel, fg))))
         return g_status.info;
     if (nbad > 0)
         return fail(PPMG_ERROR_FATAL, PPMG_ERR_PART_UNASS,

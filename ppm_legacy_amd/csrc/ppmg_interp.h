@@ -31,6 +31,8 @@ int p2m3d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t 
 // 2D tiled deposit (ppmg_p2m2d.hip), M'4 or BSP2: per axis the compact layout
 // of a periodic axis or a stored box holding the real range 0..N.
 bool p2m2d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
+// ... and it reads a slot-table binning (bin_particles' `table`) there.
+bool p2m2d_table_applicable(const DevMesh& m, int kernel, const FieldGeom& fg);
 int p2m2d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, const void* xp, const FieldGeom& fg,
                const void* up, void* field, hipStream_t s);
 

@@ -27,10 +27,13 @@ namespace ppmg {
 
 constexpr int M2_RING = 8;   // node rows in the ring
 
-template <int KS, int LDA, typename PosT, typename ValT, typename AccT, bool OPEN>
+// TABLE: the binning is a slot table (byte counts, particle r of cell c at
+// bt.table[r * ncells + c]) instead of heads and a permutation.
+template <int KS, int LDA, typename PosT, typename ValT, typename AccT, bool OPEN, bool TABLE>
 __global__ void __launch_bounds__(64)
-k_m2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
-          const ValT* __restrict__ field, FieldGeom fg, ValT* __restrict__ up, int lda, int c0, int ty) {
+k_m2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, BinTable bt,
+          const PosT* __restrict__ xp, const ValT* __restrict__ field, FieldGeom fg, ValT* __restrict__ up, int lda,
+          int c0, int ty) {
     constexpr int B = Stencil<KS>::B, TX = 17 - KS;
     constexpr int ROWV = 16 * LDA;                 // values per ring row: [c][nx]
     __shared__ ValT ring[M2_RING * ROWV];
@@ -69,17 +72,37 @@ k_m2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
 
     const bool plane_lane = ex >= B && ex - B < nxo;   // lanes that carry particles (row checked per step)
     const int cx = tx0 - B + ex;
-    auto heads = [&](int st, int& f, int& n) {
+    // (f, n): the cell's range [f, n) of the permutation, or (TABLE) its key
+    // and count word, with t its first particle
+    auto heads = [&](int st, int& f, int& n, int& t) {
         f = 0;
         n = 0;
+        t = 0;
         const int cy = y0 + 4 * st + erow;
         if (plane_lane && st < nsteps && cy < y1) {
             const int key = cx + Nx * cy;
-            f = start[key];
-            n = start[key + 1];   // end, subtracted where used: the load stays in flight
+            if constexpr (TABLE) {
+                f = key;
+                n = (int)bt.count[count_word(key)];   // decoded where used: the load stays in flight
+                t = bt.table[key];
+            } else {
+                f = start[key];
+                n = start[key + 1];   // end, subtracted where used: the load stays in flight
+            }
         }
     };
-    auto lookup = [&](int f, int n) { return n > 0 ? (perm ? perm[f] : f) : 0; };
+    auto count_of = [&](int f, int n) {
+        if constexpr (TABLE) return (int)(((unsigned)n >> count_shift(f)) & 255u);
+        else return n - f;
+    };
+    auto lookup = [&](int f, int n, int t) {
+        if constexpr (TABLE) return count_of(f, n) > 0 ? t : 0;
+        else return n > f ? (perm ? perm[f] : f) : 0;
+    };
+    auto entry = [&](int f, int r) {
+        if constexpr (TABLE) return bt.table[(int64_t)r * bt.ncells + f];
+        else return perm ? perm[f + r] : f + r;
+    };
     auto position = [&](int p, int n, PosT (&x)[2]) {
         x[0] = x[1] = PosT(0);
         if (n > 0) {
@@ -87,21 +110,22 @@ k_m2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             x[1] = xp[(int64_t)p * 2 + 1];
         }
     };
-    int first_n, cnt_n, first_nn, cnt_nn;
-    heads(0, first_n, cnt_n);
-    heads(1, first_nn, cnt_nn);
-    int pn = lookup(first_n, cnt_n - first_n);
+    int first_n, cnt_n, t_n, first_nn, cnt_nn, t_nn;
+    heads(0, first_n, cnt_n, t_n);
+    heads(1, first_nn, cnt_nn, t_nn);
+    int pn = lookup(first_n, cnt_n, t_n);
     PosT xn[2];
-    position(pn, cnt_n - first_n, xn);
+    position(pn, count_of(first_n, cnt_n), xn);
     wave_lds_sync();
 
     for (int st = 0; st < nsteps; ++st) {
-        const int first = first_n, cnt = cnt_n - first_n;
+        const int first = first_n, cnt = count_of(first_n, cnt_n);
         asm volatile("" ::"v"(first), "v"(cnt) : "memory");
         first_n = first_nn;
         cnt_n = cnt_nn;
-        const int pnext = lookup(first_n, cnt_n - first_n);
-        heads(st + 2, first_nn, cnt_nn);
+        t_n = t_nn;
+        const int pnext = lookup(first_n, cnt_n, t_n);
+        heads(st + 2, first_nn, cnt_nn, t_nn);
         // the four node rows the step after this one adds to the ring
         const bool more = st + 1 < nsteps;
         ValT stage[LDA];
@@ -158,12 +182,12 @@ k_m2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         if (__ballot(cnt > 0) != 0ull) round(pn, cnt > 0, xn);
         for (int r = 1; __ballot(cnt > r) != 0ull; ++r) {
             PosT x[2];
-            const int p = lookup(first + r, cnt - r);
+            const int p = cnt > r ? entry(first, r) : 0;
             position(p, cnt - r, x);
             round(p, cnt > r, x);
         }
         pn = pnext;
-        position(pn, cnt_n - first_n, xn);   // next step's round 0
+        position(pn, count_of(first_n, cnt_n), xn);   // next step's round 0
         // rows 4st .. 4st+3 are done with: their slots take rows 4st+8 .. 4st+11
         wave_lds_sync();
         if (more) store_row(4 * st + 8 + erow, stage);
@@ -195,17 +219,18 @@ bool m2p2d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
     return true;
 }
 
-template <int KS, int LL, typename PosT, typename ValT, typename AccT, bool OPEN>
+template <int KS, int LL, typename PosT, typename ValT, typename AccT, bool OPEN, bool TABLE = false>
 static void launch_m2_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const FieldGeom& fg, int c0,
                            const ValT* field, ValT* up, hipStream_t s) {
     constexpr int TX = 17 - KS;
     static std::atomic<int> slots{0};
-    if (slots.load() == 0) slots.store(resident_workgroups(k_m2_tile<KS, LL, PosT, ValT, AccT, OPEN>, 64));
+    if (slots.load() == 0) slots.store(resident_workgroups(k_m2_tile<KS, LL, PosT, ValT, AccT, OPEN, TABLE>, 64));
     const int tx = (m.nbin[0] + TX - 1) / TX;
     const int ty = pick_z_chunk(tx, m.nbin[1], slots.load(), 0);
     dim3 grid((unsigned)((int64_t)tx * ((m.nbin[1] + ty - 1) / ty)));
-    hipLaunchKernelGGL((k_m2_tile<KS, LL, PosT, ValT, AccT, OPEN>), grid, dim3(64), 0, s, m,
-                       plan->start.as<int>(), tile_perm(plan), xp, field, fg, up, fg.lda, c0, ty);
+    const BinTable bt{plan->count.as<uint32_t>(), plan->table.as<int>(), plan->ncells};
+    hipLaunchKernelGGL((k_m2_tile<KS, LL, PosT, ValT, AccT, OPEN, TABLE>), grid, dim3(64), 0, s, m,
+                       plan->start.as<int>(), tile_perm(plan), bt, xp, field, fg, up, fg.lda, c0, ty);
 }
 
 template <int KS, typename PosT, typename ValT, typename AccT>
@@ -217,7 +242,8 @@ static int m2p2d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const 
         const int L = lda - c0 < 4 ? lda - c0 : 4;
 #define TL(LL)                                                                                   \
     do {                                                                                         \
-        if (open) launch_m2_open<KS, LL, PosT, ValT, AccT, true>(plan, m, xp, fg, c0, field, up, s);  \
+        if (plan->bin_table) launch_m2_open<KS, LL, PosT, ValT, AccT, false, true>(plan, m, xp, fg, c0, field, up, s); \
+        else if (open) launch_m2_open<KS, LL, PosT, ValT, AccT, true>(plan, m, xp, fg, c0, field, up, s);  \
         else launch_m2_open<KS, LL, PosT, ValT, AccT, false>(plan, m, xp, fg, c0, field, up, s);      \
     } while (0)
         if (L == 1) TL(1);
@@ -236,7 +262,8 @@ static int m2p2d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const 
 int m2p2d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t np, const void* xp,
                const FieldGeom& fg, const void* field, void* up, hipStream_t s) {
     int64_t nbad = 0;
-    if (bin_particles(plan, m, prec, np, xp, s, &nbad)) return PPMG_ERROR_FATAL;
+    // a slot table on a periodic mesh
+    if (bin_particles(plan, m, prec, np, xp, s, &nbad, m.periodic[0] && m.periodic[1])) return PPMG_ERROR_FATAL;
     if (nbad > 0) return 1;
     mark(plan, s);
     if (np <= 0) return 0;

@@ -32,10 +32,14 @@ namespace ppmg {
 constexpr int P2_SERIAL = 4;   // later rounds with at most this many active lanes go one particle at a time
 constexpr int P2_RING = 8;     // node rows in the ring (a step touches 4 + H <= 7)
 
-template <int KS, int LDA, typename PosT, typename ValT, bool OPEN>
+// TABLE: the binning is a slot table (byte counts, particle r of cell c at
+// bt.table[r * ncells + c]) instead of heads and a permutation; a cell's
+// particles come in the same order.
+template <int KS, int LDA, typename PosT, typename ValT, bool OPEN, bool TABLE>
 __global__ void __launch_bounds__(64)
-k_p2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, const PosT* __restrict__ xp,
-          const ValT* __restrict__ up, FieldGeom fg, int c0, int ty, ValT* __restrict__ field) {
+k_p2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm, BinTable bt,
+          const PosT* __restrict__ xp, const ValT* __restrict__ up, FieldGeom fg, int c0, int ty,
+          ValT* __restrict__ field) {
     constexpr int H = Stencil<KS>::H, B = Stencil<KS>::B;
     constexpr int TT = 16 - H;                    // owned node columns per strip
     constexpr int ROW = TT * LDA;                 // ring slots per node row: [c][x]
@@ -68,19 +72,36 @@ k_p2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     const int cx = wcell(tx0 - (H - B) + ex, 0);
     // cell rows of the window: y0 - (H - B) .. y1 - 1 + B, four per step
     const int nsteps = (nyo + H + 3) / 4;
-    auto heads = [&](int st, int& f, int& n) {
+    // (f, n): the cell's range [f, n) of the permutation, or (TABLE) its key
+    // and count word, with t its first particle
+    auto heads = [&](int st, int& f, int& n, int& t) {
         f = 0;
         n = 0;
+        t = 0;
         const int r = 4 * st + erow;                // window row
         const int cr = y0 - (H - B) + r;
         if (st < nsteps && cr <= y1 - 1 + B) {
             const int cy = wcell(cr, 1);
             if (!OPEN || (cx >= 0 && cy >= 0)) {
                 const int key = cx + m.nbin[0] * cy;
-                f = start[key];
-                n = start[key + 1];   // end, subtracted where used: the load stays in flight
+                if constexpr (TABLE) {
+                    f = key;
+                    n = (int)bt.count[count_word(key)];   // decoded where used: the load stays in flight
+                    t = bt.table[key];
+                } else {
+                    f = start[key];
+                    n = start[key + 1];   // end, subtracted where used: the load stays in flight
+                }
             }
         }
+    };
+    auto count_of = [&](int f, int n) {
+        if constexpr (TABLE) return (int)(((unsigned)n >> count_shift(f)) & 255u);
+        else return n - f;
+    };
+    auto entry = [&](int f, int r) {
+        if constexpr (TABLE) return bt.table[(int64_t)r * bt.ncells + f];
+        else return perm ? perm[f + r] : f + r;
     };
     auto load = [&](int p, bool has, PosT (&x)[2], ValT (&u)[LDA]) {
         x[0] = x[1] = PosT(0);
@@ -98,22 +119,26 @@ k_p2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         fr[0] = (ValT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci);
         fr[1] = (ValT)cell_offset<PosT>(x[1], (PosT)m.minv[1], (PosT)m.dxi[1], ci);
     };
-    auto lookup = [&](int f, int n) { return n > f ? (perm ? perm[f] : f) : 0; };
-    int f1, n1, f2, n2;
-    heads(0, f1, n1);
-    heads(1, f2, n2);
-    int pn = lookup(f1, n1);
+    auto lookup = [&](int f, int n, int t) {
+        if constexpr (TABLE) return count_of(f, n) > 0 ? t : 0;
+        else return n > f ? (perm ? perm[f] : f) : 0;
+    };
+    int f1, n1, t1, f2, n2, t2;
+    heads(0, f1, n1, t1);
+    heads(1, f2, n2, t2);
+    int pn = lookup(f1, n1, t1);
     PosT xn[2];
     ValT un[LDA];
-    load(pn, n1 > f1, xn, un);
+    load(pn, count_of(f1, n1) > 0, xn, un);
 
     for (int st = 0; st < nsteps; ++st) {
-        const int first = f1, cnt = n1 - f1;
+        const int first = f1, cnt = count_of(f1, n1);
         asm volatile("" ::"v"(first), "v"(cnt) : "memory");
         f1 = f2;
         n1 = n2;
-        pn = lookup(f1, n1);
-        heads(st + 2, f2, n2);
+        t1 = t2;
+        pn = lookup(f1, n1, t1);
+        heads(st + 2, f2, n2, t2);
         const int ly0 = 4 * st + erow - H;          // chunk node row of y offset 0
         // one round: every lane deposits one particle of its cell (zero
         // strength where it has none)
@@ -165,7 +190,7 @@ k_p2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             if (__popcll(act) > P2_SERIAL) {
                 PosT x[2];
                 ValT u[LDA];
-                load(cnt > r ? (perm ? perm[first + r] : first + r) : 0, cnt > r, x, u);
+                load(cnt > r ? entry(first, r) : 0, cnt > r, x, u);
                 round(x, u);
                 continue;
             }
@@ -182,7 +207,7 @@ k_p2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                 for (int rr = r; rr < sc; ++rr) {
                     PosT x[2];
                     ValT u[LDA], fr[2];
-                    load(perm ? perm[sf + rr] : sf + rr, true, x, u);
+                    load(entry(sf, rr), true, x, u);
                     offsets(x, fr);
                     const ValT w = tile_weight_at<KS, ValT>(fr[0], ox) * tile_weight_at<KS, ValT>(fr[1], oy);
                     if (own) {
@@ -193,7 +218,7 @@ k_p2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             } while (act != 0ull);
             break;
         }
-        load(pn, n1 > f1, xn, un);
+        load(pn, count_of(f1, n1) > 0, xn, un);
         // node rows 4s-H .. 4s-H+3 have all their cells: write the owned ones
         // and clear their slots for rows 4 steps on
         wave_lds_sync();
@@ -215,6 +240,12 @@ k_p2_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     }
 }
 
+// The 2D deposit (and its stored box's shell) reads a slot-table binning on
+// a periodic mesh.
+bool p2m2d_table_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
+    return p2m2d_fast_applicable(m, kernel, fg) && m.periodic[0] && m.periodic[1];
+}
+
 // Per axis the compact layout of a periodic axis (the N unique nodes) or a
 // stored box holding the whole real range 0..N.
 bool p2m2d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
@@ -230,18 +261,19 @@ bool p2m2d_fast_applicable(const DevMesh& m, int kernel, const FieldGeom& fg) {
 
 // One launch for components c0 .. c0+LL-1.  The chunk height comes from the
 // resident workgroup count (each chunk marches H extra cell rows).
-template <int KS, int LL, typename PosT, typename ValT, bool OPEN>
+template <int KS, int LL, typename PosT, typename ValT, bool OPEN, bool TABLE = false>
 static void launch_p2_open(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const ValT* up, const int* perm,
                            const FieldGeom& fg, int c0, ValT* field, hipStream_t s) {
     constexpr int H = Stencil<KS>::H, TT = 16 - H;
     static std::atomic<int> slots{0};
-    if (slots.load() == 0) slots.store(resident_workgroups(k_p2_tile<KS, LL, PosT, ValT, OPEN>, 64));
+    if (slots.load() == 0) slots.store(resident_workgroups(k_p2_tile<KS, LL, PosT, ValT, OPEN, TABLE>, 64));
     const int Rx = m.N[0] + !m.periodic[0], Ry = m.N[1] + !m.periodic[1];
     const int tx = (Rx + TT - 1) / TT;
     const int ty = pick_z_chunk(tx, Ry, slots.load(), H);
     dim3 grid((unsigned)((int64_t)tx * ((Ry + ty - 1) / ty)));
-    hipLaunchKernelGGL((k_p2_tile<KS, LL, PosT, ValT, OPEN>), grid, dim3(64), 0, s, m, plan->start.as<int>(), perm,
-                       xp, up, fg, c0, ty, field);
+    const BinTable bt{plan->count.as<uint32_t>(), plan->table.as<int>(), plan->ncells};
+    hipLaunchKernelGGL((k_p2_tile<KS, LL, PosT, ValT, OPEN, TABLE>), grid, dim3(64), 0, s, m, plan->start.as<int>(),
+                       perm, bt, xp, up, fg, c0, ty, field);
 }
 
 template <int KS, typename PosT, typename ValT>
@@ -249,13 +281,16 @@ static int p2m2d_typed(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const 
                        ValT* field, hipStream_t s) {
     mark(plan, s);
     const int lda = fg.lda;
-    const int* pk = tile_perm(plan);
     const bool open = !(m.periodic[0] && m.periodic[1]);
+    // a slot table is binned on periodic meshes only (p2m2d_table_applicable)
+    if (plan->bin_table && open && materialize_heads(plan, s)) return 1;
+    const int* pk = tile_perm(plan);
     for (int c0 = 0; c0 < lda; c0 += 4) {
         const int L = lda - c0 < 4 ? lda - c0 : 4;
 #define TL(LL)                                                                              \
     do {                                                                                    \
-        if (open) launch_p2_open<KS, LL, PosT, ValT, true>(plan, m, xp, up, pk, fg, c0, field, s); \
+        if (plan->bin_table) launch_p2_open<KS, LL, PosT, ValT, false, true>(plan, m, xp, up, pk, fg, c0, field, s); \
+        else if (open) launch_p2_open<KS, LL, PosT, ValT, true>(plan, m, xp, up, pk, fg, c0, field, s); \
         else launch_p2_open<KS, LL, PosT, ValT, false>(plan, m, xp, up, pk, fg, c0, field, s);     \
     } while (0)
         if (L == 1) TL(1);

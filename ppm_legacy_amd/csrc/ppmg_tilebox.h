@@ -62,8 +62,8 @@ __global__ void __launch_bounds__(256) k_box_dup(FieldGeom fg, int3 n, int3 r, i
 // (y, z real).  Weights and products in the value precision, sums in FP64;
 // with weight tables (wt.w[0] set) the weights are the tables'.  KS: stencil
 // width (4 M'4, 2 BSP2); DIM 2: one plane (no z stencil), positions of two
-// coordinates.  TABLE: the binning is a slot table (3D only) instead of
-// heads and a permutation; a cell's particles come in the same order.
+// coordinates.  TABLE: the binning is a slot table instead of heads and a
+// permutation; a cell's particles come in the same order.
 template <int DIM, int KS, typename PosT, typename ValT, int LL, bool TABLE>
 __global__ void __launch_bounds__(256) k_box_shell(DevMesh m, FieldGeom fg, int3 r, const int* __restrict__ start,
                                                   const int* __restrict__ perm, BinTable bt,
@@ -158,21 +158,16 @@ int fill_stored_box(ppmg_plan* plan, const DevMesh& m, const PosT* xp, const Val
     if (nshell > 0) {
         const unsigned g = (unsigned)((nshell + 255) / 256);
         const BinTable bt{plan->count.as<uint32_t>(), plan->table.as<int>(), plan->ncells};
-        // a slot table is binned for 3D deposits only
-        const bool table = DIM == 3 && plan->bin_table;
+        const bool table = plan->bin_table;
         if (!table && materialize_heads(plan, s)) return 1;
         for (int c0 = 0; c0 < lda; c0 += 4) {
             const int L = lda - c0 < 4 ? lda - c0 : 4;
 #define SH(LL, TB) hipLaunchKernelGGL((k_box_shell<DIM, KS, PosT, ValT, LL, TB>), dim3(g), dim3(256), 0, s, m, fg, r3, \
                                       plan->start.as<int>(), tile_perm(plan), bt, xp, up, c0, field, nshell, wt)
-#define SHL(LL)                               \
-    do {                                      \
-        if constexpr (DIM == 3) {             \
-            if (table) SH(LL, true);          \
-            else SH(LL, false);               \
-        } else {                              \
-            SH(LL, false);                    \
-        }                                     \
+#define SHL(LL)                      \
+    do {                             \
+        if (table) SH(LL, true);     \
+        else SH(LL, false);          \
     } while (0)
             if (L == 1) SHL(1);
             else if (L == 2) SHL(2);

@@ -277,6 +277,62 @@ def test_m2p_slot_table(core, oracle, kernel, lo, monkeypatch):
 
 
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
+@pytest.mark.parametrize("prec", [F64, F32C])
+@pytest.mark.parametrize("lo", [0, -2])
+def test_2d_slot_table(core, oracle, kernel, prec, lo, monkeypatch):
+    """2D tiled deposit and gather on a periodic mesh, compact or the
+    reference's stored box, with the slot-table binning of unsorted
+    particles: a displaced lattice (table) and the same with 40 particles in
+    one cell (the table overflows: heads).  Fields and gathered values equal
+    bitwise those binned with heads (PPMG_NO_TABLE=1), the field the oracle's,
+    the gather the brute-force sum; sort_view gives numpy's stable argsort and
+    heads."""
+    N, lda = 48, 3
+    rng = np.random.default_rng(47 + kernel - lo + 10 * prec)
+    base = (wl.jittered_lattice(N, seed=47, dim=2) + rng.uniform(-0.2, 0.2, size=(N * N, 2)) / N) % 1.0
+    crowded = base.copy()
+    crowded[:40] = (np.array([5, 9]) + rng.uniform(0.1, 0.9, size=(40, 2))) / N
+    t = oracle.topo(2, [N + 1] * 2)
+    mesh = core.Mesh(2, N + 1)
+    g = [-lo] * 2
+    vdt = _val_dtype(prec)
+    uniq = rng.normal(size=(N, N, lda))
+    for case, pts in (("lattice", base), ("crowded", crowded)):
+        xp = pts[rng.permutation(len(pts))]
+        up = rng.normal(size=(len(xp), lda))
+        x, u = _dev(xp, torch.float64), _dev(up, vdt)
+        shape = mesh.compact_shape(lda) if lo == 0 else mesh.padded_shape(lda, g)
+        plan = core.Plan()
+        f = torch.empty(shape, dtype=vdt, device="cuda")
+        core.p2m(mesh, x, u, f, [lo] * 2, kernel, prec, plan)
+        cells = (xp * N).astype(np.int64)
+        key = cells[:, 0] + N * cells[:, 1]
+        k, perm, start = plan.sort_view()
+        assert np.array_equal(k.cpu().numpy(), key), case
+        assert np.array_equal(perm.cpu().numpy(), np.argsort(key, kind="stable")), case
+        assert np.array_equal(start.cpu().numpy(), np.concatenate([[0], np.cumsum(np.bincount(key, minlength=N * N))]))
+        monkeypatch.setenv("PPMG_NO_TABLE", "1")
+        href = torch.empty_like(f)
+        core.p2m(mesh, x, u, href, [lo] * 2, kernel, prec)
+        monkeypatch.delenv("PPMG_NO_TABLE")
+        assert torch.equal(f, href), case
+        info, ref = oracle.p2m(t, kernel, xp, up, [1, 1] if lo == 0 else g)
+        assert info == 0
+        if lo == 0:
+            ref = oracle.unique_nodes(ref, [1, 1], 2)
+        assert _rel(f.double().cpu().numpy(), ref) < TOL[prec], case
+        field = _dev(uniq, vdt)
+        out = torch.empty((len(xp), lda), dtype=vdt, device="cuda")
+        core.m2p(mesh, x, field, [0] * 2, out, kernel, prec, core.Plan())
+        monkeypatch.setenv("PPMG_NO_TABLE", "1")
+        oref = torch.empty_like(out)
+        core.m2p(mesh, x, field, [0] * 2, oref, kernel, prec)
+        monkeypatch.delenv("PPMG_NO_TABLE")
+        assert torch.equal(out, oref), case
+        assert _rel(out.double().cpu().numpy(), bf.m2p_periodic(xp, uniq, N, kernel)) < TOL[prec], case
+
+
+@pytest.mark.parametrize("kernel", [MP4, BSP2])
 @pytest.mark.parametrize("prec", [F64, F32C, F32])
 def test_m2p_lattice_without_binning(core, oracle, kernel, prec):
     """A plan whose last binning found a lattice in cell order (particle p in
