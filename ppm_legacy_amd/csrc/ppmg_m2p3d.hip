@@ -219,6 +219,7 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     position(pn, count_of(first_n, cnt_n), xn);
     __syncthreads();
 
+    bool lat_bad = false;   // (LAT) a particle found outside the cell it was taken for
     for (int cz = z0; cz < z1; ++cz) {
         const int first = first_n, cnt = count_of(first_n, cnt_n);
         // take this plane's heads before the next loads are issued, so that
@@ -237,7 +238,6 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             AccT wx[KS], wy[KS], wz[KS];
 #pragma unroll
             for (int o = 0; o < KS; ++o) wx[o] = wy[o] = wz[o] = AccT(0);
-            bool lat_off = false;
             if (has) {
                 int ci[3];
                 tile_weights<KS, AccT>((AccT)cell_offset<PosT>(x[0], (PosT)m.minv[0], (PosT)m.dxi[0], ci[0]), wx);
@@ -247,16 +247,17 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
                     // The particle taken for this lane's cell must lie in it: the
                     // cell key's test on a periodic mesh (inside [min, max], INT
                     // truncation, index N wrapping to 0), from the cell indices
-                    // the weights took.  No branch here: the flag is stored with
-                    // the results (a store amid the weights kept the ring reads
-                    // from being scheduled beside them).
+                    // the weights took.  No branch here: the outcome is kept and
+                    // stored once, at the end of the kernel (a store amid the
+                    // rounds kept the ring reads from being scheduled beside
+                    // the weights).
                     const int cell[3] = {cx, cy, cz};
                     bool in = true;
 #pragma unroll
                     for (int d = 0; d < 3; ++d)
                         in = in && x[d] >= (PosT)m.minv[d] && x[d] <= (PosT)m.maxv[d] &&
                              (ci[d] == cell[d] || (ci[d] == m.N[d] && cell[d] == 0));
-                    lat_off = !in;
+                    lat_bad = lat_bad || !in;
                 }
             }
             // (y,z) weights of the particle whose stencil takes this lane's column
@@ -309,7 +310,6 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             if (has) {
 #pragma unroll
                 for (int c = 0; c < LDA; ++c) up[(int64_t)p * lda + c0 + c] = res[c];
-                if (LAT && lat_off) *lat_err = 1;
             }
         };
         if (__ballot(cnt > 0) != 0ull) round(pn, cnt > 0, xn);
@@ -324,6 +324,9 @@ k_m3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         __syncthreads();   // every row is done with plane cz-1: its slot takes plane cz+3
         if (more) store_plane(next);
         __syncthreads();
+    }
+    if constexpr (LAT) {
+        if (__ballot(lat_bad) != 0ull && (threadIdx.x & 63) == 0) *lat_err = 1;
     }
 }
 
