@@ -419,6 +419,7 @@ struct ppmg_plan {
     ppmg::DevMesh lattice_mesh{};
     bool bin_lattice = false;
     bool lattice_checked = false;   // ... and a gather has checked it
+    int lattice_skip = 0;           // binnings to run without the lattice after a check failed
     bool perm_identity = false;  // the permutation of the last binning is the identity ...
     bool perm_written = true;    // ... and has (or has not yet) been written into `perm`
     hipStream_t bin_stream = nullptr;   // stream of the last binning

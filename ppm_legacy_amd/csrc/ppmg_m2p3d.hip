@@ -475,8 +475,9 @@ int m2p3d_fast(ppmg_plan* plan, const DevMesh& m, int kernel, int prec, int64_t 
         plan->lattice_checked = true;   // the plan's binning is the lattice (heads written on demand)
         return 0;
     }
-    // not that lattice: bin and gather again
+    // not that lattice: bin and gather again (and bin the next 16 inputs)
     plan->lattice_pred = false;
+    plan->lattice_skip = 16;
     if (bin_particles(plan, m, prec, np, xp, s, &nbad, periodic, false)) return PPMG_ERROR_FATAL;
     if (nbad > 0) return 1;
     return gather();

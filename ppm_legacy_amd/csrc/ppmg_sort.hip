@@ -955,6 +955,11 @@ int bin_particles(ppmg_plan* plan, const DevMesh& m, int prec, int64_t np, const
     // binning of this plan found on the same mesh: for a reader that checks
     // every particle's cell itself (`lattice`), no pass at all; the reader
     // reports a particle outside its cell in err[6] and bins again.
+    // A plan whose lattice check failed takes its next 16 inputs without it.
+    if (lattice && plan->lattice_skip > 0) {
+        --plan->lattice_skip;
+        lattice = false;
+    }
     if (lattice && plan->lattice_pred && np == ncells && plan->lattice_prec == prec &&
         same_mesh(plan->lattice_mesh, m)) {
         plan->bin_lattice = true;

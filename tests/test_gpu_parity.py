@@ -341,8 +341,8 @@ def test_m2p_lattice_without_binning(core, oracle, kernel, prec):
     its cell sends the call back to the binning.  One plan is fed: the
     lattice (binned; the lattice is recorded), the same (no pass), through
     ppmg_m2p_prebin (no pass), with particles 0 and 1 swapped (checked, then
-    binned again), with particle 7 outside the domain (the general path), the
-    lattice again.  Every output equals bitwise that of a fresh plan (a
+    binned again; the plan then bins its next 16 inputs), the lattice, with
+    particle 7 outside the domain (the general path), the lattice again.  Every output equals bitwise that of a fresh plan (a
     particle's sum does not depend on the binning) and the oracle's; after a
     checked call sort_view gives the identity heads and permutation."""
     N, lda = 20, 3
