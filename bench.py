@@ -48,15 +48,17 @@ def parse():
                          "P2M then M2P of the same particles")
     ap.add_argument("--no-box", action="store_true", help="skip the stored-box (reference layout) leg")
     ap.add_argument("--paths", action="store_true",
-                    help="only time P2M and M2P per path family, tiled against the general path: 3D BSP2 on "
-                         "the 512^3 advected lattice, 2D M'4 and BSP2 on an 8192^2 jittered lattice")
+                    help="only time P2M and M2P per path family, tiled against the general path: 3D M'4 and "
+                         "BSP2 on the 512^3 advected lattice (M'4 also in a random particle order), 2D M'4 and "
+                         "BSP2 on an 8192^2 jittered lattice")
     return ap.parse_args()
 
 
 def paths_run(core, wl, torch, args):
     """P2M and M2P (FP64, lda 3, binning included) of the tiled kernels against
     the general node-gather / per-particle paths (PPMG_GENERAL=1) on the same
-    inputs: 3D BSP2 on the advected 512^3 lattice (M'4 beside it), 2D M'4 and
+    inputs: 3D BSP2 on the advected 512^3 lattice (M'4 beside it, and M'4 on the
+    same particles in a random order), 2D M'4 and
     BSP2 on an 8192^2 lattice jittered by +-0.3h and displaced by up to
     +-0.2h.  Median of three calls after one warm-up, CUDA events around each
     call on the current stream."""
@@ -93,6 +95,16 @@ def paths_run(core, wl, torch, args):
                      "tiled_ms": one(mesh, x, w, kernel, mesh.compact_shape(lda), False),
                      "general_ms": one(mesh, x, w, kernel, mesh.compact_shape(lda), True)}
         print(f"# paths {name}: {out[name]}", file=sys.stderr, flush=True)
+    # the same particles in a random order: every lane's particle is then a
+    # scattered record and no path reads its stencils in order
+    gen = torch.Generator(device="cuda").manual_seed(args.seed + 1)
+    order = torch.randperm(N ** 3, generator=gen, device="cuda")
+    x, w = x[order].contiguous(), w[order].contiguous()
+    del order
+    out["3d_mp4_shuffled"] = {"cells": f"{N}^3", "particles": N ** 3,
+                              "tiled_ms": one(mesh, x, w, 2, mesh.compact_shape(lda), False),
+                              "general_ms": one(mesh, x, w, 2, mesh.compact_shape(lda), True)}
+    print(f"# paths 3d_mp4_shuffled: {out['3d_mp4_shuffled']}", file=sys.stderr, flush=True)
     del mesh, x, w
     N2 = 8192 if N == 512 else 16 * N
     n2 = N2 * N2
