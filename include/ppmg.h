@@ -66,6 +66,15 @@ extern "C" {
 #define PPMG_MAP_GHOST_GET 5
 #define PPMG_MAP_GHOST_PUT 6
 
+/* element kinds of a ghost-mapped field: the reference's specifics
+ * ppm_map_field_ghost_*_{d,s,i,l,dc,sc} (ppm_map_field_ghost.f:134-240) */
+#define PPMG_KIND_D  0   /* double (= PPMG_PREC_F64) */
+#define PPMG_KIND_S  2   /* float (= PPMG_PREC_F32) */
+#define PPMG_KIND_I  3   /* 32-bit integer */
+#define PPMG_KIND_L  4   /* logical, 32-bit: nonzero is .TRUE., written as 1 / 0 */
+#define PPMG_KIND_DC 5   /* double complex: (re, im) double pairs */
+#define PPMG_KIND_SC 6   /* single complex: (re, im) float pairs */
+
 /* memory space of caller arrays for the registry layer */
 #define PPMG_MEM_DEVICE 0
 #define PPMG_MEM_HOST   1
@@ -219,6 +228,17 @@ int ppmg_rmsh_create_part(const ppmg_mesh* mesh, int prec, const ppmg_field* fie
  * Element type as for the field of ppmg_p2m (double for PPMG_PREC_F64). */
 int ppmg_map_field_ghost(const ppmg_mesh* mesh, int prec, const ppmg_field* field, const int* ghostsize,
                          int maptype, ppmg_plan* plan, void* stream);
+/* The same for every element kind of the reference's specifics, with its
+ * optional mask (ppm_map_field_ghost.f:134-240, the mask argument; applied
+ * by the pop, ppm_map_field_pop_3d.f:1818-1897 and pop_2d.f:1757-1834).
+ * kind: PPMG_KIND_*; field->lda counts components (complex: complex ones).
+ * Integer fields round-trip exactly and ghost_put adds them (wrapping on
+ * overflow); logical fields are copied by ghost_get and combined by .AND. in
+ * ghost_put.  mask: NULL, or one 32-bit logical per stored node of the
+ * field's box (no component axis, same node order, device memory): a node
+ * whose mask is 0 is left unchanged by the pop. */
+int ppmg_map_field_ghost_kind(const ppmg_mesh* mesh, int kind, const ppmg_field* field, const int* ghostsize,
+                              int maptype, const int* mask, ppmg_plan* plan, void* stream);
 
 /* Phase timing with HIP events on the call's stream (off by default; the
  * PPMG_TIMING environment variable switches it on for new plans).
@@ -297,6 +317,11 @@ int ppmg_rmsh_create_part_reg(const void* field_up, int lda, int topo_id, int me
  * padded field_up (lower bound 1-ghostsize), host or device. */
 int ppmg_map_field_ghost_reg(void* field_up, int lda, int topo_id, int mesh_id, const int* ghostsize, int maptype,
                              int prec, int mem);
+/* The same for every element kind (PPMG_KIND_*), with the optional mask of
+ * ppm_map_field_ghost (NULL: none): one 32-bit logical per node of the
+ * padded box (lower bound 1-ghostsize), in the memory space of field_up. */
+int ppmg_map_field_ghost_kind_reg(void* field_up, int lda, int topo_id, int mesh_id, const int* ghostsize,
+                                  int maptype, int kind, const int* mask, int mem);
 /* ppm_util_sort3d / sort2d(xp,Np,xmin,xmax,Nm,npbx,info) with host or device arrays. */
 int ppmg_util_sort_reg(int dim, void* xp, int np, const void* xmin, const void* xmax, const int* ncells, int* npbx,
                        int prec, int mem);

@@ -143,6 +143,25 @@ def ghost_get(t, field, ghost):
     return info, field
 
 
+# element kinds of ghost_kind (OR_KIND_* of ppm_oracle.h)
+KIND_D, KIND_S, KIND_I, KIND_L, KIND_DC, KIND_SC = 0, 2, 3, 4, 5, 6
+_KIND_DTYPE = {KIND_D: np.float64, KIND_S: np.float32, KIND_I: np.int32, KIND_L: np.int32,
+               KIND_DC: np.complex128, KIND_SC: np.complex64}
+
+
+def ghost_kind(t, field, ghost, kind, get, mask=None):
+    """ppm_map_field_ghost (ghost_get if get else ghost_put) for element kind
+    KIND_*; field (field_shape, the kind's dtype; logicals as int32), mask
+    None or one int32 logical per stored node (field.shape[:-1])."""
+    field = np.ascontiguousarray(field, dtype=_KIND_DTYPE[kind]).copy()
+    if mask is not None:
+        mask = np.ascontiguousarray(mask, dtype=np.int32)
+        assert mask.shape == field.shape[:-1]
+    info = lib().or_ghost_kind(ctypes.byref(t), int(kind), field.shape[-1], _ghost(ghost, t.dim), _ptr(field),
+                               _ptr(mask), int(bool(get)))
+    return info, field
+
+
 def rank(xp, xmin, xmax, Nm, Ngl, dtype=np.float64):
     """ppm_util_rank{2,3}d: returns (lpdx[:icount] 1-based, lhbx 1-based, icount, nout, ncorr)."""
     xp = np.ascontiguousarray(xp, dtype=dtype)

@@ -77,6 +77,111 @@ static int or_fidx(const or_fgeom* f, int i, int j, int k, int64_t* off)
 #undef SQRT
 
 /* ---------------------------------------------------------------------------
+ * ppm_map_field_ghost (ghost_get / ghost_put, push, send, pop on one sub) for
+ * every element kind of the reference's specifics, with the optional mask.
+ * Push packs every node of every block into the double send buffer
+ * (ppm_map_field_push_3d.f:556-592: REAL(.,ppm_kind_double) of reals and
+ * integers, real and imaginary part of complex values, 1.0 / 0.0 of
+ * logicals; its mask test is commented out at :546-549); the pop reads the
+ * buffer back block by block in list order (ppm_map_field_pop_3d.f:
+ * 1818-1897, pop_2d.f:1757-1834).  Where the mask (one logical per stored
+ * node of the padded box) is .FALSE. the node keeps its value and the buffer
+ * steps past it.  Replace converts back (INT() of integers, value >
+ * 1 - ppm_myepsd of logicals); add adds (integers, reals, complex parts) or
+ * takes .AND. (logicals).  ghost_get packs the real block (start) and
+ * replaces its images (start + off); ghost_put packs the images and adds them
+ * onto the real block (ppm_map_field_ghost_put.f:133-134,242-281).
+ * ------------------------------------------------------------------------- */
+int or_ghost_kind(const or_topo* t, int kind, int lda, const int* g, void* field, const int* mask, int get)
+{
+    /* ppm_myepsd = 10**tolexp (ppm_init.f:356); any value in (0,1) reads the
+     * packed 1.0 / 0.0 the same way */
+    const double myeps = 1e-15;
+    int parts;
+    switch (kind) {
+    case OR_KIND_D: case OR_KIND_S: case OR_KIND_I: case OR_KIND_L: parts = 1; break;
+    case OR_KIND_DC: case OR_KIND_SC: parts = 2; break;
+    default: return OR_ERR_ARGUMENT;
+    }
+    blk_d b[26];
+    int nb = ghost_blocks_d(t, g, b);
+    or_fgeom f = or_fgeom_make(t, lda, g);
+    int64_t total = 0;
+    for (int q = 0; q < nb; ++q) total += (int64_t)b[q].size[0] * b[q].size[1] * b[q].size[2] * lda * parts;
+    double* buf = (double*)malloc(sizeof(double) * (size_t)(total > 0 ? total : 1));
+    if (!buf) return OR_ERR_ALLOC;
+    double* fd = (double*)field;
+    float* fs = (float*)field;
+    int* fi = (int*)field;
+    int64_t p = 0;
+    for (int q = 0; q < nb; ++q) {
+        const int sh = get ? 0 : 1;   /* put packs the image block */
+        for (int k = 0; k < b[q].size[2]; ++k)
+            for (int j = 0; j < b[q].size[1]; ++j)
+                for (int i = 0; i < b[q].size[0]; ++i) {
+                    int64_t off;
+                    or_fidx(&f, b[q].start[0] + sh * b[q].off[0] + i, b[q].start[1] + sh * b[q].off[1] + j,
+                            b[q].start[2] + sh * b[q].off[2] + k, &off);
+                    for (int c = 0; c < lda; ++c) {
+                        const int64_t e = off + c;
+                        switch (kind) {
+                        case OR_KIND_D: buf[p++] = fd[e]; break;
+                        case OR_KIND_S: buf[p++] = (double)fs[e]; break;
+                        case OR_KIND_I: buf[p++] = (double)fi[e]; break;
+                        case OR_KIND_L: buf[p++] = fi[e] ? 1.0 : 0.0; break;
+                        case OR_KIND_DC: buf[p++] = fd[2 * e]; buf[p++] = fd[2 * e + 1]; break;
+                        default: buf[p++] = (double)fs[2 * e]; buf[p++] = (double)fs[2 * e + 1]; break;
+                        }
+                    }
+                }
+    }
+    p = 0;
+    for (int q = 0; q < nb; ++q) {
+        const int sh = get ? 1 : 0;   /* get writes the image block */
+        for (int k = 0; k < b[q].size[2]; ++k)
+            for (int j = 0; j < b[q].size[1]; ++j)
+                for (int i = 0; i < b[q].size[0]; ++i) {
+                    int64_t off;
+                    or_fidx(&f, b[q].start[0] + sh * b[q].off[0] + i, b[q].start[1] + sh * b[q].off[1] + j,
+                            b[q].start[2] + sh * b[q].off[2] + k, &off);
+                    if (mask && !mask[off / lda]) {
+                        p += (int64_t)lda * parts;   /* skip this point */
+                        continue;
+                    }
+                    for (int c = 0; c < lda; ++c) {
+                        const int64_t e = off + c;
+                        switch (kind) {
+                        case OR_KIND_D: fd[e] = get ? buf[p] : fd[e] + buf[p]; p++; break;
+                        case OR_KIND_S: fs[e] = get ? (float)buf[p] : fs[e] + (float)buf[p]; p++; break;
+                        case OR_KIND_I: {
+                            const int v = (int)buf[p++];   /* INT(): exact for every 32-bit value */
+                            fi[e] = get ? v : (int)((unsigned)fi[e] + (unsigned)v);
+                            break;
+                        }
+                        case OR_KIND_L: {
+                            const int v = buf[p++] > 1.0 - myeps;
+                            fi[e] = get ? v : (fi[e] != 0 && v);
+                            break;
+                        }
+                        case OR_KIND_DC:
+                            fd[2 * e] = get ? buf[p] : fd[2 * e] + buf[p];
+                            fd[2 * e + 1] = get ? buf[p + 1] : fd[2 * e + 1] + buf[p + 1];
+                            p += 2;
+                            break;
+                        default:
+                            fs[2 * e] = get ? (float)buf[p] : fs[2 * e] + (float)buf[p];
+                            fs[2 * e + 1] = get ? (float)buf[p + 1] : fs[2 * e + 1] + (float)buf[p + 1];
+                            p += 2;
+                            break;
+                        }
+                    }
+                }
+    }
+    free(buf);
+    return OR_OK;
+}
+
+/* ---------------------------------------------------------------------------
  * Multi-threaded FP64 P2M for the CPU baseline.  The reference parallelises
  * P2M by spatial decomposition: each MPI rank scatters the particles of its
  * subs into its own field with ghost layers (ppm_interp_p2m.f:783-3567), then

@@ -42,6 +42,15 @@ typedef struct or_topo {
 #define OR_ERR_ALLOC       101   /* ppm_err_alloc,      ppm_error.h:161 */
 #define OR_ERR_BOUNDS      9001  /* oracle only: stencil node outside the padded field */
 
+/* element kinds of or_ghost_kind (the reference's _d, _s, _i, _l, _dc, _sc
+ * specifics of ppm_map_field_ghost); logicals are 32-bit, nonzero = .TRUE. */
+#define OR_KIND_D  0
+#define OR_KIND_S  2
+#define OR_KIND_I  3
+#define OR_KIND_L  4
+#define OR_KIND_DC 5
+#define OR_KIND_SC 6
+
 #define OR_KERNEL_BSP2 1         /* ppm_param_rmsh_kernel_bsp2, ppm_param.h:405 */
 #define OR_KERNEL_MP4  2         /* ppm_param_rmsh_kernel_mp4,  ppm_param.h:406 */
 
@@ -74,6 +83,11 @@ int or_p2m_adjoint##SFX(const or_topo* t, int kernel, int np, const REAL* xp, in
 
 OR_DECLARE(_d, double)
 OR_DECLARE(_s, float)
+
+/* ppm_map_field_ghost with ghost_get (get = 1) or ghost_put (get = 0), push,
+ * send and pop, for element kind OR_KIND_*, on the padded field
+ * (lda, 1-g:Nm+g, ...); mask: NULL or one logical per stored node. */
+int or_ghost_kind(const or_topo* t, int kind, int lda, const int* ghost, void* field, const int* mask, int get);
 
 /* Multi-threaded FP64 P2M (slab decomposition in z over OpenMP threads, each
  * thread scattering its particles into a private slab with ghost layers, then

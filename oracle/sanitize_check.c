@@ -86,6 +86,23 @@ static void run_double(int dim, int kernel, int periodic, int gw, int np) {
     CHECK(info == 0, "ghost_get_d: info %d", info);
     info = or_ghost_put_d(&t, lda, g, field);
     CHECK(info == 0, "ghost_put_d: info %d", info);
+    {
+        /* every element kind, with and without a mask (16 bytes per element
+         * covers the complex double kind) */
+        const long nn = nf / lda;
+        void* fk = calloc((size_t)nf, 16);
+        int* mk = malloc(sizeof(int) * (size_t)nn);
+        for (long i = 0; i < nn; ++i) mk[i] = (int)(i % 3 != 0);
+        const int kinds[6] = {OR_KIND_D, OR_KIND_S, OR_KIND_I, OR_KIND_L, OR_KIND_DC, OR_KIND_SC};
+        for (int q = 0; q < 6; ++q)
+            for (int use_mask = 0; use_mask < 2; ++use_mask)
+                for (int get = 0; get < 2; ++get) {
+                    info = or_ghost_kind(&t, kinds[q], lda, g, fk, use_mask ? mk : NULL, get);
+                    CHECK(info == 0, "ghost_kind %d mask %d get %d: info %d", kinds[q], use_mask, get, info);
+                }
+        free(fk);
+        free(mk);
+    }
     for (long i = 0; i < nf; ++i) field[i] = 1.5;
     info = or_m2p_d(&t, kernel, np, xp, lda, field, g, back, 1);
     CHECK(info == 0, "m2p_d: info %d", info);

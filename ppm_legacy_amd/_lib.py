@@ -19,6 +19,8 @@ PREC_F32C = 1
 PREC_F32 = 2
 MAP_GHOST_GET = 5          # ppm_param.h:187-188
 MAP_GHOST_PUT = 6
+# element kinds of a ghost-mapped field (the _d, _s, _i, _l, _dc, _sc specifics)
+KIND_D, KIND_S, KIND_I, KIND_L, KIND_DC, KIND_SC = 0, 2, 3, 4, 5, 6
 MEM_DEVICE = 0
 MEM_HOST = 1
 
@@ -59,7 +61,8 @@ EXPORTS = [
     "ppmg_finalize", "ppmg_set_precision_mode", "ppmg_mktopo_field", "ppmg_topo_destroy",
     "ppmg_interp_p2m", "ppmg_interp_m2p", "ppmg_rmsh_comp_weights_reg", "ppmg_rmsh_remesh_reg",
     "ppmg_util_rank_reg", "ppmg_topo_info", "ppmg_impose_part_bc_reg", "ppmg_rmsh_create_part_reg", "ppmg_gen_jittered_lattice", "ppmg_gen_normals", "ppmg_gen_divfree_velocity",
-    "ppmg_map_field_ghost", "ppmg_map_field_ghost_reg", "ppmg_util_sort", "ppmg_util_sort_reg",
+    "ppmg_map_field_ghost", "ppmg_map_field_ghost_reg", "ppmg_map_field_ghost_kind", "ppmg_map_field_ghost_kind_reg",
+    "ppmg_util_sort", "ppmg_util_sort_reg",
 ]
 
 _lib = None

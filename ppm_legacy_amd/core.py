@@ -211,6 +211,31 @@ def map_field_ghost(mesh, field, lo, ghostsize, maptype, prec=PREC_F64, plan=Non
     return plan
 
 
+KIND_DTYPE = {L.KIND_D: torch.float64, L.KIND_S: torch.float32, L.KIND_I: torch.int32, L.KIND_L: torch.int32,
+              L.KIND_DC: torch.complex128, L.KIND_SC: torch.complex64}
+
+
+def map_field_ghost_kind(mesh, field, lo, ghostsize, maptype, kind, mask=None, plan=None):
+    """map_field_ghost for every element kind of the reference's specifics
+    (L.KIND_*: float64, float32, int32, logical as int32, complex128,
+    complex64 tensors) with the optional mask of ppm_map_field_ghost: an int32
+    tensor of the field's node shape (no component axis); the pop leaves a
+    node whose mask is 0 unchanged.  In place."""
+    _check_tensor(field, KIND_DTYPE[kind], "field")
+    mptr = None
+    if mask is not None:
+        _check_tensor(mask, torch.int32, "mask")
+        if tuple(mask.shape) != tuple(field.shape[:-1]):
+            raise ValueError(f"mask must have shape {tuple(field.shape[:-1])}, got {tuple(mask.shape)}")
+        mptr = ctypes.c_void_p(mask.data_ptr())
+    plan = plan or Plan()
+    f = field_desc(mesh, field, lo)
+    g = (ctypes.c_int * 3)(*([int(v) for v in ghostsize] + [0] * 3)[:3])
+    check(lib().ppmg_map_field_ghost_kind(mesh.ref(), int(kind), ctypes.byref(f), g, int(maptype), mptr,
+                                          plan.handle, _stream()))
+    return plan
+
+
 def util_rank(xp, xmin, xmax, ncells, ngl, plan=None):
     """ppm_util_rank3d / rank2d on device: returns (lpdx[:icount] 1-based, lhbx 1-based)."""
     dim = xp.shape[1]

@@ -245,11 +245,22 @@ int ppmg_plan_timings(ppmg_plan* plan, double* ms, int n) {
 
 int ppmg_map_field_ghost(const ppmg_mesh* mesh, int prec, const ppmg_field* field, const int* ghostsize, int maptype,
                          ppmg_plan* plan, void* stream) {
+    return ppmg_map_field_ghost_kind(mesh, prec == PPMG_PREC_F64 ? PPMG_KIND_D : PPMG_KIND_S, field, ghostsize,
+                                     maptype, nullptr, plan, stream);
+}
+
+int ppmg_map_field_ghost_kind(const ppmg_mesh* mesh, int kind, const ppmg_field* field, const int* ghostsize,
+                              int maptype, const int* mask, ppmg_plan* plan, void* stream) {
     clear_error();
     if (check_plan(plan)) return PPMG_ERROR_ERROR;
     if (maptype != PPMG_MAP_GHOST_GET && maptype != PPMG_MAP_GHOST_PUT)
         return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "maptype must be ghost_get or ghost_put");
     if (!ghostsize) return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "ghostsize is NULL");
+    if (kind != PPMG_KIND_D && kind != PPMG_KIND_S && kind != PPMG_KIND_I && kind != PPMG_KIND_L &&
+        kind != PPMG_KIND_DC && kind != PPMG_KIND_SC)
+        return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "unknown element kind");
+    // the mesh geometry is read for its node counts and boundary conditions only
+    const int prec = kind == PPMG_KIND_S || kind == PPMG_KIND_SC ? PPMG_PREC_F32 : PPMG_PREC_F64;
     DevMesh m;
     if (make_mesh(mesh, prec, Spacing::MK, &m)) return g_status.info;
     FieldGeom fg;
@@ -262,7 +273,8 @@ int ppmg_map_field_ghost(const ppmg_mesh* mesh, int prec, const ppmg_field* fiel
         if (fg.lo[d] > -g[d] || fg.lo[d] + fg.ext[d] - 1 < m.N[d] + g[d])
             return fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "field does not hold the mesh and its ghost layers");
     }
-    if (map_field_ghost(m, prec, fg, field->data, g, maptype == PPMG_MAP_GHOST_GET, plan->ghost, as_stream(stream)))
+    if (map_field_ghost(m, kind, fg, field->data, g, maptype == PPMG_MAP_GHOST_GET, mask, plan->ghost,
+                        as_stream(stream)))
         return g_status.info;
     return 0;
 }

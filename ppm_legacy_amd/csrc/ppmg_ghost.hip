@@ -15,6 +15,19 @@
 // blocks are packed before any is unpacked (push before pop), and blocks are
 // unpacked in list order: a node in several image blocks keeps the last
 // block's value (get) or receives the blocks' additions in list order (put).
+//
+// Element kinds are those of the reference's specifics (_s, _d, _sc, _dc, _i,
+// _l): complex fields are real fields of twice the components; the push packs
+// every kind into a real buffer and the pop converts back
+// (ppm_map_field_push_3d.f:564-592, ppm_map_field_pop_3d.f:1818-1897), which
+// for integers is an exact round trip (INT of the packed value) and for
+// logicals 1.0 / 0.0 read back as value > 1 - eps, "add" being .AND.  The
+// optional mask (one logical per stored node) is read at the pop: a node
+// whose mask is .FALSE. keeps its value, and the buffer still steps past it
+// (the push packs every node: its mask test is commented out at
+// ppm_map_field_push_3d.f:546-549).
+#include <type_traits>
+
 #include "ppmg_internal.h"
 #include "ppmg_interp.h"
 
@@ -122,11 +135,13 @@ int ghost_blocks(const DevMesh& m, const int* g, GhostBlock* out) {
 //   PACK_IMG: buf <- field(image block)      (ghost_put push)
 //   POP_SET:  field(image block) <- buf      (ghost_get pop, replace)
 //   POP_ADD:  field(src block) += buf        (ghost_put pop, add)
+// LOGIC: the elements are logicals (nonzero = .TRUE.), set as 1 / 0; the pop
+// adds by .AND.  mask (nullptr: every node) holds one logical per stored node.
 enum GhostOp { PACK_SRC = 0, PACK_IMG = 1, POP_SET = 2, POP_ADD = 3 };
 
-template <typename T>
+template <typename T, bool LOGIC>
 __global__ void __launch_bounds__(256) k_ghost_block(T* __restrict__ field, FieldGeom fg, GhostBlock b,
-                                                     T* __restrict__ buf, int op) {
+                                                     T* __restrict__ buf, int op, const int* __restrict__ mask) {
     const int lda = fg.lda;
     const int64_t n = (int64_t)b.size[0] * b.size[1] * b.size[2] * lda;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
@@ -141,16 +156,28 @@ __global__ void __launch_bounds__(256) k_ghost_block(T* __restrict__ field, Fiel
         const int nx = b.src[0] + i - 1 + (img ? b.off[0] : 0);
         const int ny = b.src[1] + j - 1 + (img ? b.off[1] : 0);
         const int nz = b.src[2] + k - 1 + (img ? b.off[2] : 0);
-        const int64_t at = (((int64_t)(nz - fg.lo[2]) * fg.ext[1] + (ny - fg.lo[1])) * fg.ext[0] + (nx - fg.lo[0])) * lda + c;
-        if (op == PACK_SRC || op == PACK_IMG) buf[e] = field[at];
-        else if (op == POP_SET) field[at] = buf[e];
-        else field[at] += buf[e];
+        const int64_t node = ((int64_t)(nz - fg.lo[2]) * fg.ext[1] + (ny - fg.lo[1])) * fg.ext[0] + (nx - fg.lo[0]);
+        const int64_t at = node * lda + c;
+        if (op == PACK_SRC || op == PACK_IMG) {
+            buf[e] = field[at];
+            continue;
+        }
+        if (mask && mask[node] == 0) continue;
+        if constexpr (LOGIC) {
+            const bool v = buf[e] != T(0);
+            field[at] = (op == POP_SET ? v : (field[at] != T(0) && v)) ? T(1) : T(0);
+        } else if constexpr (std::is_integral<T>::value) {
+            // two's-complement wrap on overflow, as the hardware add
+            field[at] = op == POP_SET ? buf[e] : (T)((uint32_t)field[at] + (uint32_t)buf[e]);
+        } else {
+            field[at] = op == POP_SET ? buf[e] : field[at] + buf[e];
+        }
     }
 }
 
-template <typename T>
-static int ghost_typed(T* field, const FieldGeom& fg, const GhostBlock* b, int nb, bool get, DevBuf& scratch,
-                       hipStream_t s) {
+template <typename T, bool LOGIC>
+static int ghost_typed(T* field, const FieldGeom& fg, const GhostBlock* b, int nb, bool get, const int* mask,
+                       DevBuf& scratch, hipStream_t s) {
     int64_t base[27];
     base[0] = 0;
     for (int q = 0; q < nb; ++q) base[q + 1] = base[q] + (int64_t)b[q].size[0] * b[q].size[1] * b[q].size[2] * fg.lda;
@@ -160,22 +187,36 @@ static int ghost_typed(T* field, const FieldGeom& fg, const GhostBlock* b, int n
     auto grid = [](int64_t n) { return dim3((unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536)); };
     // push: every block is read before any is written
     for (int q = 0; q < nb; ++q)
-        hipLaunchKernelGGL(k_ghost_block<T>, grid(base[q + 1] - base[q]), dim3(256), 0, s, field, fg, b[q],
-                           buf + base[q], get ? (int)PACK_SRC : (int)PACK_IMG);
+        hipLaunchKernelGGL((k_ghost_block<T, LOGIC>), grid(base[q + 1] - base[q]), dim3(256), 0, s, field, fg, b[q],
+                           buf + base[q], get ? (int)PACK_SRC : (int)PACK_IMG, (const int*)nullptr);
     // pop, in list order (a launch per block keeps the order of overlapping blocks)
     for (int q = 0; q < nb; ++q)
-        hipLaunchKernelGGL(k_ghost_block<T>, grid(base[q + 1] - base[q]), dim3(256), 0, s, field, fg, b[q],
-                           buf + base[q], get ? (int)POP_SET : (int)POP_ADD);
+        hipLaunchKernelGGL((k_ghost_block<T, LOGIC>), grid(base[q + 1] - base[q]), dim3(256), 0, s, field, fg, b[q],
+                           buf + base[q], get ? (int)POP_SET : (int)POP_ADD, mask);
     PPMG_HIP(hipGetLastError());
     return 0;
 }
 
-int map_field_ghost(const DevMesh& m, int prec, const FieldGeom& fg, void* field, const int* g, bool get,
-                    DevBuf& scratch, hipStream_t s) {
+int map_field_ghost(const DevMesh& m, int kind, const FieldGeom& fg_in, void* field, const int* g, bool get,
+                    const int* mask, DevBuf& scratch, hipStream_t s) {
     GhostBlock b[26];
     const int nb = ghost_blocks(m, g, b);
-    if (prec == PPMG_PREC_F64) return ghost_typed<double>((double*)field, fg, b, nb, get, scratch, s);
-    return ghost_typed<float>((float*)field, fg, b, nb, get, scratch, s);
+    FieldGeom fg = fg_in;
+    switch (kind) {
+    case PPMG_KIND_D: return ghost_typed<double, false>((double*)field, fg, b, nb, get, mask, scratch, s);
+    case PPMG_KIND_S: return ghost_typed<float, false>((float*)field, fg, b, nb, get, mask, scratch, s);
+    case PPMG_KIND_I: return ghost_typed<int, false>((int*)field, fg, b, nb, get, mask, scratch, s);
+    case PPMG_KIND_L: return ghost_typed<int, true>((int*)field, fg, b, nb, get, mask, scratch, s);
+    case PPMG_KIND_DC:
+        fg.lda *= 2;   // (re, im) pairs: each part maps and adds as a real component
+        return ghost_typed<double, false>((double*)field, fg, b, nb, get, mask, scratch, s);
+    case PPMG_KIND_SC:
+        fg.lda *= 2;
+        return ghost_typed<float, false>((float*)field, fg, b, nb, get, mask, scratch, s);
+    default:
+        set_error(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "unknown element kind");
+        return PPMG_ERROR_ERROR;
+    }
 }
 
 }  // namespace ppmg

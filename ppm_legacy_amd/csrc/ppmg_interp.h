@@ -69,7 +69,8 @@ struct GhostBlock {
     int off[3];
 };
 int ghost_blocks(const DevMesh& m, const int* g, GhostBlock* out);
-int map_field_ghost(const DevMesh& m, int prec, const FieldGeom& fg, void* field, const int* g, bool get,
-                    DevBuf& scratch, hipStream_t s);
+// kind: PPMG_KIND_*; mask: one logical per stored node (nullptr: all nodes)
+int map_field_ghost(const DevMesh& m, int kind, const FieldGeom& fg, void* field, const int* g, bool get,
+                    const int* mask, DevBuf& scratch, hipStream_t s);
 
 }  // namespace ppmg

@@ -428,17 +428,31 @@ int ppmg_rmsh_remesh_reg(const void* xp, int np, const void* up, int lda, int to
 
 int ppmg_map_field_ghost_reg(void* field_up, int lda, int topo_id, int mesh_id, const int* ghostsize, int maptype,
                              int prec, int mem) {
+    // a copy / add of stored values: the double arrays take the FP64 form whatever the precision mode
+    return ppmg_map_field_ghost_kind_reg(field_up, lda, topo_id, mesh_id, ghostsize, maptype,
+                                         prec == PPMG_PREC_F32 ? PPMG_KIND_S : PPMG_KIND_D, nullptr, mem);
+}
+
+int ppmg_map_field_ghost_kind_reg(void* field_up, int lda, int topo_id, int mesh_id, const int* ghostsize,
+                                  int maptype, int kind, const int* mask, int mem) {
     clear_error();
     Topo t;
     if (lookup(topo_id, mesh_id, &t)) return current_info();
     if (!ghostsize || !field_up) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "ghostsize or field_up is NULL");
     if (lda < 1) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "lda must be >0");
+    size_t esz;
+    switch (kind) {
+    case PPMG_KIND_D: esz = 8; break;
+    case PPMG_KIND_S: case PPMG_KIND_I: case PPMG_KIND_L: esz = 4; break;
+    case PPMG_KIND_DC: esz = 16; break;
+    case PPMG_KIND_SC: esz = 8; break;
+    default: return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "unknown element kind");
+    }
     int g[3] = {0, 0, 0};
-    for (int d = 0; d < t.mesh.dim; ++d) g[d] = ghostsize[d];
-    // a copy / add of stored values: the double arrays take the FP64 form whatever the precision mode
-    const bool single = prec == PPMG_PREC_F32;
-    const int form = single ? PPMG_PREC_F32 : PPMG_PREC_F64;
-    const size_t esz = single ? 4 : 8;
+    for (int d = 0; d < t.mesh.dim; ++d) {
+        g[d] = ghostsize[d];
+        if (g[d] < 0) return reg_fail(PPMG_ERROR_ERROR, PPMG_ERR_ARGUMENT, "ghostsize must be >=0 in all dimensions");
+    }
     ppmg_field f = padded_field(t, lda, g, nullptr);
     const size_t bytes = esz * (size_t)field_count(f);
     ppmg_plan* plan = stage_plan();
@@ -446,8 +460,11 @@ int ppmg_map_field_ghost_reg(void* field_up, int lda, int topo_id, int mesh_id, 
     void* dfield = nullptr;
     if (stage_out_ptr(g_stage.field, field_up, bytes, mem, &dfield)) return current_info();
     if (mem == PPMG_MEM_HOST) PPMG_HIP(hipMemcpy(dfield, field_up, bytes, hipMemcpyHostToDevice));
+    const void* dmask = nullptr;
+    if (mask && stage_in(g_stage.lh, mask, sizeof(int) * (size_t)(field_count(f) / lda), mem, &dmask))
+        return current_info();
     f.data = dfield;
-    int rc = ppmg_map_field_ghost(&t.mesh, form, &f, g, maptype, plan, nullptr);
+    int rc = ppmg_map_field_ghost_kind(&t.mesh, kind, &f, g, maptype, (const int*)dmask, plan, nullptr);
     if (rc) return rc;
     if (mem == PPMG_MEM_HOST) PPMG_HIP(hipMemcpy(field_up, dfield, bytes, hipMemcpyDeviceToHost));
     PPMG_HIP(hipDeviceSynchronize());

@@ -49,6 +49,9 @@ MODULE ppm_module_ppmg
   ! library-side selectors
   INTEGER(C_INT), PARAMETER :: ppmg_prec_f64 = 0, ppmg_prec_f32c = 1, ppmg_prec_f32 = 2
   INTEGER(C_INT), PARAMETER :: ppmg_mem_host = 1
+  ! element kinds of ppmg_map_field_ghost_kind_reg (PPMG_KIND_* of ppmg.h)
+  INTEGER(C_INT), PARAMETER :: ppmg_kind_d = 0, ppmg_kind_s = 2, ppmg_kind_i = 3, ppmg_kind_l = 4, &
+                               ppmg_kind_dc = 5, ppmg_kind_sc = 6
 
   INTEGER :: ppm_debug = 0
   INTEGER :: ppm_dim = 0
@@ -122,6 +125,13 @@ MODULE ppm_module_ppmg
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: field_up
       INTEGER(C_INT), VALUE :: lda, topo_id, mesh_id, maptype, prec, mem
+      INTEGER(C_INT) :: ghostsize(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION ppmg_map_field_ghost_kind_reg(field_up, lda, topo_id, mesh_id, ghostsize, maptype, &
+                                                          kind, mask, mem) BIND(C, NAME='ppmg_map_field_ghost_kind_reg')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: field_up, mask
+      INTEGER(C_INT), VALUE :: lda, topo_id, mesh_id, maptype, kind, mem
       INTEGER(C_INT) :: ghostsize(*)
     END FUNCTION
     INTEGER(C_INT) FUNCTION ppmg_util_sort_reg(dim, xp, np, xmin, xmax, ncells, npbx, prec, mem) &
@@ -386,24 +396,99 @@ CONTAINS
     CALL ppmg_report('ppm_rmsh_remesh', info)
   END SUBROUTINE remesh_core_s
 
-  SUBROUTINE ghost_core_d(field, lda, topo_id, mesh_id, cg, maptype, info)
+  SUBROUTINE ghost_core_d(field, lda, topo_id, mesh_id, cg, maptype, mi, has_mask, info)
     REAL(C_DOUBLE), TARGET :: field(*)
     INTEGER, INTENT(IN) :: lda, topo_id, mesh_id, maptype
     INTEGER(C_INT), INTENT(IN) :: cg(3)
+    INTEGER(C_INT), TARGET, INTENT(IN) :: mi(*)
+    LOGICAL, INTENT(IN) :: has_mask
     INTEGER, INTENT(OUT) :: info
-    info = ppmg_map_field_ghost_reg(C_LOC(field(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
-                                    cg, INT(maptype, C_INT), ppmg_prec_f64, ppmg_mem_host)
+    info = ppmg_map_field_ghost_kind_reg(C_LOC(field(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
+                                         cg, INT(maptype, C_INT), ppmg_kind_d, mask_ptr(mi, has_mask), ppmg_mem_host)
     CALL ppmg_report('ppm_map_field_ghost', info)
   END SUBROUTINE ghost_core_d
 
-  SUBROUTINE ghost_core_s(field, lda, topo_id, mesh_id, cg, maptype, info)
+  SUBROUTINE ghost_core_s(field, lda, topo_id, mesh_id, cg, maptype, mi, has_mask, info)
     REAL(C_FLOAT), TARGET :: field(*)
     INTEGER, INTENT(IN) :: lda, topo_id, mesh_id, maptype
     INTEGER(C_INT), INTENT(IN) :: cg(3)
+    INTEGER(C_INT), TARGET, INTENT(IN) :: mi(*)
+    LOGICAL, INTENT(IN) :: has_mask
     INTEGER, INTENT(OUT) :: info
-    info = ppmg_map_field_ghost_reg(C_LOC(field(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
-                                    cg, INT(maptype, C_INT), ppmg_prec_f32, ppmg_mem_host)
+    info = ppmg_map_field_ghost_kind_reg(C_LOC(field(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
+                                         cg, INT(maptype, C_INT), ppmg_kind_s, mask_ptr(mi, has_mask), ppmg_mem_host)
     CALL ppmg_report('ppm_map_field_ghost', info)
   END SUBROUTINE ghost_core_s
+
+  SUBROUTINE ghost_core_dc(field, lda, topo_id, mesh_id, cg, maptype, mi, has_mask, info)
+    COMPLEX(C_DOUBLE_COMPLEX), TARGET :: field(*)
+    INTEGER, INTENT(IN) :: lda, topo_id, mesh_id, maptype
+    INTEGER(C_INT), INTENT(IN) :: cg(3)
+    INTEGER(C_INT), TARGET, INTENT(IN) :: mi(*)
+    LOGICAL, INTENT(IN) :: has_mask
+    INTEGER, INTENT(OUT) :: info
+    info = ppmg_map_field_ghost_kind_reg(C_LOC(field(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
+                                         cg, INT(maptype, C_INT), ppmg_kind_dc, mask_ptr(mi, has_mask), ppmg_mem_host)
+    CALL ppmg_report('ppm_map_field_ghost', info)
+  END SUBROUTINE ghost_core_dc
+
+  SUBROUTINE ghost_core_sc(field, lda, topo_id, mesh_id, cg, maptype, mi, has_mask, info)
+    COMPLEX(C_FLOAT_COMPLEX), TARGET :: field(*)
+    INTEGER, INTENT(IN) :: lda, topo_id, mesh_id, maptype
+    INTEGER(C_INT), INTENT(IN) :: cg(3)
+    INTEGER(C_INT), TARGET, INTENT(IN) :: mi(*)
+    LOGICAL, INTENT(IN) :: has_mask
+    INTEGER, INTENT(OUT) :: info
+    info = ppmg_map_field_ghost_kind_reg(C_LOC(field(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
+                                         cg, INT(maptype, C_INT), ppmg_kind_sc, mask_ptr(mi, has_mask), ppmg_mem_host)
+    CALL ppmg_report('ppm_map_field_ghost', info)
+  END SUBROUTINE ghost_core_sc
+
+  SUBROUTINE ghost_core_i(field, lda, topo_id, mesh_id, cg, maptype, mi, has_mask, info)
+    INTEGER(C_INT), TARGET :: field(*)
+    INTEGER, INTENT(IN) :: lda, topo_id, mesh_id, maptype
+    INTEGER(C_INT), INTENT(IN) :: cg(3)
+    INTEGER(C_INT), TARGET, INTENT(IN) :: mi(*)
+    LOGICAL, INTENT(IN) :: has_mask
+    INTEGER, INTENT(OUT) :: info
+    info = ppmg_map_field_ghost_kind_reg(C_LOC(field(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
+                                         cg, INT(maptype, C_INT), ppmg_kind_i, mask_ptr(mi, has_mask), ppmg_mem_host)
+    CALL ppmg_report('ppm_map_field_ghost', info)
+  END SUBROUTINE ghost_core_i
+
+  ! Logical fields travel as C logicals (1 / 0) and come back as .TRUE. / .FALSE.
+  SUBROUTINE ghost_core_l(field, lda, topo_id, mesh_id, cg, maptype, mi, has_mask, info)
+    LOGICAL :: field(*)
+    INTEGER, INTENT(IN) :: lda, topo_id, mesh_id, maptype
+    INTEGER(C_INT), INTENT(IN) :: cg(3)
+    INTEGER(C_INT), TARGET, INTENT(IN) :: mi(*)
+    LOGICAL, INTENT(IN) :: has_mask
+    INTEGER, INTENT(OUT) :: info
+    INTEGER(C_INT), ALLOCATABLE, TARGET :: fi(:)
+    INTEGER :: n, d, nm(3), tdim
+    CALL ppmg_topo_nm(topo_id, tdim, nm, info)
+    IF (info /= 0) THEN
+      CALL ppmg_report('ppm_map_field_ghost', info)
+      RETURN
+    END IF
+    n = lda
+    DO d = 1, tdim
+      n = n * (nm(d) + 2 * cg(d))
+    END DO
+    ALLOCATE(fi(n))
+    fi = MERGE(1_C_INT, 0_C_INT, field(1:n))
+    info = ppmg_map_field_ghost_kind_reg(C_LOC(fi(1)), INT(lda, C_INT), INT(topo_id, C_INT), INT(mesh_id, C_INT), &
+                                         cg, INT(maptype, C_INT), ppmg_kind_l, mask_ptr(mi, has_mask), ppmg_mem_host)
+    IF (info == 0) field(1:n) = fi /= 0
+    CALL ppmg_report('ppm_map_field_ghost', info)
+  END SUBROUTINE ghost_core_l
+
+  FUNCTION mask_ptr(mi, has_mask) RESULT(p)
+    INTEGER(C_INT), TARGET, INTENT(IN) :: mi(*)
+    LOGICAL, INTENT(IN) :: has_mask
+    TYPE(C_PTR) :: p
+    p = C_NULL_PTR
+    IF (has_mask) p = C_LOC(mi(1))
+  END FUNCTION mask_ptr
 
 END MODULE ppmg_fcore

@@ -227,12 +227,31 @@ def ppm_rmsh_create_part(field_up, topo_id, mesh_id, cutoff, ghostsize, resetpos
 _ghost_map = {"type": None, "pushed": False}
 
 
-def ppm_map_field_ghost(field_up, topo_id, mesh_id, ghostsize, maptype, lda=None):
-    """ppm_map_field_ghost(fv,[lda,]topo_id,mesh_id,ghostsize,maptype,info)
+def _kind_of(dtype):
+    """Element kind of a field dtype: the reference's _d, _s, _i, _l, _dc, _sc
+    specifics (None: not one of them)."""
+    for kind, dts in ((L.KIND_D, (np.float64, torch.float64)), (L.KIND_S, (np.float32, torch.float32)),
+                      (L.KIND_I, (np.int32, torch.int32)), (L.KIND_L, (np.bool_, torch.bool)),
+                      (L.KIND_DC, (np.complex128, torch.complex128)), (L.KIND_SC, (np.complex64, torch.complex64))):
+        if any(dtype == d if isinstance(d, torch.dtype) else dtype == np.dtype(d) for d in dts):
+            return kind
+    return None
+
+
+def _as_int32(a):
+    """A 32-bit logical copy of a boolean array, in its memory space."""
+    return a.to(torch.int32) if isinstance(a, torch.Tensor) else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def ppm_map_field_ghost(field_up, topo_id, mesh_id, ghostsize, maptype, lda=None, mask=None):
+    """ppm_map_field_ghost(fv,[lda,]topo_id,mesh_id,ghostsize,maptype,info,[mask])
     (ppm_map_field_ghost.f:627-714) for one sub: init, ghost_get or ghost_put,
     push, send, pop in the reference's order.  The mapping is applied to
     field_up at pop (ghost_get: periodic images := real nodes; ghost_put:
-    images added onto their real nodes)."""
+    images added onto their real nodes).  field_up may be float64, float32,
+    int32, bool (LOGICAL: ghost_put combines by .AND.), complex128 or
+    complex64; mask (bool or int32, the field's node shape, in the field's
+    memory space) leaves the nodes where it is false unchanged at the pop."""
     if maptype == ppm_param_map_init or maptype == ppm_param_map_cancel:
         _ghost_map.update(type=None, pushed=False)
         return 0
@@ -249,14 +268,28 @@ def ppm_map_field_ghost(field_up, topo_id, mesh_id, ghostsize, maptype, lda=None
     if maptype == ppm_param_map_pop:
         if not _ghost_map["pushed"]:
             return _fail_map("pop without push")
-        mem, pf, dt = _mem_ptr(field_up)
         info, dim = _topo_dim(topo_id)
         if info:
             return info
+        kind = _kind_of(field_up.dtype)
+        if kind is None:
+            return _fail_map("field_up has no ppm_map_field_ghost specific")
         if lda is None:
             lda = field_up.shape[-1] if field_up.ndim == dim + 1 else 1
-        info = lib().ppmg_map_field_ghost_reg(pf, int(lda), int(topo_id), int(mesh_id), _gs(ghostsize, dim),
-                                              int(_ghost_map["type"]), _prec_of(dt), mem)
+        work = _as_int32(field_up) if kind == L.KIND_L else field_up
+        mem, pf, _ = _mem_ptr(work)
+        mp = None
+        if mask is not None:
+            mask = _as_int32(mask)
+            if tuple(mask.shape) != tuple(field_up.shape[:dim]):
+                return _fail_map("mask does not match the field's nodes")
+            mmem, mp, _ = _mem_ptr(mask)
+            if mmem != mem:
+                return _fail_map("mask and field_up are in different memory spaces")
+        info = lib().ppmg_map_field_ghost_kind_reg(pf, int(lda), int(topo_id), int(mesh_id), _gs(ghostsize, dim),
+                                                   int(_ghost_map["type"]), int(kind), mp, mem)
+        if info == 0 and kind == L.KIND_L:
+            field_up[...] = work != 0
         _ghost_map.update(pushed=False)
         return info
     return _fail_map("unknown maptype")

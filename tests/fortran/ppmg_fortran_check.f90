@@ -47,6 +47,11 @@ PROGRAM ppmg_fortran_check
   INTEGER, DIMENSION(:,:), POINTER :: nmc => NULL()
   TYPE(ppm_type_ptr_to_clist), DIMENSION(:), POINTER :: clist => NULL()
   REAL(MK), DIMENSION(3) :: rc
+  INTEGER, DIMENSION(:,:,:,:,:), POINTER :: fi => NULL()
+  LOGICAL, DIMENSION(:,:,:,:), POINTER :: fl => NULL(), msk => NULL()
+  LOGICAL, DIMENSION(:,:,:), POINTER :: msk2 => NULL()
+  COMPLEX(MK), DIMENSION(:,:,:,:), POINTER :: fc => NULL()
+  INTEGER :: topo3, mesh3, j, k, c
 
   CALL GET_COMMAND_ARGUMENT(1, fin)
   CALL GET_COMMAND_ARGUMENT(2, fout)
@@ -190,6 +195,61 @@ PROGRAM ppmg_fortran_check
   CALL ppm_clist_destroy(clist, info)
   CALL check(info, 'ppm_clist_destroy')
   WRITE(20) ASSOCIATED(clist)
+
+  ! 14. ghost mapping of the other element kinds, with the mask: an integer
+  !     vector field (ghost_put, masked), a logical scalar field (ghost_put,
+  !     then ghost_get, masked), a double complex vector field on a 2D
+  !     topology (ghost_get, masked); values and masks are closed forms of the
+  !     node indices
+  ALLOCATE(fi(2, 1-g:nm+g, 1-g:nm+g, 1-g:nm+g, 1), fl(1-g:nm+g, 1-g:nm+g, 1-g:nm+g, 1))
+  ALLOCATE(msk(1-g:nm+g, 1-g:nm+g, 1-g:nm+g, 1))
+  DO k = 1-g, nm+g
+    DO j = 1-g, nm+g
+      DO i = 1-g, nm+g
+        DO c = 1, 2
+          fi(c, i, j, k, 1) = MOD(7*i + 13*j + 29*k + 3*c + 1000, 101) - 50
+        END DO
+        fl(i, j, k, 1) = MOD(i + 2*j + 3*k + 100, 5) /= 0
+        msk(i, j, k, 1) = MOD(i*j + k + 100, 3) /= 0
+      END DO
+    END DO
+  END DO
+  CALL ppm_map_field_ghost(fi, 2, topo, mesh, gs, ppm_param_map_init, info)
+  CALL ppm_map_field_ghost(fi, 2, topo, mesh, gs, ppm_param_map_ghost_put, info)
+  CALL ppm_map_field_ghost(fi, 2, topo, mesh, gs, ppm_param_map_push, info)
+  CALL ppm_map_field_ghost(fi, 2, topo, mesh, gs, ppm_param_map_send, info)
+  CALL ppm_map_field_ghost(fi, 2, topo, mesh, gs, ppm_param_map_pop, info, msk)
+  CALL check(info, 'ppm_map_field_ghost 3d_vec_i pop (put, masked)')
+  WRITE(20) fi
+  CALL ppm_map_field_ghost(fl, topo, mesh, gs, ppm_param_map_ghost_put, info)
+  CALL ppm_map_field_ghost(fl, topo, mesh, gs, ppm_param_map_push, info)
+  CALL ppm_map_field_ghost(fl, topo, mesh, gs, ppm_param_map_send, info)
+  CALL ppm_map_field_ghost(fl, topo, mesh, gs, ppm_param_map_pop, info)
+  CALL check(info, 'ppm_map_field_ghost 3d_sca_l pop (put)')
+  CALL ppm_map_field_ghost(fl, topo, mesh, gs, ppm_param_map_ghost_get, info)
+  CALL ppm_map_field_ghost(fl, topo, mesh, gs, ppm_param_map_push, info)
+  CALL ppm_map_field_ghost(fl, topo, mesh, gs, ppm_param_map_send, info)
+  CALL ppm_map_field_ghost(fl, topo, mesh, gs, ppm_param_map_pop, info, msk)
+  CALL check(info, 'ppm_map_field_ghost 3d_sca_l pop (get, masked)')
+  WRITE(20) MERGE(1, 0, fl)
+  CALL ppm_mktopo(xp, np, (/ nm, nm /), decomp, ppm_param_assign_internal, minp(1:2), maxp(1:2), bc(1:4), &
+                  gs(1:2), topo3, mesh3, min_sub, max_sub, cost, sub2proc, nsubs, isublist, nsublist, istart, ndata, info)
+  CALL check(info, 'ppm_mktopo 2d (complex ghost mapping)')
+  ALLOCATE(fc(2, 1-g:nm+g, 1-g:nm+g, 1), msk2(1-g:nm+g, 1-g:nm+g, 1))
+  DO j = 1-g, nm+g
+    DO i = 1-g, nm+g
+      DO c = 1, 2
+        fc(c, i, j, 1) = CMPLX(REAL(i, MK) + 0.5_MK * c, REAL(j, MK) - 0.25_MK * c, MK)
+      END DO
+      msk2(i, j, 1) = MOD(i + 3*j + 100, 4) /= 0
+    END DO
+  END DO
+  CALL ppm_map_field_ghost(fc, 2, topo3, mesh3, gs(1:2), ppm_param_map_ghost_get, info)
+  CALL ppm_map_field_ghost(fc, 2, topo3, mesh3, gs(1:2), ppm_param_map_push, info)
+  CALL ppm_map_field_ghost(fc, 2, topo3, mesh3, gs(1:2), ppm_param_map_send, info)
+  CALL ppm_map_field_ghost(fc, 2, topo3, mesh3, gs(1:2), ppm_param_map_pop, info, msk2)
+  CALL check(info, 'ppm_map_field_ghost 2d_vec_dc pop (get, masked)')
+  WRITE(20) fc
 
   CLOSE(20)
   CALL ppm_finalize(info)

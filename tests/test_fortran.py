@@ -167,4 +167,26 @@ def test_fortran_dropin_against_oracle(tmp_path, oracle, kernel):
     lref, href, ic, _, _ = oracle.rank(xq, [0.0] * 3, [1.0] * 3, ncell, [1] * 6)
     assert nl == ic and np.array_equal(lpdx, lref) and np.array_equal(lhbx, href)
     assert int(rd.take(np.int32, 1)[0]) == 0            # clist deallocated
+    # 14. the other element kinds with the mask (closed-form values of the node
+    #     indices, built here as the driver builds them)
+    b = np.arange(1 - g, nm + g + 1)
+    k3, j3, i3 = np.meshgrid(b, b, b, indexing="ij")                  # (z, y, x) node indices
+    cc = np.arange(1, 3)
+    fi0 = ((7 * i3 + 13 * j3 + 29 * k3)[..., None] + 3 * cc + 1000) % 101 - 50
+    mk = ((i3 * j3 + k3 + 100) % 3 != 0).astype(np.int32)
+    fi = rd.take(np.int32, 2 * n ** 3).reshape(n, n, n, 2)
+    info, ref = oracle.ghost_kind(t, fi0.astype(np.int32), [g] * 3, oracle.KIND_I, False, mk)
+    assert info == 0 and np.array_equal(fi, ref)
+    fl0 = (((i3 + 2 * j3 + 3 * k3 + 100) % 5 != 0).astype(np.int32))[..., None]
+    info, ref = oracle.ghost_kind(t, fl0, [g] * 3, oracle.KIND_L, False)
+    assert info == 0
+    info, ref = oracle.ghost_kind(t, ref, [g] * 3, oracle.KIND_L, True, mk)
+    fl = rd.take(np.int32, n ** 3).reshape(n, n, n, 1)
+    assert info == 0 and np.array_equal(fl, ref)
+    j2, i2 = np.meshgrid(b, b, indexing="ij")
+    fc0 = (i2[..., None] + 0.5 * cc) + 1j * (j2[..., None] - 0.25 * cc)
+    mk2 = ((i2 + 3 * j2 + 100) % 4 != 0).astype(np.int32)
+    fc = rd.take(np.complex128, 2 * n ** 2).reshape(n, n, 2)
+    info, ref = oracle.ghost_kind(oracle.topo(2, [nm] * 2), fc0, [g] * 2, oracle.KIND_DC, True, mk2)
+    assert info == 0 and np.array_equal(fc, ref)
     assert rd.done()

@@ -2,6 +2,8 @@
 same seeded inputs.  Bars: bit-exact cell keys and sort permutation; FP64
 fields within 1e-12 of the field maximum; FP32-compensated and FP32 within
 1e-6 of the field maximum (tolerances written per test)."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -1695,6 +1697,122 @@ def test_map_field_ghost_rejects_bad_arguments(core):
     with pytest.raises(L.PpmgError) as e:
         core.map_field_ghost(mesh, f, [-1] * 3, [1] * 3, 7)
     assert e.value.code == L.ERR_ARGUMENT
+
+
+def _kind_values(rng, kind, shape, L):
+    """Random values of an element kind: (numpy array, torch dtype)."""
+    if kind == L.KIND_I:
+        return rng.integers(-2 ** 31, 2 ** 31 - 1, size=shape, dtype=np.int64).astype(np.int32), torch.int32
+    if kind == L.KIND_L:
+        return (rng.random(size=shape) < 0.7).astype(np.int32) * rng.integers(1, 4, size=shape, dtype=np.int32), \
+            torch.int32
+    if kind == L.KIND_DC:
+        return rng.normal(size=shape) + 1j * rng.normal(size=shape), torch.complex128
+    if kind == L.KIND_SC:
+        return (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64), torch.complex64
+    if kind == L.KIND_S:
+        return rng.normal(size=shape).astype(np.float32), torch.float32
+    return rng.normal(size=shape), torch.float64
+
+
+@pytest.mark.parametrize("dim,bc,g", [(3, [1] * 6, 2), (3, [1, 1, 2, 2, 1, 1], 1), (2, [1] * 4, 3),
+                                      (2, [2, 2, 1, 1], 2)])
+@pytest.mark.parametrize("kind", [0, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("masked", [False, True])
+def test_map_field_ghost_kinds_vs_oracle(core, oracle, dim, bc, g, kind, masked):
+    """ppm_map_field_ghost for every element kind of the reference's specifics
+    (double, single, integer, logical, double and single complex), with and
+    without the mask, ghost_put then ghost_get, against the oracle's
+    restatement of the reference's push into a double buffer and per-kind pop
+    (ppm_map_field_pop_3d.f:1818-1897): bit-exact, integers wrapping on
+    overflow, logicals combined by .AND."""
+    from ppm_legacy_amd import _lib as L
+    nm = [9, 7, 6][:dim]
+    rng = np.random.default_rng(41 + 7 * kind + g + 10 * dim + masked)
+    t = oracle.topo(dim, nm, bcdef=bc)
+    mesh = core.Mesh(dim, nm, bcdef=bc)
+    shape = tuple(reversed([n + 2 * g for n in nm])) + (2,)
+    vals, tdt = _kind_values(rng, kind, shape, L)
+    mask = (rng.random(size=shape[:-1]) < 0.6).astype(np.int32) if masked else None
+    f = torch.as_tensor(np.ascontiguousarray(vals)).to(tdt).cuda()
+    m = torch.as_tensor(mask).cuda() if masked else None
+    core.map_field_ghost_kind(mesh, f, [-g] * dim, [g] * dim, L.MAP_GHOST_PUT, kind, m)
+    info, ref = oracle.ghost_kind(t, vals, [g] * dim, kind, False, mask)
+    assert info == 0
+    assert np.array_equal(f.cpu().numpy(), ref)
+    core.map_field_ghost_kind(mesh, f, [-g] * dim, [g] * dim, L.MAP_GHOST_GET, kind, m)
+    info, ref = oracle.ghost_kind(t, ref, [g] * dim, kind, True, mask)
+    assert info == 0
+    assert np.array_equal(f.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("on_device", [False, True])
+def test_ppm_map_field_ghost_sequence_kinds(oracle, on_device):
+    """The mirror's ppm_map_field_ghost call sequence (init, ghost_get or
+    ghost_put, push, send, pop) on host arrays and on device tensors of every
+    kind: a LOGICAL field as a bool array, a complex one as complex, with a
+    bool mask; results as the oracle's.  Out of sequence calls and a field
+    type with no specific are errors."""
+    from ppm_legacy_amd import _lib as L
+    from ppm_legacy_amd import ppm
+    N, g = 8, 2
+    assert ppm.ppm_init(3, ppm.ppm_kind_double, -15, None, 0) == 0
+    try:
+        info, topo, meshid = ppm.ppm_mktopo([0.0] * 3, [1.0] * 3, [1, 1, 2, 2, 1, 1], [N + 1] * 3, [g] * 3)
+        assert info == 0
+        t = oracle.topo(3, [N + 1] * 3, bcdef=[1, 1, 2, 2, 1, 1])
+        rng = np.random.default_rng(9)
+        shape = (N + 1 + 2 * g,) * 3 + (3,)
+        mask = rng.random(size=shape[:-1]) < 0.5
+        cases = [(L.KIND_D, rng.normal(size=shape)),
+                 (L.KIND_I, rng.integers(-1000, 1000, size=shape).astype(np.int32)),
+                 (L.KIND_L, rng.random(size=shape) < 0.8),
+                 (L.KIND_DC, rng.normal(size=shape) + 1j * rng.normal(size=shape))]
+        wrap = (lambda a: torch.as_tensor(a).cuda()) if on_device else (lambda a: a.copy())
+        back = (lambda a: a.cpu().numpy()) if on_device else (lambda a: a)
+        for kind, vals in cases:
+            for mt in (ppm.ppm_param_map_ghost_put, ppm.ppm_param_map_ghost_get):
+                for use_mask in (False, True):
+                    fv = wrap(vals)
+                    mk = wrap(mask) if use_mask else None
+                    for step in (ppm.ppm_param_map_init, mt, ppm.ppm_param_map_push, ppm.ppm_param_map_send):
+                        assert ppm.ppm_map_field_ghost(fv, topo, meshid, [g] * 3, step, lda=3) == 0
+                    assert ppm.ppm_map_field_ghost(fv, topo, meshid, [g] * 3, ppm.ppm_param_map_pop, lda=3,
+                                                   mask=mk) == 0
+                    ref_in = vals.astype(np.int32) if kind == L.KIND_L else vals
+                    info, ref = oracle.ghost_kind(t, ref_in, [g] * 3, kind, mt == ppm.ppm_param_map_ghost_get,
+                                                  mask.astype(np.int32) if use_mask else None)
+                    assert info == 0
+                    got = back(fv)
+                    assert np.array_equal(got.astype(np.int32) if kind == L.KIND_L else got, ref), (kind, mt, use_mask)
+        # pop without push; a field type with no specific
+        fv = wrap(cases[0][1])
+        assert ppm.ppm_map_field_ghost(fv, topo, meshid, [g] * 3, ppm.ppm_param_map_pop, lda=3) != 0
+        for step in (ppm.ppm_param_map_ghost_get, ppm.ppm_param_map_push):
+            assert ppm.ppm_map_field_ghost(fv, topo, meshid, [g] * 3, step, lda=3) == 0
+        bad = wrap(np.zeros(shape, dtype=np.int16))
+        assert ppm.ppm_map_field_ghost(bad, topo, meshid, [g] * 3, ppm.ppm_param_map_pop, lda=3) != 0
+    finally:
+        assert ppm.ppm_finalize() == 0
+
+
+def test_map_field_ghost_kind_rejects_bad_arguments(core):
+    from ppm_legacy_amd import _lib as L
+    mesh = core.Mesh(3, 9)
+    f = torch.zeros(mesh.padded_shape(1, [1] * 3), dtype=torch.int32, device="cuda")
+    with pytest.raises(L.PpmgError) as e:
+        core.map_field_ghost_kind(mesh, f, [-1] * 3, [2] * 3, L.MAP_GHOST_PUT, L.KIND_I)   # one ghost layer held
+    assert e.value.code == L.ERR_ARGUMENT
+    with pytest.raises(L.PpmgError) as e:
+        plan = core.Plan()
+        fd = core.field_desc(mesh, f, [-1] * 3)
+        g = (ctypes.c_int * 3)(1, 1, 1)
+        L.check(L.lib().ppmg_map_field_ghost_kind(mesh.ref(), 9, ctypes.byref(fd), g, L.MAP_GHOST_PUT, None,
+                                                  plan.handle, None))
+    assert e.value.code == L.ERR_ARGUMENT
+    with pytest.raises(ValueError):
+        core.map_field_ghost_kind(mesh, f, [-1] * 3, [1] * 3, L.MAP_GHOST_PUT, L.KIND_I,
+                                  torch.zeros((3, 3, 3), dtype=torch.int32, device="cuda"))
 
 
 # --------------------------------------------------------------- util_sort --

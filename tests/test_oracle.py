@@ -229,6 +229,106 @@ def test_ghost_get_put_closed_forms(oracle, dim, bc, g):
     assert np.array_equal(got, expect)
 
 
+_KIND_CASES = [(3, [1] * 6, 2), (3, [1, 1, 2, 2, 1, 1], 1), (2, [1] * 4, 2), (2, [2, 2, 1, 1], 3)]
+
+
+def _kind_topo(oracle, dim, bc, g, lda, seed):
+    nm = [7, 6, 5][:dim]
+    t = oracle.topo(dim, nm, bcdef=bc)
+    shape = tuple(reversed([n + 2 * g for n in nm])) + (lda,)
+    return t, shape, np.random.default_rng(seed)
+
+
+@pytest.mark.parametrize("dim,bc,g", _KIND_CASES)
+@pytest.mark.parametrize("get", [True, False])
+def test_ghost_kind_real_and_complex(oracle, dim, bc, g, get):
+    """The double and single kinds of the per-kind ghost map equal the
+    existing ghost_get / ghost_put of that precision (pinned by the closed
+    forms above); a complex field maps as its real and imaginary parts, each
+    on its own."""
+    t, shape, rng = _kind_topo(oracle, dim, bc, g, 2, 3 + g)
+    ref_fn = oracle.ghost_get if get else oracle.ghost_put
+    for kind, cplx, dt in ((oracle.KIND_D, oracle.KIND_DC, np.float64), (oracle.KIND_S, oracle.KIND_SC, np.float32)):
+        re = rng.normal(size=shape).astype(dt)
+        im = rng.normal(size=shape).astype(dt)
+        info, got = oracle.ghost_kind(t, re, [g] * dim, kind, get)
+        assert info == 0
+        info, ref = ref_fn(t, re, [g] * dim)
+        assert info == 0
+        assert np.array_equal(got, ref)
+        info, gc = oracle.ghost_kind(t, re + 1j * im, [g] * dim, cplx, get)
+        assert info == 0
+        assert np.array_equal(gc.real, ref_fn(t, re, [g] * dim)[1])
+        assert np.array_equal(gc.imag, ref_fn(t, im, [g] * dim)[1])
+
+
+@pytest.mark.parametrize("dim,bc,g", _KIND_CASES)
+def test_ghost_kind_integer(oracle, dim, bc, g):
+    """Integers round-trip exactly: ghost_get copies them, ghost_put adds them,
+    wrapping modulo 2^32 (values near +-2^31 overflow in the sum); the double
+    map of the same values, exact below 2^53, reduced modulo 2^32, is the
+    expected result."""
+    t, shape, rng = _kind_topo(oracle, dim, bc, g, 3, 11 + g)
+    for lo, hi in ((-1000, 1000), (2 ** 30, 2 ** 31 - 1), (-2 ** 31, -2 ** 30)):
+        f = rng.integers(lo, hi, size=shape, dtype=np.int64).astype(np.int32)
+        for get in (True, False):
+            info, got = oracle.ghost_kind(t, f, [g] * dim, oracle.KIND_I, get)
+            assert info == 0
+            info, ref = (oracle.ghost_get if get else oracle.ghost_put)(t, f.astype(np.float64), [g] * dim)
+            assert info == 0
+            wrapped = ((ref.astype(np.int64) + 2 ** 31) % 2 ** 32 - 2 ** 31).astype(np.int32)
+            assert np.array_equal(got, wrapped)
+
+
+@pytest.mark.parametrize("dim,bc,g", _KIND_CASES)
+def test_ghost_kind_logical(oracle, dim, bc, g):
+    """Logicals: ghost_get copies .TRUE. / .FALSE. (any nonzero reads as
+    .TRUE., written as 1); ghost_put leaves a real node .TRUE. exactly when it
+    and every image added onto it are .TRUE. -- with integer 0/1 data, when
+    the integer ghost_put's count equals that of an all-ones field."""
+    t, shape, rng = _kind_topo(oracle, dim, bc, g, 2, 17 + g)
+    f01 = (rng.random(size=shape) < 0.8).astype(np.int32)
+    raw = f01 * rng.integers(1, 7, size=shape, dtype=np.int32)      # nonzero values other than 1
+    info, got = oracle.ghost_kind(t, raw, [g] * dim, oracle.KIND_L, True)
+    assert info == 0
+    info, ref = oracle.ghost_kind(t, raw, [g] * dim, oracle.KIND_I, True)
+    assert info == 0
+    # images take the real node's truth value (1 / 0); nodes no block writes keep theirs
+    written = oracle.ghost_kind(t, np.zeros(shape, np.int32) + 5, [g] * dim, oracle.KIND_L, True)[1] == 1
+    assert np.array_equal(got[written], (ref[written] != 0).astype(np.int32))
+    assert np.array_equal(got[~written], raw[~written])
+    info, got = oracle.ghost_kind(t, f01, [g] * dim, oracle.KIND_L, False)
+    assert info == 0
+    cnt = oracle.ghost_kind(t, f01, [g] * dim, oracle.KIND_I, False)[1]
+    full = oracle.ghost_kind(t, np.ones(shape, np.int32), [g] * dim, oracle.KIND_I, False)[1]
+    assert np.array_equal(got, (cnt == full).astype(np.int32))
+
+
+@pytest.mark.parametrize("dim,bc,g", _KIND_CASES)
+@pytest.mark.parametrize("kind", [0, 2, 3, 4, 5, 6])
+def test_ghost_kind_mask(oracle, dim, bc, g, kind):
+    """The pop leaves nodes whose mask is .FALSE. unchanged and steps the
+    buffer past them, so every other node gets the value it gets without a
+    mask: masked result = where(mask, unmasked result, input).  A buffer that
+    did not step past the skipped nodes would shift every later value."""
+    t, shape, rng = _kind_topo(oracle, dim, bc, g, 2, 23 + g + kind)
+    if kind in (oracle.KIND_I, oracle.KIND_L):
+        f = rng.integers(0, 2 if kind == oracle.KIND_L else 100, size=shape).astype(np.int32)
+    elif kind in (oracle.KIND_DC, oracle.KIND_SC):
+        f = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    else:
+        f = rng.normal(size=shape)
+    mask = (rng.random(size=shape[:-1]) < 0.6).astype(np.int32)
+    for get in (True, False):
+        info, full = oracle.ghost_kind(t, f, [g] * dim, kind, get)
+        assert info == 0
+        info, part = oracle.ghost_kind(t, f, [g] * dim, kind, get, mask)
+        assert info == 0
+        base = np.ascontiguousarray(f, dtype=full.dtype)
+        assert np.array_equal(part, np.where(mask[..., None] != 0, full, base))
+        assert not np.array_equal(part, full)     # the mask does hide some writes
+
+
 # -------------------------------------------------------------------- M2P ----
 @pytest.mark.parametrize("kernel", [MP4, BSP2])
 @pytest.mark.parametrize("dim", [2, 3])
