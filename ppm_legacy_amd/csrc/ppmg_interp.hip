@@ -56,18 +56,25 @@ k_sorted_weights(const int* __restrict__ perm, const PosT* __restrict__ xp, cons
 }
 
 // A stencil cell of node n along axis d; false when it does not exist.
-__device__ __forceinline__ bool cell_of(const DevMesh& m, int d, bool wrap, int& c) {
+// one_period (a stored box holding the real nodes): the fold adds the images
+// of a node one period away only (ppm_map_field_ghost_init.f:236-425, one
+// block per face, edge and corner), so a cell more than one period from the
+// node does not reach it -- which happens on a mesh of one cell per axis,
+// where an M'4 stencil spans two periods.
+__device__ __forceinline__ bool cell_of(const DevMesh& m, int d, bool wrap, bool one_period, int& c) {
     if (wrap) {
-        int N = m.N[d];
+        const int N = m.N[d];
+        const int c0 = c;
         c %= N;
         if (c < 0) c += N;
-        return true;
+        return !one_period || abs(c - c0) <= N;
     }
     return c >= 0 && c < m.nbin[d];
 }
 
 // P2M gather: one thread per stored node.  Nodes real on every axis (0..N)
-// sum all periodic images (the ghost fold of ppm_map_field_ghost_put); nodes
+// sum their periodic images (the ghost fold of ppm_map_field_ghost_put: on a
+// stored box the images one period away, in the compact layout all); nodes
 // outside the real range keep the raw scatter, as in the reference.
 template <int DIM, int KS, typename ValT, int LDA>
 __global__ void __launch_bounds__(256)
@@ -94,14 +101,14 @@ k_p2m_gather(DevMesh m, FieldGeom f, ValT* __restrict__ field, const int* __rest
         int cz = 0;
         if (DIM == 3) {
             cz = n[2] + BASE - oz;
-            if (!cell_of(m, 2, real && m.periodic[2], cz)) continue;
+            if (!cell_of(m, 2, real && m.periodic[2], f.swap[2], cz)) continue;
         }
         for (int oy = 0; oy < KS; ++oy) {
             int cy = n[1] + BASE - oy;
-            if (!cell_of(m, 1, real && m.periodic[1], cy)) continue;
+            if (!cell_of(m, 1, real && m.periodic[1], f.swap[1], cy)) continue;
             for (int ox = 0; ox < KS; ++ox) {
                 int cx = n[0] + BASE - ox;
-                if (!cell_of(m, 0, real && m.periodic[0], cx)) continue;
+                if (!cell_of(m, 0, real && m.periodic[0], f.swap[0], cx)) continue;
                 int key = cx + m.nbin[0] * (cy + m.nbin[1] * cz);
                 int b = start[key], e = start[key + 1];
                 for (int s = b; s < e; ++s) {
@@ -223,18 +230,21 @@ k_m2p_gather(DevMesh m, FieldGeom f, const ValT* __restrict__ field, const PosT*
 
 // Stored indices along axis d of the real nodes that receive stencil node v
 // in the forward P2M: on a periodic axis its image r = v mod N and, for r = 0,
-// node N (both real, both given every periodic image's sum); on an open axis
-// v itself if it is a real node 0..N.  Indices outside the box are dropped.
+// node N (both real, both given every periodic image's sum; on a stored box
+// only those one period from v); on an open axis v itself if it is a real
+// node 0..N.  Indices outside the box are dropped.
 __device__ __forceinline__ int real_images(const DevMesh& m, const FieldGeom& f, int d, int v, int (&out)[2]) {
     int n = 0;
     if (m.periodic[d]) {
         const int N = m.N[d];
         int r = v % N;
         if (r < 0) r += N;
+        // a stored box folds images one period away only (cell_of)
+        const bool one = f.swap[d];
         int l = r - f.lo[d];
-        if (l >= 0 && l < f.ext[d]) out[n++] = l;
+        if (l >= 0 && l < f.ext[d] && (!one || abs(v - r) <= N)) out[n++] = l;
         l = N - f.lo[d];
-        if (r == 0 && l >= 0 && l < f.ext[d]) out[n++] = l;
+        if (r == 0 && l >= 0 && l < f.ext[d] && (!one || abs(v - N) <= N)) out[n++] = l;
     } else if (v >= 0 && v <= m.N[d]) {
         const int l = v - f.lo[d];
         if (l >= 0 && l < f.ext[d]) out[n++] = l;

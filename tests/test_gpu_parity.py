@@ -829,14 +829,16 @@ def test_tiled_crowded_cells(core, oracle, kernel, prec, spread, monkeypatch):
 
 
 @pytest.mark.parametrize("bc", [[1] * 6, [2] * 6, [1, 1, 1, 1, 2, 2]], ids=["periodic", "all_open", "open_z"])
-@pytest.mark.parametrize("N", [2, 3, 4])
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
 def test_smallest_meshes(core, oracle, bc, N, monkeypatch):
-    """Meshes of 2-4 cells per axis, where every stencil wraps onto itself
-    (ppm_topo_mkfield.f:421 allows Nm >= 2).  N = 4 is the smallest mesh the
-    tiled kernels take, N = 2 and 3 run the general paths.  P2M and M2P on
-    the reference's stored box (ghost size 2) against the oracle; on a
-    periodic mesh also the compact layout against the numpy brute force;
-    at N = 4 the tiled kernels against the general ones."""
+    """Meshes of 1-4 cells per axis, where every stencil wraps onto itself
+    (ppm_topo_mkfield.f:421 allows Nm >= 2, one cell).  N = 4 is the
+    smallest mesh the tiled kernels take, N = 1-3 run the general paths.  P2M
+    and M2P on the reference's stored box (ghost size 2) against the oracle
+    (at N = 1 the fold's one-period images leave the nodes two periods away
+    in the ghost layers, as the reference's block lists do); on a periodic
+    mesh also the compact layout against the numpy brute force; at N = 4 the
+    tiled kernels against the general ones."""
     rng = np.random.default_rng(40 + 3 * N + sum(bc))
     lda = 3
     xp = rng.uniform(0.0, 1.0, size=(600, 3))
@@ -860,6 +862,13 @@ def test_smallest_meshes(core, oracle, bc, N, monkeypatch):
     q = torch.empty((len(xp), lda), dtype=torch.float64, device="cuda")
     core.m2p(mesh, x, _dev(fr, torch.float64), [-2] * 3, q)
     assert _rel(q.cpu().numpy(), qref) < 1e-12
+    # the adjoint of the stored-box P2M: <P2M(u), g> = <u, P2M^T g> for a random cotangent g
+    gco = rng.normal(size=oracle.field_shape(t, lda, g))
+    plan = core.p2m(mesh, x, u, field, [-2] * 3)
+    gu, _ = core.p2m_adjoint(mesh, x, u, _dev(gco, torch.float64), [-2] * 3, plan, want_x=False)
+    lhs = float((field.cpu().numpy() * gco).sum())
+    rhs = float((up * gu.cpu().numpy()).sum())
+    assert abs(lhs - rhs) < 1e-11 * (abs(lhs) + 1.0)
     if bc == [1] * 6:
         comp = torch.empty(mesh.compact_shape(lda), dtype=torch.float64, device="cuda")
         core.p2m(mesh, x, u, comp, [0] * 3)

@@ -229,6 +229,45 @@ def test_ghost_get_put_closed_forms(oracle, dim, bc, g):
     assert np.array_equal(got, expect)
 
 
+@pytest.mark.parametrize("nm,bc,g", [([2, 2, 2], [1] * 6, 2), ([2, 3, 2], [1, 1, 1, 1, 2, 2], 2),
+                                      ([2, 2], [1] * 4, 3), ([7, 6, 5], [1] * 6, 3), ([3, 4], [1, 1, 2, 2], 2)])
+def test_ghost_put_one_period_images(oracle, nm, bc, g):
+    """ghost_put's images lie one period (Nm-1 nodes) away on each periodic
+    axis (ppm_map_field_ghost_init.f:236-425, one image per face, edge and
+    corner): every real node r receives, in addition to its own value, the
+    stored node r + s for each non-zero shift s in {-(Nm-1), 0, Nm-1} per
+    periodic axis (0 on the others) that stays inside the stored box.  On
+    meshes of one cell per axis (Nm = 2) a stencil reaches two periods, and
+    what lands there stays in the ghost layers; the raw scatter itself still
+    conserves the strengths."""
+    import itertools
+    dim = len(nm)
+    rng = np.random.default_rng(sum(nm) + g)
+    t = oracle.topo(dim, nm, bcdef=bc)
+    per = [bc[2 * d] == 1 for d in range(dim)]
+    shape = tuple(reversed([n + 2 * g for n in nm])) + (2,)
+    f = rng.integers(-50, 50, size=shape).astype(np.float64)
+    info, got = oracle.ghost_put(t, f, [g] * dim)
+    assert info == 0
+    expect = f.copy()
+    shifts = [s for s in itertools.product(*[(-(nm[d] - 1), 0, nm[d] - 1) if per[d] else (0,) for d in range(dim)])
+              if any(s)]
+    for r in itertools.product(*[range(1, n + 1) for n in nm]):               # real node, 1-based (x, y[, z])
+        at = tuple(r[d] - 1 + g for d in reversed(range(dim)))
+        for s in shifts:
+            q = [r[d] + s[d] for d in range(dim)]
+            if all(1 - g <= q[d] <= nm[d] + g for d in range(dim)):
+                expect[at] += f[tuple(q[d] - 1 + g for d in reversed(range(dim)))]
+    assert np.array_equal(got, expect)
+    if nm[0] == 2:
+        xp = rng.uniform(0, 1, size=(40, dim))
+        up = rng.normal(size=(40, 2))
+        for kernel in (MP4, BSP2):
+            info, raw = oracle.p2m(t, kernel, xp, up, [g] * dim, fold=False)
+            assert info == 0
+            assert np.allclose(raw.reshape(-1, 2).sum(0), up.sum(0), rtol=0, atol=1e-12)
+
+
 _KIND_CASES = [(3, [1] * 6, 2), (3, [1, 1, 2, 2, 1, 1], 1), (2, [1] * 4, 2), (2, [2, 2, 1, 1], 3)]
 
 
