@@ -101,16 +101,29 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     // A thread stages whole nodes (all LDA components, adjacent in the field):
     // the ring stores of one component then write consecutive nodes.
     constexpr int NN = (A3_PL + A3_THREADS - 1) / A3_THREADS;   // nodes per thread and plane
-    // (BOXED reads each plane straight into the ring between the barriers)
-    ValT stage[BOXED ? 1 : NN][LDA];
+    // BOXED and box_pipe: the tile's ring window lies inside the real range
+    // in x and y (0 < u < N) and a box plane fits 32-bit byte offsets; its
+    // planes inside the real range in z are then plain stored nodes (no other
+    // copy) and take the same staged fetch as the compact layout.  Other
+    // planes of a BOXED tile are read through box_cotangent between the
+    // barriers.
+    const bool box_pipe = BOXED && tx0 - 1 > 0 && tx0 + 14 < Nx && ty0 - 1 > 0 && ty0 + A3_RY - 2 < Ny &&
+                          (int64_t)fg.ext[0] * fg.ext[1] * fg.lda * (int64_t)sizeof(ValT) < ((int64_t)1 << 31);
+    auto staged = [&](int n) { return !BOXED || (box_pipe && n > 0 && n < Nz); };
+    ValT stage[NN][LDA];
     // Byte offset within a field plane of each node this thread stages (the
-    // same in every plane; a plane fits in 2^31 bytes, checked at launch).
-    unsigned soff[BOXED ? 1 : NN];
-    if constexpr (!BOXED) {
+    // same in every plane; a compact plane fits in 2^31 bytes, checked at
+    // launch).
+    unsigned soff[NN];
 #pragma unroll
-        for (int j = 0; j < NN; ++j) {
-            const int node = tid + j * A3_THREADS;
-            const int nx = node & 15, ry = node >> 4;
+    for (int j = 0; j < NN; ++j) {
+        const int node = tid + j * A3_THREADS;
+        const int nx = node & 15, ry = node >> 4;
+        if constexpr (BOXED) {
+            soff[j] = box_pipe ? (unsigned)((((int64_t)(ty0 - 1 + ry - fg.lo[1]) * fg.ext[0] + (tx0 - 1 + nx - fg.lo[0])) *
+                                                 fg.lda + c0) * (int64_t)sizeof(ValT))
+                               : 0u;
+        } else {
             int gx = (tx0 - 1 + nx) % Nx;
             if (gx < 0) gx += Nx;
             int gy = (ty0 - 1 + ry) % Ny;
@@ -119,11 +132,16 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         }
     }
     auto fetch_plane = [&](int n) {
-        int nz = n % Nz;
-        if (nz < 0) nz += Nz;
-        const char* base = (const char*)(g + (int64_t)nz * plane * lda);
+        const char* base;
+        if constexpr (BOXED) {
+            base = (const char*)(g + (int64_t)(n - fg.lo[2]) * fg.ext[0] * fg.ext[1] * fg.lda);
+        } else {
+            int nz = n % Nz;
+            if (nz < 0) nz += Nz;
+            base = (const char*)(g + (int64_t)nz * plane * lda);
+        }
 #pragma unroll
-        for (int j = 0; j < (BOXED ? 0 : NN); ++j) {
+        for (int j = 0; j < NN; ++j) {
             if (tid + j * A3_THREADS < A3_PL) {
                 const ValT* src = (const ValT*)(base + soff[j]);
 #pragma unroll
@@ -131,9 +149,24 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
             }
         }
     };
-    // BOXED: plane n of the ring read through box_cotangent straight into LDS
+    // BOXED: plane n of the ring read through box_cotangent straight into LDS.
+    // When every node of the ring plane lies inside the real range on every
+    // axis (0 < u < N: interior tiles and planes), each node holds its own
+    // cotangent and has no other stored copy: one load per node.  Otherwise
+    // (tiles on the periodic faces) every node goes through box_cotangent.
     auto fetch_store_boxed = [&](int n) {
         ValT* dst = ring + (n & 3) * (LDA * A3_PL);
+        const bool inner = tx0 - 1 > 0 && tx0 + 14 < m.N[0] && ty0 - 1 > 0 && ty0 + A3_RY - 2 < m.N[1] && n > 0 &&
+                           n < m.N[2];
+        if (inner) {
+            const ValT* src = g + (((int64_t)(n - fg.lo[2]) * fg.ext[1] + (ty0 - 1 - fg.lo[1])) * fg.ext[0] +
+                                   (tx0 - 1 - fg.lo[0])) * fg.lda + c0;
+            for (int i = tid; i < A3_PL * LDA; i += A3_THREADS) {
+                const int node = i / LDA;
+                dst[(i % LDA) * A3_PL + node] = src[((int64_t)(node >> 4) * fg.ext[0] + (node & 15)) * fg.lda + i % LDA];
+            }
+            return;
+        }
         for (int i = tid; i < A3_PL * LDA; i += A3_THREADS) {
             const int node = i / LDA;
             const int u[3] = {tx0 - 1 + (node & 15), ty0 - 1 + (node >> 4), n};
@@ -143,7 +176,7 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
     auto store_plane = [&](int n) {
         ValT* dst = ring + (n & 3) * (LDA * A3_PL);
 #pragma unroll
-        for (int j = 0; j < (BOXED ? 0 : NN); ++j) {
+        for (int j = 0; j < NN; ++j) {
             const int node = tid + j * A3_THREADS;
             if (node < A3_PL) {
 #pragma unroll
@@ -230,7 +263,7 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         const int pnext = lookup(first_n, cnt_n, t_n);   // next plane's first particle
         heads(cz + 2, first_nn, cnt_nn, t_nn);
         const bool more = cz + 1 < z1;
-        if (!BOXED && more) fetch_plane(cz + 3);
+        if (more && staged(cz + 3)) fetch_plane(cz + 3);
         auto round = [&](int p, bool has, const PosT (&x)[3], const ValT (&uv)[LDA]) {
             AccT wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, wz[4] = {0, 0, 0, 0};
             AccT dx[4] = {0, 0, 0, 0}, dy[4] = {0, 0, 0, 0}, dz[4] = {0, 0, 0, 0};
@@ -337,8 +370,8 @@ k_a3_tile(DevMesh m, const int* __restrict__ start, const int* __restrict__ perm
         particle(pn, count_of(first_n, cnt_n), xn, un);   // next plane's round 0
         __syncthreads();   // every row is done with plane cz-1: its slot takes plane cz+3
         if (more) {
-            if constexpr (BOXED) fetch_store_boxed(cz + 3);
-            else store_plane(cz + 3);
+            if (staged(cz + 3)) store_plane(cz + 3);
+            else fetch_store_boxed(cz + 3);
         }
         __syncthreads();
     }
