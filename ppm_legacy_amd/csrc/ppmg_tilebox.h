@@ -96,38 +96,79 @@ __global__ void __launch_bounds__(256) k_box_shell(DevMesh m, FieldGeom fg, int3
 #pragma unroll
     for (int c = 0; c < LL; ++c) acc[c] = 0.0;
     constexpr int B = Stencil<KS>::B;
-    constexpr int KZ = DIM == 3 ? KS : 1;
-    for (int oz = 0; oz < KZ; ++oz) {
+    // The stencil offsets o of each axis whose cell n + B - o exists, walked
+    // in the order z, y, x (x fastest) with the next cell's list head (count
+    // and first particle) fetched while the current cell is summed: the same
+    // additions in the same order as a plain loop over the KS^DIM offsets,
+    // with one cell's lookups in flight instead of none.
+    int olo[3], ohi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        olo[d] = d < DIM ? max(0, n[d] + B - (m.nbin[d] - 1)) : 0;
+        ohi[d] = d < DIM ? min(KS - 1, n[d] + B) : 0;
+    }
+    auto key_of = [&](int ox, int oy, int oz) {
         const int cz = DIM == 3 ? n[2] + B - oz : 0;
-        if (cz < 0 || cz >= m.nbin[2]) continue;
-        for (int oy = 0; oy < KS; ++oy) {
-            const int cy = n[1] + B - oy;
-            if (cy < 0 || cy >= m.nbin[1]) continue;
-            for (int ox = 0; ox < KS; ++ox) {
-                const int cx = n[0] + B - ox;
-                if (cx < 0 || cx >= m.nbin[0]) continue;
-                const int key = cx + m.nbin[0] * (cy + m.nbin[1] * cz);
-                const int b = TABLE ? 0 : start[key];
-                const int e = TABLE ? (int)((bt.count[count_word(key)] >> count_shift(key)) & 255u) : start[key + 1];
-                for (int q = b; q < e; ++q) {
-                    const int p = TABLE ? bt.table[(int64_t)q * bt.ncells + key] : perm ? perm[q] : q;
-                    ValT w;
-                    if (DIM == 3 && KS == 4 && wt.w[0]) {
-                        w = (wt.w[0][(int64_t)p * 4 + ox] * wt.w[1][(int64_t)p * 4 + oy]) * wt.w[2][(int64_t)p * 4 + oz];
-                    } else {
-                        ValT fr[DIM];
+        return (n[0] + B - ox) + m.nbin[0] * ((n[1] + B - oy) + m.nbin[1] * cz);
+    };
+    // list of cell `key`: particles q in [b, e), particle p0 at q = b (if any)
+    auto head = [&](int key, int& b, int& e, int& p0) {
+        b = TABLE ? 0 : start[key];
+        e = TABLE ? (int)((bt.count[count_word(key)] >> count_shift(key)) & 255u) : start[key + 1];
+        p0 = e > b ? (TABLE ? bt.table[key] : perm ? perm[b] : b) : 0;
+    };
+    auto particle_of = [&](int key, int q) {
+        return TABLE ? bt.table[(int64_t)q * bt.ncells + key] : perm ? perm[q] : q;
+    };
+    auto add = [&](int p, int ox, int oy, int oz) {
+        ValT w;
+        if (DIM == 3 && KS == 4 && wt.w[0]) {
+            w = (wt.w[0][(int64_t)p * 4 + ox] * wt.w[1][(int64_t)p * 4 + oy]) * wt.w[2][(int64_t)p * 4 + oz];
+        } else {
+            ValT fr[DIM];
 #pragma unroll
-                        for (int d = 0; d < DIM; ++d) {
-                            int ci;
-                            fr[d] = (ValT)cell_offset<PosT>(xp[(int64_t)p * DIM + d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
-                        }
-                        w = tile_weight_at<KS, ValT>(fr[0], ox) * tile_weight_at<KS, ValT>(fr[1], oy);
-                        if constexpr (DIM == 3) w = w * tile_weight_at<KS, ValT>(fr[2], oz);
-                    }
+            for (int d = 0; d < DIM; ++d) {
+                int ci;
+                fr[d] = (ValT)cell_offset<PosT>(xp[(int64_t)p * DIM + d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
+            }
+            w = tile_weight_at<KS, ValT>(fr[0], ox) * tile_weight_at<KS, ValT>(fr[1], oy);
+            if constexpr (DIM == 3) w = w * tile_weight_at<KS, ValT>(fr[2], oz);
+        }
 #pragma unroll
-                    for (int c = 0; c < LL; ++c) acc[c] += (double)(w * up[(int64_t)p * fg.lda + c0 + c]);
+        for (int c = 0; c < LL; ++c) acc[c] += (double)(w * up[(int64_t)p * fg.lda + c0 + c]);
+    };
+    if (olo[0] <= ohi[0] && olo[1] <= ohi[1] && olo[2] <= ohi[2]) {
+        int ox = olo[0], oy = olo[1], oz = olo[2];
+        int key = key_of(ox, oy, oz), b, e, p0;
+        head(key, b, e, p0);
+        for (;;) {
+            // the cell after this one
+            int nx = ox + 1, ny = oy, nz = oz;
+            if (nx > ohi[0]) {
+                nx = olo[0];
+                if (++ny > ohi[1]) {
+                    ny = olo[1];
+                    ++nz;
                 }
             }
+            const bool more = nz <= ohi[2];
+            int nkey = 0, nb = 0, ne = 0, np0 = 0;
+            if (more) {
+                nkey = key_of(nx, ny, nz);
+                head(nkey, nb, ne, np0);
+            }
+            if (e > b) {
+                add(p0, ox, oy, oz);
+                for (int q = b + 1; q < e; ++q) add(particle_of(key, q), ox, oy, oz);
+            }
+            if (!more) break;
+            ox = nx;
+            oy = ny;
+            oz = nz;
+            key = nkey;
+            b = nb;
+            e = ne;
+            p0 = np0;
         }
     }
     ValT* out = field + (((int64_t)(n[2] - fg.lo[2]) * e1 + (n[1] - fg.lo[1])) * e0 + (n[0] - fg.lo[0])) * fg.lda + c0;
