@@ -147,6 +147,47 @@ static int check_gpu(void) {
         if (!(worst <= 1e-13)) return fail("m2p of a constant field");
         printf("kernel %d: m2p of a constant field, max error %.3g\n", kernel, worst);
     }
+    /* ghost_get of an integer field with a mask: every stored node holds the
+     * code of the real node it images (coordinates wrapped into 1..N), ghost
+     * nodes start at -7; afterwards a ghost node holds its real node's code
+     * where the mask is set and still -7 where it is not */
+    {
+        const size_t nn = (size_t)EXT * EXT * EXT;
+        int* fi = malloc(sizeof(int) * LDA * nn);
+        int* mk = malloc(sizeof(int) * nn);
+        if (!fi || !mk) return fail("host allocation");
+#define WRAP(v) ((((v) - 1) % N + N) % N + 1)
+#define CODE(i, j, k, c) (WRAP(i) + 100 * WRAP(j) + 10000 * WRAP(k) + 1000000 * (c))
+        for (int k = 1 - G; k <= NM + G; ++k)
+            for (int j = 1 - G; j <= NM + G; ++j)
+                for (int i = 1 - G; i <= NM + G; ++i) {
+                    const int real = i >= 1 && i <= NM && j >= 1 && j <= NM && k >= 1 && k <= NM;
+                    for (int c = 0; c < LDA; ++c) fi[at(c, i, j, k, LDA, G, EXT)] = real ? CODE(i, j, k, c) : -7;
+                    mk[at(0, i, j, k, 1, G, EXT)] = (i + 2 * j + 3 * k) % 3 != 0;
+                }
+        if (ppmg_map_field_ghost_kind_reg(fi, LDA, topo, mesh, g, PPMG_MAP_GHOST_GET, PPMG_KIND_I, mk, PPMG_MEM_HOST) != 0)
+            return fail("ppmg_map_field_ghost_kind_reg");
+        long set = 0, kept = 0;
+        for (int k = 1 - G; k <= NM + G; ++k)
+            for (int j = 1 - G; j <= NM + G; ++j)
+                for (int i = 1 - G; i <= NM + G; ++i) {
+                    const int real = i >= 1 && i <= NM && j >= 1 && j <= NM && k >= 1 && k <= NM;
+                    const int on = mk[at(0, i, j, k, 1, G, EXT)];
+                    for (int c = 0; c < LDA; ++c) {
+                        const int want = real || on ? CODE(i, j, k, c) : -7;
+                        if (fi[at(c, i, j, k, LDA, G, EXT)] != want) return fail("masked integer ghost_get");
+                    }
+                    if (!real) {
+                        set += on;
+                        kept += !on;
+                    }
+                }
+#undef CODE
+#undef WRAP
+        printf("integer ghost_get with a mask: %ld ghost nodes set, %ld kept\n", set, kept);
+        free(fi);
+        free(mk);
+    }
     if (ppmg_topo_destroy(topo) != 0) return fail("ppmg_topo_destroy");
     if (ppmg_finalize() != 0) return fail("ppmg_finalize");
     free(xp);

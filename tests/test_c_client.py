@@ -2,7 +2,8 @@
 against libppmg.so by __graft_entry__.build().  Without a device it checks
 the error paths; on the GPU it runs the registry entry points on host arrays
 and checks mass conservation of the periodic fold, the duplicate face nodes,
-and M2P of a constant field (both kernels)."""
+M2P of a constant field (both kernels), and a masked ghost_get of an integer
+field against its closed form."""
 import os
 import subprocess
 
@@ -39,4 +40,5 @@ def test_c_client_registry_on_host_arrays():
     r = _run("gpu")
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("duplicate face nodes equal node 1") == 2
+    assert "integer ghost_get with a mask" in r.stdout
     assert r.stdout.strip().endswith("ok")
