@@ -48,8 +48,11 @@ __device__ __forceinline__ double box_cotangent(const DevMesh& m, const FieldGeo
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const int N = m.N[d], hi = fg.lo[d] + fg.ext[d] - 1;
-        int w = u[d] % N;
-        if (w < 0) w += N;
+        // u lies a few periods at most from 0..N-1 (one for N >= 16): wrap
+        // by steps rather than an integer division
+        int w = u[d];
+        while (w < 0) w += N;
+        while (w >= N) w -= N;
         s0[d] = w;
         s1[d] = w == 0 && hi >= N ? N : -1;
         if (u[d] < 0 || u[d] > N) real = false;
