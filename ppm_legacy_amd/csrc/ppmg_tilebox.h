@@ -120,24 +120,96 @@ __global__ void __launch_bounds__(256) k_box_shell(DevMesh m, FieldGeom fg, int3
     auto particle_of = [&](int key, int q) {
         return TABLE ? bt.table[(int64_t)q * bt.ncells + key] : perm ? perm[q] : q;
     };
-    auto add = [&](int p, int ox, int oy, int oz) {
-        ValT w;
-        if (DIM == 3 && KS == 4 && wt.w[0]) {
-            w = (wt.w[0][(int64_t)p * 4 + ox] * wt.w[1][(int64_t)p * 4 + oy]) * wt.w[2][(int64_t)p * 4 + oz];
-        } else {
-            ValT fr[DIM];
+    // one particle's contribution from its position and strengths
+    auto add_vals = [&](const PosT (&x)[DIM], const ValT (&u)[LL], int ox, int oy, int oz) {
+        ValT fr[DIM];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                int ci;
-                fr[d] = (ValT)cell_offset<PosT>(xp[(int64_t)p * DIM + d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
-            }
-            w = tile_weight_at<KS, ValT>(fr[0], ox) * tile_weight_at<KS, ValT>(fr[1], oy);
-            if constexpr (DIM == 3) w = w * tile_weight_at<KS, ValT>(fr[2], oz);
+        for (int d = 0; d < DIM; ++d) {
+            int ci;
+            fr[d] = (ValT)cell_offset<PosT>(x[d], (PosT)m.minv[d], (PosT)m.dxi[d], ci);
+        }
+        ValT w = tile_weight_at<KS, ValT>(fr[0], ox) * tile_weight_at<KS, ValT>(fr[1], oy);
+        if constexpr (DIM == 3) w = w * tile_weight_at<KS, ValT>(fr[2], oz);
+#pragma unroll
+        for (int c = 0; c < LL; ++c) acc[c] += (double)(w * u[c]);
+    };
+    auto add = [&](int p, int ox, int oy, int oz) {
+        if (DIM == 3 && KS == 4 && wt.w[0]) {
+            const ValT w =
+                (wt.w[0][(int64_t)p * 4 + ox] * wt.w[1][(int64_t)p * 4 + oy]) * wt.w[2][(int64_t)p * 4 + oz];
+#pragma unroll
+            for (int c = 0; c < LL; ++c) acc[c] += (double)(w * up[(int64_t)p * fg.lda + c0 + c]);
+            return;
+        }
+        PosT x[DIM];
+        ValT u[LL];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) x[d] = xp[(int64_t)p * DIM + d];
+#pragma unroll
+        for (int c = 0; c < LL; ++c) u[c] = up[(int64_t)p * fg.lda + c0 + c];
+        add_vals(x, u, ox, oy, oz);
+    };
+    // A node outside the real range on an axis f of a periodic 3D M'4 box
+    // meets one stencil offset there (node -1: cell 0, node N+1: cell N-1),
+    // so its cells are at most 4 x 4 along the other two axes: they are taken
+    // as 16 fixed slots (outer free axis, then inner), all 16 list heads
+    // loaded at once and the first particles of 4 slots at a time, then
+    // summed in the same order as the walk below.
+    int f = -1;
+    if constexpr (DIM == 3 && KS == 4) {
+        if (!wt.w[0] && olo[0] <= ohi[0] && olo[1] <= ohi[1] && olo[2] <= ohi[2]) {
+#pragma unroll
+            for (int d = 2; d >= 0; --d)
+                if (olo[d] == ohi[d]) f = d;
+        }
+    }
+    if (f >= 0) {
+        const int A = f == 2 ? 1 : 2, Bx = f == 0 ? 1 : 0;   // outer and inner free axes
+        auto offs = [&](int sl, int& ox, int& oy, int& oz) {
+            int o[3];
+            o[f] = olo[f];
+            o[A] = olo[A] + (sl >> 2);
+            o[Bx] = olo[Bx] + (sl & 3);
+            ox = o[0];
+            oy = o[1];
+            oz = o[2];
+            return o[A] <= ohi[A] && o[Bx] <= ohi[Bx];
+        };
+        int cb[16], ce[16], cp[16];
+#pragma unroll
+        for (int sl = 0; sl < 16; ++sl) {
+            int ox, oy, oz;
+            cb[sl] = ce[sl] = cp[sl] = 0;
+            if (offs(sl, ox, oy, oz)) head(key_of(ox, oy, oz), cb[sl], ce[sl], cp[sl]);
         }
 #pragma unroll
-        for (int c = 0; c < LL; ++c) acc[c] += (double)(w * up[(int64_t)p * fg.lda + c0 + c]);
-    };
-    if (olo[0] <= ohi[0] && olo[1] <= ohi[1] && olo[2] <= ohi[2]) {
+        for (int s0 = 0; s0 < 16; s0 += 4) {
+            // one batch's particle loads in flight at a time (register bound)
+            asm volatile("" ::: "memory");
+            PosT xs[4][DIM];
+            ValT us[4][LL];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (ce[s0 + j] > cb[s0 + j]) {
+                    const int p = cp[s0 + j];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) xs[j][d] = xp[(int64_t)p * DIM + d];
+#pragma unroll
+                    for (int c = 0; c < LL; ++c) us[j][c] = up[(int64_t)p * fg.lda + c0 + c];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int sl = s0 + j;
+                if (ce[sl] > cb[sl]) {
+                    int ox, oy, oz;
+                    offs(sl, ox, oy, oz);
+                    add_vals(xs[j], us[j], ox, oy, oz);
+                    for (int q = cb[sl] + 1; q < ce[sl]; ++q) add(particle_of(key_of(ox, oy, oz), q), ox, oy, oz);
+                }
+            }
+        }
+    } else if (olo[0] <= ohi[0] && olo[1] <= ohi[1] && olo[2] <= ohi[2]) {
         int ox = olo[0], oy = olo[1], oz = olo[2];
         int key = key_of(ox, oy, oz), b, e, p0;
         head(key, b, e, p0);
