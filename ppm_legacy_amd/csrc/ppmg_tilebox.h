@@ -153,11 +153,12 @@ __global__ void __launch_bounds__(256) k_box_shell(DevMesh m, FieldGeom fg, int3
     // meets one stencil offset there (node -1: cell 0, node N+1: cell N-1),
     // so its cells are at most 4 x 4 along the other two axes: they are taken
     // as 16 fixed slots (outer free axis, then inner), all 16 list heads
-    // loaded at once and the first particles of 4 slots at a time, then
-    // summed in the same order as the walk below.
+    // loaded at once and the first particles (position or table weights, and
+    // strengths) of 4 slots at a time, then summed in the same order as the
+    // walk below.
     int f = -1;
     if constexpr (DIM == 3 && KS == 4) {
-        if (!wt.w[0] && olo[0] <= ohi[0] && olo[1] <= ohi[1] && olo[2] <= ohi[2]) {
+        if (olo[0] <= ohi[0] && olo[1] <= ohi[1] && olo[2] <= ohi[2]) {
 #pragma unroll
             for (int d = 2; d >= 0; --d)
                 if (olo[d] == ohi[d]) f = d;
@@ -186,14 +187,27 @@ __global__ void __launch_bounds__(256) k_box_shell(DevMesh m, FieldGeom fg, int3
         for (int s0 = 0; s0 < 16; s0 += 4) {
             // one batch's particle loads in flight at a time (register bound)
             asm volatile("" ::: "memory");
+            // per slot: the position, or with weight tables the particle's
+            // three table weights at the slot's offsets (held in the same
+            // registers: a weight converts to the position type and back
+            // exactly)
             PosT xs[4][DIM];
             ValT us[4][LL];
+            const bool tab = wt.w[0] != nullptr;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (ce[s0 + j] > cb[s0 + j]) {
                     const int p = cp[s0 + j];
+                    if (tab) {
+                        int ox, oy, oz;
+                        offs(s0 + j, ox, oy, oz);
+                        xs[j][0] = (PosT)wt.w[0][(int64_t)p * 4 + ox];
+                        xs[j][1] = (PosT)wt.w[1][(int64_t)p * 4 + oy];
+                        xs[j][2] = (PosT)wt.w[2][(int64_t)p * 4 + oz];
+                    } else {
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) xs[j][d] = xp[(int64_t)p * DIM + d];
+                        for (int d = 0; d < DIM; ++d) xs[j][d] = xp[(int64_t)p * DIM + d];
+                    }
 #pragma unroll
                     for (int c = 0; c < LL; ++c) us[j][c] = up[(int64_t)p * fg.lda + c0 + c];
                 }
@@ -204,7 +218,13 @@ __global__ void __launch_bounds__(256) k_box_shell(DevMesh m, FieldGeom fg, int3
                 if (ce[sl] > cb[sl]) {
                     int ox, oy, oz;
                     offs(sl, ox, oy, oz);
-                    add_vals(xs[j], us[j], ox, oy, oz);
+                    if (tab) {
+                        const ValT w = ((ValT)xs[j][0] * (ValT)xs[j][1]) * (ValT)xs[j][2];
+#pragma unroll
+                        for (int c = 0; c < LL; ++c) acc[c] += (double)(w * us[j][c]);
+                    } else {
+                        add_vals(xs[j], us[j], ox, oy, oz);
+                    }
                     for (int q = cb[sl] + 1; q < ce[sl]; ++q) add(particle_of(key_of(ox, oy, oz), q), ox, oy, oz);
                 }
             }
